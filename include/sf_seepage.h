@@ -1,0 +1,229 @@
+/*
+ * sf_seepage.h -- C-ABI of the MI355X (gfx950) coupled SPH-DEM seepage step.
+ *
+ * One handle = one scene on one GPU: particle state (SoA, sorted by grid cell
+ * every step), boundary particles, uniform-grid searcher, solver passes and the
+ * integrator.  Every entry point returns 0 on success and a negative SF_E* code
+ * on failure; sf_last_error() then describes the failure (thread-local).
+ * Nothing here throws, allocates torch objects or needs a Python interpreter.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   sf_params            <- CudaSeepageflowParams     seepage_flows_cuda/include/kiri_pbs_cuda/data/cuda_seepageflow_params.h:25-69
+ *   sf_boundary_params   <- CudaBoundaryParams        .../data/cuda_boundary_params.h:19-27
+ *   sf_solver_options    <- CudaWCSphSFSolver ctor    .../solver/seepageflow/cuda_wcsph_sf_solver.cuh:22-32
+ *   sf_create            <- CudaSFParticles ctor + CudaGNSearcher ctor + CudaWCSphSFSolver ctor + CudaSFSystem ctor
+ *                           (cuda_sf_particles.cuh:201-235, cuda_neighbor_searcher.cu:19-34, cuda_sf_system.cu:17-64)
+ *   sf_upload_particles  <- CudaSFParticles(maxNum, pos, col, label, mass, radius, cda0asat, amcamcp) cuda_sf_particles.cuh:201-235
+ *   sf_upload_boundary   <- CudaBoundaryParticles(pos, label) cuda_boundary_particles.cuh:21-28
+ *                           + CudaBaseSystem ctor boundary sort (cuda_base_system.cu:26)
+ *                           + ComputeBoundaryVolume (cuda_sf_system.cu:121-136)
+ *   sf_build_neighbors   <- CudaGNBaseSearcher::BuildGNSearcher  cuda_neighbor_searcher.cu:36-51
+ *   sf_update_system     <- CudaBaseSystem::UpdateSystem  cuda_base_system.cu:29-56 (-> CudaSFSystem::OnUpdateSolver cuda_sf_system.cu:66-119)
+ *   sf_step              <- the same per-step pass sequence with a caller-fixed dt (cuda_wcsph_sf_solver.cpp:44-177)
+ *   sf_num_sub_steps     <- CudaBaseSystem::GetNumOfSubTimeSteps  cuda_base_system.cuh:38
+ *   sf_download / sf_device_buffer <- CudaSFParticles::Get*Ptr()  cuda_sf_particles.cuh:170-190,
+ *                           CudaGNBaseSearcher::GetCellStart()  cuda_neighbor_searcher.cuh:44-50
+ *   sf_export_bgeo       <- ExportBgeoFileCUDA  seepage_flows/include/kiri_utils.h:364-448
+ *   sf_write_bgeo        <- Partio::writeBGEO (uncompressed V5)  extlib/include/partio/io/BGEO.cpp:311-476
+ *   sf_build_world_boundary <- CudaBoundaryEmitter::BuildWorldBoundary  cuda_boundary_emitter.cpp:12-80
+ *   sf_grid_size         <- make_int3((highest - lowest) / kernel_radius)  main.cpp:104, cuda_neighbor_searcher.cu:26
+ *
+ * Threading: a handle is single-threaded; all its GPU work is ordered on the
+ * handle's own HIP stream (sf_set_stream can replace it).
+ */
+#ifndef SF_SEEPAGE_H
+#define SF_SEEPAGE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SF_ABI_VERSION 1
+
+/* ---- error codes ------------------------------------------------------- */
+#define SF_OK 0
+#define SF_E_INVALID (-1)   /* bad argument / parameter                     */
+#define SF_E_CAPACITY (-2)  /* particle count above the handle's capacity   */
+#define SF_E_HIP (-3)       /* a HIP runtime call failed                    */
+#define SF_E_STATE (-4)     /* call out of order (e.g. step before upload)  */
+#define SF_E_IO (-5)        /* file could not be written                    */
+#define SF_E_NOTFOUND (-6)  /* unknown field id                             */
+
+/* ---- plain vector types (layout of CUDA float3 / int3) ----------------- */
+typedef struct sf_float3 { float x, y, z; } sf_float3;
+typedef struct sf_int3 { int32_t x, y, z; } sf_int3;
+
+/* CudaSphSolverType (cuda_sph_params.h:19-24) */
+#define SF_SPH_SOLVER 0
+#define SF_WCSPH_SOLVER 1
+/* CudaSeepageflowType (cuda_seepageflow_params.h:19-23) */
+#define SF_TYPE_SF 0
+#define SF_TYPE_MULTI_SF 1
+
+/* Mirrors CudaSeepageflowParams field for field (cuda_seepageflow_params.h:25-69). */
+typedef struct sf_params {
+    /* water (SPH) */
+    float sph_mass;
+    float sph_density;
+    float sph_particle_radius;
+    float sph_kernel_radius;
+    float sph_stiff;
+    float sph_visc;
+    float sph_nu;
+    float sph_bnu;
+    /* sand (DEM) */
+    float dem_mass;
+    float dem_density;
+    float dem_particle_radius;
+    float dem_kernel_radius;
+    float dem_young;
+    float dem_poisson;
+    float dem_tan_friction_angle;
+    float dem_damping;
+    /* seepage flow */
+    float sf_c0;
+    float sf_csat;
+    float sf_cmc;
+    float sf_cmc_p;
+    float sf_cd;
+    float sf_a0;
+    float sf_asat;
+    float sf_amc;
+    float sf_amc_p;
+    sf_float3 sf_dry_sand_color;
+    sf_float3 sf_wet_sand_color;
+    sf_float3 gravity;
+    float dt;
+    int32_t solver_type; /* SF_SPH_SOLVER | SF_WCSPH_SOLVER */
+    int32_t sf_type;     /* SF_TYPE_SF | SF_TYPE_MULTI_SF  */
+} sf_params;
+
+/* Mirrors CudaBoundaryParams (cuda_boundary_params.h:19-27). grid_size must equal
+ * sf_grid_size(lowest_point, highest_point, kernel_radius). */
+typedef struct sf_boundary_params {
+    float kernel_radius;
+    sf_float3 lowest_point;
+    sf_float3 highest_point;
+    sf_float3 world_size;
+    sf_float3 world_center;
+    sf_int3 grid_size;
+} sf_boundary_params;
+
+/* Mirrors the CudaWCSphSFSolver constructor (cuda_wcsph_sf_solver.cuh:22-32):
+ * negativeScale = 0, timeStepLimitScale = 3, speedOfSound = 100. */
+typedef struct sf_solver_options {
+    float negative_scale;
+    float time_step_limit_scale;
+    float speed_of_sound;
+} sf_solver_options;
+
+/* ---- field ids for sf_download / sf_device_buffer ----------------------
+ * float3 fields download as packed float[3*n]; labels as uint64 (the
+ * reference's size_t); keys, permutation and cell starts as uint32.       */
+enum sf_field {
+    SF_FIELD_POS = 0,            /* float3  particle position (current order)      */
+    SF_FIELD_VEL = 1,            /* float3                                          */
+    SF_FIELD_ACC = 2,            /* float3  acceleration of the last step           */
+    SF_FIELD_COL = 3,            /* float3  colour                                  */
+    SF_FIELD_LABEL = 4,          /* uint64  0 = water, 1 = sand                      */
+    SF_FIELD_MASS = 5,           /* float                                           */
+    SF_FIELD_RADIUS = 6,         /* float                                           */
+    SF_FIELD_DENSITY = 7,        /* float                                           */
+    SF_FIELD_PRESSURE = 8,       /* float                                           */
+    SF_FIELD_VOIDAGE = 9,        /* float                                           */
+    SF_FIELD_SATURATION = 10,    /* float                                           */
+    SF_FIELD_MAX_SATURATION = 11,/* float                                           */
+    SF_FIELD_AVG_FLOW_VEL = 12,  /* float3                                          */
+    SF_FIELD_AVG_DRAG_FORCE = 13,/* float3  sand entries; water entries are 0       */
+    SF_FIELD_ADHESION_FORCE = 14,/* float3  water entries; sand entries are 0       */
+    SF_FIELD_AVG_ADHESION_FORCE = 15, /* float3 water entries; sand entries are 0   */
+    SF_FIELD_CDA0ASAT = 16,      /* float3  per-particle (cd, a0, asat)             */
+    SF_FIELD_AMCAMCP = 17,       /* float2  per-particle (amc, amcp)                */
+    SF_FIELD_PARTICLE2CELL = 18, /* uint32  cell key of each particle before the last sort */
+    SF_FIELD_SORT_PERM = 19,     /* uint32  perm[new] = old index of the last sort  */
+    SF_FIELD_SORTED_KEYS = 20,   /* uint32  keys after the last sort                */
+    SF_FIELD_CELL_START = 21,    /* uint32  numCells + 1 entries; [k+1] is cellEnd[k] */
+    SF_FIELD_NBR_COUNT = 22,     /* uint32  per particle: 27-cell candidates with r^2 < h^2 (self included) */
+    SF_FIELD_NBR_CANDIDATES = 23,/* uint32  per particle: 27-cell candidate count   */
+    SF_FIELD_B_POS = 24,         /* float3  boundary particles, sorted by cell      */
+    SF_FIELD_B_LABEL = 25,       /* uint64                                          */
+    SF_FIELD_B_VOLUME = 26,      /* float                                           */
+    SF_FIELD_B_CELL_START = 27,  /* uint32  numCells + 1 entries                    */
+    SF_FIELD_COUNT_ = 28
+};
+
+typedef struct sf_system* sf_handle;
+
+/* ---- host-only helpers (no GPU needed) --------------------------------- */
+int sf_abi_version(void);
+const char* sf_last_error(void);
+/* Validates parameter structs exactly as sf_create does, without a GPU. */
+int sf_validate_params(const sf_params* p, const sf_boundary_params* b);
+/* make_int3((hi - lo) / h), FP32, truncation (main.cpp:104). */
+int sf_grid_size(sf_float3 lowest, sf_float3 highest, float kernel_radius, sf_int3* out);
+/* Six wall planes of boundary particles at spacing 2r, label 0, in the
+ * reference's plane order. Call with pos3 == NULL to get the count in *n_out;
+ * otherwise `capacity` is the number of particles pos3/label can hold. */
+int sf_build_world_boundary(sf_float3 lowest, sf_float3 highest, float particle_radius,
+                            size_t capacity, float* pos3, uint64_t* label, size_t* n_out);
+/* Writes an uncompressed Houdini bgeo V5 file with position, Cd, pscale, label. */
+int sf_write_bgeo(const char* path, size_t n, const float* pos3, const float* col3,
+                  const float* radius, const uint64_t* label);
+
+/* ---- scene lifetime ----------------------------------------------------- */
+/* opts may be NULL (reference defaults). device < 0 keeps the current device. */
+int sf_create(const sf_params* p, const sf_boundary_params* b, const sf_solver_options* opts,
+              size_t max_num, int device, sf_handle* out);
+int sf_destroy(sf_handle h);
+/* Replace the parameter structs used by the next steps (the reference passes
+ * them by value to every UpdateSolver call). grid/box must not change. */
+int sf_set_params(sf_handle h, const sf_params* p);
+/* hipStream_t as void*; NULL restores the handle's own stream. */
+int sf_set_stream(sf_handle h, void* hip_stream);
+int sf_synchronize(sf_handle h);
+
+/* Host arrays; vel3, col3, cda0asat3, amcamcp2 may be NULL (zero / default).
+ * Labels must be 0 (water) or 1 (sand); masses must be > 0. */
+int sf_upload_particles(sf_handle h, size_t n, const float* pos3, const float* vel3,
+                        const float* col3, const uint64_t* label, const float* mass,
+                        const float* radius, const float* cda0asat3, const float* amcamcp2);
+/* Sorts boundary particles by cell and computes their volumes. */
+int sf_upload_boundary(sf_handle h, size_t nb, const float* pos3, const uint64_t* label);
+
+/* ---- stepping ----------------------------------------------------------- */
+int sf_build_neighbors(sf_handle h);
+int sf_step(sf_handle h, float dt);
+int sf_step_n(sf_handle h, float dt, int n_steps);
+int sf_update_system(sf_handle h, float render_interval, float* ms_out);
+int sf_num_sub_steps(sf_handle h, size_t* out);
+/* Passes of one step, in launch order, for sf_step_profiled. */
+enum sf_pass {
+    SF_PASS_HASH = 0, SF_PASS_SORT = 1, SF_PASS_CELL_START = 2, SF_PASS_GATHER = 3,
+    SF_PASS_DENSITY = 4, SF_PASS_AVG_FLOW = 5, SF_PASS_VOIDAGE_ADHESION = 6,
+    SF_PASS_SAND_MOMENTUM = 7, SF_PASS_WATER_MOMENTUM = 8, SF_PASS_ADVECT = 9,
+    SF_PASS_COUNT = 10
+};
+/* n_steps fixed-dt steps with HIP events recorded on the handle's stream
+ * around every pass (no host synchronisation inside the region).  pass_ms
+ * (SF_PASS_COUNT floats, may be NULL) receives the mean duration of each pass
+ * per step; total_ms the elapsed time of the whole region.  No counterpart in
+ * the reference: it replaces the cudaEvent timing of UpdateSystem
+ * (cuda_base_system.cu:31-55) with per-pass resolution. */
+int sf_step_profiled(sf_handle h, float dt, int n_steps, float* pass_ms, float* total_ms);
+int sf_size(sf_handle h, size_t* n, size_t* nb, size_t* num_cells);
+
+/* ---- data access -------------------------------------------------------- */
+int sf_download(sf_handle h, int field, void* dst, size_t dst_bytes);
+/* Raw device pointer and element stride in bytes of a field's storage
+ * (float3 fields are stored as float4, stride 16). Valid until the next step. */
+int sf_device_buffer(sf_handle h, int field, void** dptr, size_t* stride_bytes);
+/* Writes folder/name.bgeo from the current state (creates folder). */
+int sf_export_bgeo(sf_handle h, const char* folder, const char* name);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SF_SEEPAGE_H */

@@ -1,0 +1,751 @@
+// sf_api.cpp -- the C-ABI entry points that own device memory and order the
+// passes of one step on the handle's HIP stream.
+//
+// Per step (sf_step):  hash -> radix sort -> cellStart -> gather (state is
+// physically reordered, as CudaGNSearcher::SortData does) -> P1..P5 -> advect.
+// sf_update_system adds the device-side CFL sub-step count of the WCSPH solver.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sf_internal.h"
+
+namespace sf {
+
+int hip_error(hipError_t e, const char* what) {
+    return set_error(SF_E_HIP, std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+int validate(const sf_params* p, const sf_boundary_params* b);
+int make_dirs(const std::string& path);
+
+// Constants exactly as the reference's functor constructors form them in FP32.
+static SFConst make_const(const sf_params& P, const sf_boundary_params& B, const sf_solver_options& O, int blabel1) {
+    const float PI = 3.141592653589793238462643383279502884197f;  // KIRI_PI
+    SFConst k{};
+    k.lox = B.lowest_point.x;
+    k.loy = B.lowest_point.y;
+    k.loz = B.lowest_point.z;
+    k.hix = B.highest_point.x;
+    k.hiy = B.highest_point.y;
+    k.hiz = B.highest_point.z;
+    k.h = B.kernel_radius;
+    k.gx = B.grid_size.x;
+    k.gy = B.grid_size.y;
+    k.gz = B.grid_size.z;
+    k.C = (uint32_t)((size_t)k.gx * k.gy * k.gz);
+    const float h = k.h;
+    k.h2 = h * h;
+    {  // Poly6Kernel (cuda_sph_kernel.cuh:82-89)
+        float ih = 1.f / h, ih3 = ih * ih * ih, ih9 = ih3 * ih3 * ih3;
+        k.w_coef = 315.f * ih9 / (64.f * PI);
+        k.w0 = k.w_coef * k.h2 * k.h2 * k.h2;
+    }
+    {  // SpikyKernelGrad (:103-108)
+        float h2 = h * h, h6 = h2 * h2 * h2;
+        k.g_coef = -45.f / (PI * h6);
+    }
+    k.adh_coef = 0.007f / powf(h, 3.25f);  // AdhesionAkinci13 (:192-195)
+    k.rho0 = P.sph_density;
+    k.stiff = P.sph_stiff;
+    k.tait = P.solver_type == SF_WCSPH_SOLVER ? 1 : 0;
+    k.neg_scale = k.tait ? O.negative_scale : 1.f;
+    k.nu = P.sph_nu;
+    k.bnu = P.sph_bnu;
+    k.rw = P.sph_particle_radius;
+    k.rho_s = P.dem_density;
+    k.R = P.dem_particle_radius;
+    k.young = P.dem_young;
+    k.poisson = P.dem_poisson;
+    k.kn = P.dem_young * k.R;
+    k.ks = k.kn * P.dem_poisson;
+    k.tanphi = P.dem_tan_friction_angle;
+    k.tan2 = powf(k.tanphi, 2.f);
+    k.damping = P.dem_damping;
+    k.cd = P.sf_cd;
+    {  // liquid bridge constants (cuda_dem_solver_common_gpu.cuh:63-65)
+        float theta = 30.f / 180.f * PI;
+        float vb = 4.f / 3.f * PI * powf(k.R, 3.f) * 0.01f * 0.01f;
+        k.cap_vb = vb;
+        k.cap_srup = (1.f + 0.5f * theta) * (powf(vb, 1.f / 3.f) + 0.1f * powf(vb, 2.f / 3.f));
+        k.cap_vb_piR = vb / (PI * k.R);
+        k.cap_cos = cosf(theta);
+        k.cap_sin = sinf(theta);
+        k.cap_pi = PI;
+    }
+    auto bez = [](float py0, float py1, float px1, float py2) {  // QuadraticBezierCoeff ctor (cuda_sf_utils.cuh:42-57)
+        Bez g;
+        g.py0 = py0;
+        g.py1 = py1;
+        g.px1 = px1;
+        g.py2 = py2;
+        float a = (px1 + 1.f) / 4.f;
+        g.b = -2.f * a;
+        g.c = g.b * g.b;
+        g.d = -4.f * (1.f + g.b - px1);
+        g.e = 2.f * (1.f + g.b - px1);
+        return g;
+    };
+    k.gcap = bez(P.sf_c0, P.sf_cmc, P.sf_cmc_p, P.sf_csat);  // cuda_sph_sf_solver.cu:249,308
+    k.multi = P.sf_type == SF_TYPE_MULTI_SF ? 1 : 0;
+    k.gadh = k.multi ? bez(0.f, 1.5f, 0.5f, 0.8f)                             // cuda_sph_sf_solver_gpu.cuh:667
+                     : bez(P.sf_a0, P.sf_amc, P.sf_amc_p, P.sf_asat);         // cuda_sph_sf_solver.cu:119
+    k.gvx = P.gravity.x;
+    k.gvy = P.gravity.y;
+    k.gvz = P.gravity.z;
+    k.grav_abs = fabsf(P.gravity.y);  // abs(params.gravity.y) (cuda_wcsph_sf_solver.cpp:106)
+    k.dryr = P.sf_dry_sand_color.x;
+    k.dryg = P.sf_dry_sand_color.y;
+    k.dryb = P.sf_dry_sand_color.z;
+    k.wetr = P.sf_wet_sand_color.x;
+    k.wetg = P.sf_wet_sand_color.y;
+    k.wetb = P.sf_wet_sand_color.z;
+    k.blabel1 = blabel1;
+    return k;
+}
+
+}  // namespace sf
+
+using namespace sf;
+
+struct sf_system {
+    sf_params P{};
+    sf_boundary_params B{};
+    sf_solver_options O{};
+    SFConst K{};
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    size_t cap = 0, n = 0, nb = 0;
+    uint32_t C = 0;
+    RadixPlan plan;
+    int blabel1 = 0;
+    // persistent state, ping-pong
+    DevBuf<float4> pm[2], vr[2], cs[2], cda[2];
+    DevBuf<float2> amc[2];
+    int cur = 0;      // pm / vr
+    int cur_cs = 0;   // cs
+    int cur_aux = 0;  // cda / amc
+    // searcher
+    DevBuf<uint32_t> keys, kA, kB, vA, vB, hist, cell, bcell;
+    size_t sort_cap = 0;
+    const uint32_t* sorted_keys = nullptr;
+    const uint32_t* perm = nullptr;
+    // transient fields
+    DevBuf<float2> rp, es;
+    DevBuf<float4> flow, adh, xf, acc;
+    // boundary
+    DevBuf<float4> bpv;
+    // CFL
+    DevBuf<float> cfl_part;
+    DevBuf<CflOut> cfl_out;
+    size_t nsub = 1;
+    // diagnostics
+    DevBuf<uint32_t> nbr_cnt, nbr_cand;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> ev_pool;  // sf_step_profiled
+    bool uploaded = false;
+    bool built = false;
+
+    Fields fields() {
+        Fields f;
+        f.pm = pm[cur].p;
+        f.vr = vr[cur].p;
+        f.cs = cs[cur_cs].p;
+        f.cda = cda[cur_aux].p;
+        f.cell = cell.p;
+        f.bpv = bpv.p;
+        f.bcell = bcell.p;
+        f.rp = rp.p;
+        f.es = es.p;
+        f.flow = flow.p;
+        f.adh = adh.p;
+        f.xf = xf.p;
+        f.acc = acc.p;
+        return f;
+    }
+};
+
+namespace {
+
+int ensure_sort_capacity(sf_system* h, size_t m) {
+    if (m <= h->sort_cap && h->kA.p) return SF_OK;
+    size_t c = std::max<size_t>(m, 1);
+    int rc;
+    if ((rc = h->keys.alloc(c)) || (rc = h->kA.alloc(c)) || (rc = h->kB.alloc(c)) || (rc = h->vA.alloc(c)) ||
+        (rc = h->vB.alloc(c)))
+        return rc;
+    size_t nblocks = (c + kRsTile - 1) / kRsTile;
+    if ((rc = h->hist.alloc(256 * nblocks))) return rc;
+    h->sort_cap = c;
+    return SF_OK;
+}
+
+template <class T>
+int zero(sf_system* h, DevBuf<T>& b) {
+    SF_HIP(hipMemsetAsync(b.p, 0, b.n * sizeof(T), h->stream));
+    return SF_OK;
+}
+
+// ev (optional): events recorded before each pass of the step
+#define SF_MARK(ev, k) \
+    if (ev) SF_HIP(hipEventRecord((ev)[k], s))
+
+int build_neighbors(sf_system* h, hipEvent_t* ev = nullptr) {
+    const int n = (int)h->n;
+    hipStream_t s = h->stream;
+    SF_MARK(ev, SF_PASS_HASH);
+    launch_hash(h->K, h->pm[h->cur].p, h->keys.p, n, s);
+    SortBuffers sb{h->kA.p, h->vA.p, h->kB.p, h->vB.p, h->hist.p};
+    SF_MARK(ev, SF_PASS_SORT);
+    int rc = radix_sort(h->keys.p, n, h->plan, sb, &h->sorted_keys, &h->perm, s);
+    if (rc) return rc;
+    SF_MARK(ev, SF_PASS_CELL_START);
+    launch_cell_start(h->sorted_keys, n, h->C, h->cell.p, s);
+    GatherArgs g;
+    g.perm = h->perm;
+    g.pm_in = h->pm[h->cur].p;
+    g.vr_in = h->vr[h->cur].p;
+    g.cs_in = h->cs[h->cur_cs].p;
+    g.cda_in = h->cda[h->cur_aux].p;
+    g.amc_in = h->amc[h->cur_aux].p;
+    g.pm_out = h->pm[h->cur ^ 1].p;
+    g.vr_out = h->vr[h->cur ^ 1].p;
+    g.cs_out = h->cs[h->cur_cs ^ 1].p;
+    g.cda_out = h->cda[h->cur_aux ^ 1].p;
+    g.amc_out = h->amc[h->cur_aux ^ 1].p;
+    g.multi = h->K.multi;
+    SF_MARK(ev, SF_PASS_GATHER);
+    launch_gather(g, n, s);
+    h->cur ^= 1;
+    h->cur_cs ^= 1;
+    if (h->K.multi) h->cur_aux ^= 1;
+    h->built = true;
+    SF_HIP(hipGetLastError());
+    return SF_OK;
+}
+
+int solve(sf_system* h, hipEvent_t* ev = nullptr) {
+    const int n = (int)h->n;
+    hipStream_t s = h->stream;
+    Fields f = h->fields();
+    SF_MARK(ev, SF_PASS_DENSITY);
+    launch_density(h->K, f, n, s);
+    SF_MARK(ev, SF_PASS_AVG_FLOW);
+    launch_avg_flow(h->K, f, n, s);
+    SF_MARK(ev, SF_PASS_VOIDAGE_ADHESION);
+    launch_voidage_adhesion(h->K, f, n, s);
+    SF_MARK(ev, SF_PASS_SAND_MOMENTUM);
+    launch_sand_momentum(h->K, f, n, s);
+    SF_MARK(ev, SF_PASS_WATER_MOMENTUM);
+    launch_water_momentum(h->K, f, n, s);
+    SF_HIP(hipGetLastError());
+    return SF_OK;
+}
+
+int advect(sf_system* h, float dt, const CflOut* dt_dev, float render_interval, hipEvent_t* ev = nullptr) {
+    hipStream_t s = h->stream;
+    SF_MARK(ev, SF_PASS_ADVECT);
+    AdvectArgs a;
+    a.pm_in = h->pm[h->cur].p;
+    a.vr_in = h->vr[h->cur].p;
+    a.acc = h->acc.p;
+    a.pm_out = h->pm[h->cur ^ 1].p;
+    a.vr_out = h->vr[h->cur ^ 1].p;
+    a.bpv = h->bpv.p;
+    a.bcell = h->bcell.p;
+    a.dt = dt;
+    a.dt_dev = dt_dev;
+    a.render_interval = render_interval;
+    launch_advect(h->K, a, (int)h->n, s);
+    h->cur ^= 1;
+    SF_MARK(ev, SF_PASS_COUNT);
+    SF_HIP(hipGetLastError());
+    return SF_OK;
+}
+
+int one_step(sf_system* h, float dt, hipEvent_t* ev = nullptr) {
+    int rc;
+    if ((rc = build_neighbors(h, ev))) return rc;
+    if ((rc = solve(h, ev))) return rc;
+    return advect(h, dt, nullptr, 0.f, ev);
+}
+
+int check_handle(sf_system* h) {
+    if (!h) return set_error(SF_E_INVALID, "null handle");
+    return SF_OK;
+}
+int set_device(sf_system* h) {
+    SF_HIP(hipSetDevice(h->device));
+    return SF_OK;
+}
+
+// host-side unpack of a device float4 array into k floats per element
+int fetch4(sf_system* h, const float4* src, size_t n, std::vector<float4>& out) {
+    out.resize(n);
+    if (n == 0) return SF_OK;
+    SF_HIP(hipMemcpyAsync(out.data(), src, n * sizeof(float4), hipMemcpyDeviceToHost, h->stream));
+    SF_HIP(hipStreamSynchronize(h->stream));
+    return SF_OK;
+}
+template <class T>
+int fetch(sf_system* h, const T* src, size_t n, std::vector<T>& out) {
+    out.resize(n);
+    if (n == 0) return SF_OK;
+    SF_HIP(hipMemcpyAsync(out.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, h->stream));
+    SF_HIP(hipStreamSynchronize(h->stream));
+    return SF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sf_create(const sf_params* p, const sf_boundary_params* b, const sf_solver_options* opts, size_t max_num,
+              int device, sf_handle* out) {
+    if (!out) return set_error(SF_E_INVALID, "null output handle");
+    *out = nullptr;
+    int rc = validate(p, b);
+    if (rc) return rc;
+    if (max_num > (size_t)0x7fffffff - kRsTile) return set_error(SF_E_CAPACITY, "max_num above 2^31");
+    sf_system* h = new sf_system();
+    h->P = *p;
+    h->B = *b;
+    h->O = opts ? *opts : sf_solver_options{0.f, 3.f, 100.f};
+    if (device < 0) {
+        hipError_t e = hipGetDevice(&h->device);
+        if (e != hipSuccess) {
+            delete h;
+            return hip_error(e, "hipGetDevice");
+        }
+    } else {
+        h->device = device;
+    }
+    auto bail = [&](int code) {
+        sf_destroy(h);
+        return code;
+    };
+    if ((rc = set_device(h))) return bail(rc);
+    {
+        hipError_t e = hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking);
+        if (e != hipSuccess) return bail(hip_error(e, "hipStreamCreate"));
+        h->stream = h->own_stream;
+        if ((e = hipEventCreate(&h->ev0)) != hipSuccess || (e = hipEventCreate(&h->ev1)) != hipSuccess)
+            return bail(hip_error(e, "hipEventCreate"));
+    }
+    h->cap = max_num;
+    h->C = (uint32_t)((size_t)b->grid_size.x * b->grid_size.y * b->grid_size.z);
+    h->plan = make_radix_plan(h->C);
+    h->K = make_const(h->P, h->B, h->O, 0);
+    const size_t c = std::max<size_t>(max_num, 1);
+    for (int q = 0; q < 2; ++q) {
+        if ((rc = h->pm[q].alloc(c)) || (rc = h->vr[q].alloc(c)) || (rc = h->cs[q].alloc(c)) ||
+            (rc = h->cda[q].alloc(c)) || (rc = h->amc[q].alloc(c)))
+            return bail(rc);
+    }
+    if ((rc = h->rp.alloc(c)) || (rc = h->es.alloc(c)) || (rc = h->flow.alloc(c)) || (rc = h->adh.alloc(c)) ||
+        (rc = h->xf.alloc(c)) || (rc = h->acc.alloc(c)))
+        return bail(rc);
+    if ((rc = h->cell.alloc((size_t)h->C + 1)) || (rc = h->bcell.alloc((size_t)h->C + 1))) return bail(rc);
+    if ((rc = h->cfl_part.alloc(cfl_partials(0))) || (rc = h->cfl_out.alloc(1))) return bail(rc);
+    if ((rc = ensure_sort_capacity(h, c))) return bail(rc);
+    if ((rc = h->bpv.alloc(1))) return bail(rc);
+    // zero everything like CudaArray::Clear (cuda_array.cuh:51-54)
+    for (int q = 0; q < 2; ++q) {
+        if ((rc = zero(h, h->pm[q])) || (rc = zero(h, h->vr[q])) || (rc = zero(h, h->cs[q])) ||
+            (rc = zero(h, h->cda[q])) || (rc = zero(h, h->amc[q])))
+            return bail(rc);
+    }
+    if ((rc = zero(h, h->rp)) || (rc = zero(h, h->es)) || (rc = zero(h, h->flow)) || (rc = zero(h, h->adh)) ||
+        (rc = zero(h, h->xf)) || (rc = zero(h, h->acc)) || (rc = zero(h, h->cell)) || (rc = zero(h, h->bcell)) ||
+        (rc = zero(h, h->keys)) || (rc = zero(h, h->bpv)))
+        return bail(rc);
+    {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return bail(hip_error(e, "hipStreamSynchronize"));
+    }
+    *out = h;
+    return SF_OK;
+}
+
+int sf_destroy(sf_handle h) {
+    if (!h) return SF_OK;
+    (void)hipSetDevice(h->device);
+    if (h->own_stream) (void)hipStreamSynchronize(h->own_stream);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+    if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+    delete h;
+    return SF_OK;
+}
+
+int sf_set_params(sf_handle h, const sf_params* p) {
+    int rc;
+    if ((rc = check_handle(h))) return rc;
+    if ((rc = validate(p, &h->B))) return rc;
+    if (p->sf_type != h->P.sf_type && h->uploaded)
+        return set_error(SF_E_STATE, "sf_type cannot change after particles are uploaded");
+    h->P = *p;
+    h->K = make_const(h->P, h->B, h->O, h->blabel1);
+    return SF_OK;
+}
+
+int sf_set_stream(sf_handle h, void* stream) {
+    int rc;
+    if ((rc = check_handle(h))) return rc;
+    h->stream = stream ? (hipStream_t)stream : h->own_stream;
+    return SF_OK;
+}
+
+int sf_synchronize(sf_handle h) {
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    SF_HIP(hipStreamSynchronize(h->stream));
+    return SF_OK;
+}
+
+int sf_upload_particles(sf_handle h, size_t n, const float* pos3, const float* vel3, const float* col3,
+                        const uint64_t* label, const float* mass, const float* radius, const float* cda0asat3,
+                        const float* amcamcp2) {
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (n > h->cap) return set_error(SF_E_CAPACITY, "particle count exceeds the handle's max_num");
+    if (n > 0 && (!pos3 || !label || !mass)) return set_error(SF_E_INVALID, "pos3, label and mass are required");
+    std::vector<float4> pm(n), vr(n), cs(n), cda(n);
+    std::vector<float2> amc(n);
+    for (size_t i = 0; i < n; ++i) {
+        if (label[i] > 1) return set_error(SF_E_INVALID, "labels must be 0 (water) or 1 (sand)");
+        if (!(mass[i] > 0.f) || !std::isfinite(mass[i])) return set_error(SF_E_INVALID, "masses must be finite and > 0");
+        const float m = label[i] == 1 ? -mass[i] : mass[i];
+        pm[i] = make_float4(pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2], m);
+        vr[i] = vel3 ? make_float4(vel3[3 * i], vel3[3 * i + 1], vel3[3 * i + 2], radius ? radius[i] : 0.f)
+                     : make_float4(0.f, 0.f, 0.f, radius ? radius[i] : 0.f);
+        cs[i] = col3 ? make_float4(col3[3 * i], col3[3 * i + 1], col3[3 * i + 2], 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+        cda[i] = cda0asat3 ? make_float4(cda0asat3[3 * i], cda0asat3[3 * i + 1], cda0asat3[3 * i + 2], 0.f)
+                           : make_float4(0.f, 0.f, 0.f, 0.f);
+        amc[i] = amcamcp2 ? make_float2(amcamcp2[2 * i], amcamcp2[2 * i + 1]) : make_float2(0.f, 0.f);
+    }
+    h->cur = h->cur_cs = h->cur_aux = 0;
+    if (n) {
+        SF_HIP(hipMemcpyAsync(h->pm[0].p, pm.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+        SF_HIP(hipMemcpyAsync(h->vr[0].p, vr.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+        SF_HIP(hipMemcpyAsync(h->cs[0].p, cs.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+        SF_HIP(hipMemcpyAsync(h->cda[0].p, cda.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+        SF_HIP(hipMemcpyAsync(h->amc[0].p, amc.data(), n * sizeof(float2), hipMemcpyHostToDevice, h->stream));
+    }
+    SF_HIP(hipStreamSynchronize(h->stream));
+    h->n = n;
+    h->uploaded = true;
+    h->built = false;
+    return SF_OK;
+}
+
+int sf_upload_boundary(sf_handle h, size_t nb, const float* pos3, const uint64_t* label) {
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (nb > 0 && !pos3) return set_error(SF_E_INVALID, "null boundary positions");
+    if (nb > (size_t)0x7fffffff - kRsTile) return set_error(SF_E_CAPACITY, "too many boundary particles");
+    std::vector<float4> bp(nb);
+    int l1 = 0;
+    for (size_t i = 0; i < nb; ++i) {
+        const uint64_t l = label ? label[i] : 0;
+        if (l > 1) return set_error(SF_E_INVALID, "boundary labels must be 0 or 1");
+        l1 += l == 1;
+        bp[i] = make_float4(pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2], l == 1 ? -1.f : 1.f);
+    }
+    if ((rc = ensure_sort_capacity(h, std::max(h->cap, nb)))) return rc;
+    if ((rc = h->bpv.alloc(std::max<size_t>(nb, 1)))) return rc;
+    DevBuf<float4> tmp;
+    if ((rc = tmp.alloc(std::max<size_t>(nb, 1)))) return rc;
+    if (nb) SF_HIP(hipMemcpyAsync(tmp.p, bp.data(), nb * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+    // CudaGNBoundarySearcher::SortData (cuda_neighbor_searcher.cu:125-139): stable sort by cell
+    launch_hash_boundary(h->K, tmp.p, h->keys.p, (int)nb, h->stream);
+    SortBuffers sb{h->kA.p, h->vA.p, h->kB.p, h->vB.p, h->hist.p};
+    const uint32_t *sk = nullptr, *bperm = nullptr;
+    if ((rc = radix_sort(h->keys.p, (int)nb, h->plan, sb, &sk, &bperm, h->stream))) return rc;
+    launch_cell_start(sk, (int)nb, h->C, h->bcell.p, h->stream);
+    launch_boundary_gather(bperm, tmp.p, h->bpv.p, (int)nb, h->stream);
+    launch_boundary_volume(h->K, h->bpv.p, h->bcell.p, (int)nb, h->stream);
+    SF_HIP(hipGetLastError());
+    SF_HIP(hipStreamSynchronize(h->stream));
+    h->nb = nb;
+    h->blabel1 = l1;
+    h->K = make_const(h->P, h->B, h->O, h->blabel1);
+    h->built = false;  // the particle searcher buffers were reused
+    return SF_OK;
+}
+
+int sf_build_neighbors(sf_handle h) {
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!h->uploaded) return set_error(SF_E_STATE, "no particles uploaded");
+    if ((rc = build_neighbors(h))) return rc;
+    SF_HIP(hipStreamSynchronize(h->stream));
+    return SF_OK;
+}
+
+int sf_step(sf_handle h, float dt) { return sf_step_n(h, dt, 1); }
+
+int sf_step_n(sf_handle h, float dt, int n_steps) {
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!h->uploaded) return set_error(SF_E_STATE, "no particles uploaded");
+    if (!std::isfinite(dt) || !(dt > 0.f)) return set_error(SF_E_INVALID, "dt must be finite and > 0");
+    if (n_steps < 0) return set_error(SF_E_INVALID, "n_steps must be >= 0");
+    for (int q = 0; q < n_steps; ++q)
+        if ((rc = one_step(h, dt))) return rc;
+    SF_HIP(hipStreamSynchronize(h->stream));
+    return SF_OK;
+}
+
+int sf_step_profiled(sf_handle h, float dt, int n_steps, float* pass_ms, float* total_ms) {
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!h->uploaded) return set_error(SF_E_STATE, "no particles uploaded");
+    if (!std::isfinite(dt) || !(dt > 0.f)) return set_error(SF_E_INVALID, "dt must be finite and > 0");
+    if (n_steps < 1 || n_steps > 100000) return set_error(SF_E_INVALID, "n_steps must be in [1, 100000]");
+    const size_t per = SF_PASS_COUNT + 1;
+    const size_t need = pass_ms ? per * (size_t)n_steps : 0;
+    while (h->ev_pool.size() < need) {
+        hipEvent_t e;
+        SF_HIP(hipEventCreate(&e));
+        h->ev_pool.push_back(e);
+    }
+    hipStream_t s = h->stream;
+    SF_HIP(hipEventRecord(h->ev0, s));
+    for (int q = 0; q < n_steps; ++q)
+        if ((rc = one_step(h, dt, pass_ms ? h->ev_pool.data() + per * q : nullptr))) return rc;
+    SF_HIP(hipEventRecord(h->ev1, s));
+    SF_HIP(hipEventSynchronize(h->ev1));
+    if (total_ms) SF_HIP(hipEventElapsedTime(total_ms, h->ev0, h->ev1));
+    if (pass_ms) {
+        for (int k = 0; k < SF_PASS_COUNT; ++k) pass_ms[k] = 0.f;
+        for (int q = 0; q < n_steps; ++q) {
+            hipEvent_t* e = h->ev_pool.data() + per * q;
+            for (int k = 0; k < SF_PASS_COUNT; ++k) {
+                float ms = 0.f;
+                SF_HIP(hipEventElapsedTime(&ms, e[k], e[k + 1]));
+                pass_ms[k] += ms;
+            }
+        }
+        for (int k = 0; k < SF_PASS_COUNT; ++k) pass_ms[k] /= (float)n_steps;
+    }
+    return SF_OK;
+}
+
+int sf_update_system(sf_handle h, float render_interval, float* ms_out) {
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!h->uploaded) return set_error(SF_E_STATE, "no particles uploaded");
+    if (!std::isfinite(render_interval) || !(render_interval > 0.f))
+        return set_error(SF_E_INVALID, "render interval must be finite and > 0");
+    SF_HIP(hipEventRecord(h->ev0, h->stream));
+    if ((rc = build_neighbors(h))) return rc;
+    if (h->P.solver_type == SF_WCSPH_SOLVER) {
+        if ((rc = solve(h))) return rc;
+        launch_cfl(h->K, h->acc.p, (int)h->n, h->cfl_part.p, h->cfl_out.p, render_interval, h->P.sph_mass,
+                   h->O.speed_of_sound, h->O.time_step_limit_scale, h->P.dt, h->stream);
+        if ((rc = advect(h, 0.f, h->cfl_out.p, render_interval))) return rc;
+        CflOut o;
+        SF_HIP(hipMemcpyAsync(&o, h->cfl_out.p, sizeof(CflOut), hipMemcpyDeviceToHost, h->stream));
+        SF_HIP(hipEventRecord(h->ev1, h->stream));
+        SF_HIP(hipEventSynchronize(h->ev1));
+        h->nsub = o.nsub;
+    } else {
+        // CudaSphSFSolver::UpdateSolver: nSub = renderInterval / params.dt, advect with params.dt
+        const float q = render_interval / h->P.dt;
+        h->nsub = q >= 1.f ? (size_t)q : 1;
+        if ((rc = solve(h))) return rc;
+        if ((rc = advect(h, h->P.dt, nullptr, 0.f))) return rc;
+        SF_HIP(hipEventRecord(h->ev1, h->stream));
+        SF_HIP(hipEventSynchronize(h->ev1));
+    }
+    if (ms_out) {
+        float ms = 0.f;
+        SF_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        *ms_out = ms;
+    }
+    return SF_OK;
+}
+
+int sf_num_sub_steps(sf_handle h, size_t* out) {
+    int rc;
+    if ((rc = check_handle(h))) return rc;
+    if (!out) return set_error(SF_E_INVALID, "null output");
+    *out = h->nsub;
+    return SF_OK;
+}
+
+int sf_size(sf_handle h, size_t* n, size_t* nb, size_t* num_cells) {
+    int rc;
+    if ((rc = check_handle(h))) return rc;
+    if (n) *n = h->n;
+    if (nb) *nb = h->nb;
+    if (num_cells) *num_cells = h->C;
+    return SF_OK;
+}
+
+int sf_download(sf_handle h, int field, void* dst, size_t dst_bytes) {
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    const size_t n = h->n, nb = h->nb;
+    auto need = [&](size_t bytes) -> int {
+        if (!dst && bytes) return set_error(SF_E_INVALID, "null destination");
+        if (dst_bytes < bytes) return set_error(SF_E_INVALID, "destination buffer too small");
+        return SF_OK;
+    };
+    std::vector<float4> v4;
+    std::vector<float2> v2;
+    std::vector<uint32_t> u;
+    auto put3 = [&](const float4* src, size_t cnt, int mask_label /* -1 none, 0 water only, 1 sand only */) -> int {
+        int r;
+        if ((r = need(cnt * 12))) return r;
+        if ((r = fetch4(h, src, cnt, v4))) return r;
+        std::vector<float4> pmh;
+        if (mask_label >= 0 && (r = fetch4(h, h->pm[h->cur].p, cnt, pmh))) return r;
+        float* d = (float*)dst;
+        for (size_t i = 0; i < cnt; ++i) {
+            bool keep = mask_label < 0 || (mask_label == 1) == (pmh[i].w < 0.f);
+            d[3 * i] = keep ? v4[i].x : 0.f;
+            d[3 * i + 1] = keep ? v4[i].y : 0.f;
+            d[3 * i + 2] = keep ? v4[i].z : 0.f;
+        }
+        return SF_OK;
+    };
+    auto put1w = [&](const float4* src, size_t cnt, bool absval) -> int {
+        int r;
+        if ((r = need(cnt * 4))) return r;
+        if ((r = fetch4(h, src, cnt, v4))) return r;
+        float* d = (float*)dst;
+        for (size_t i = 0; i < cnt; ++i) d[i] = absval ? fabsf(v4[i].w) : v4[i].w;
+        return SF_OK;
+    };
+    auto put2 = [&](const float2* src, size_t cnt, int comp) -> int {
+        int r;
+        if ((r = need(cnt * 4))) return r;
+        if ((r = fetch(h, src, cnt, v2))) return r;
+        float* d = (float*)dst;
+        for (size_t i = 0; i < cnt; ++i) d[i] = comp ? v2[i].y : v2[i].x;
+        return SF_OK;
+    };
+    auto putu = [&](const uint32_t* src, size_t cnt) -> int {
+        int r;
+        if ((r = need(cnt * 4))) return r;
+        if (cnt == 0) return SF_OK;
+        if (!src) return set_error(SF_E_STATE, "searcher has not run yet");
+        SF_HIP(hipMemcpyAsync(dst, src, cnt * 4, hipMemcpyDeviceToHost, h->stream));
+        SF_HIP(hipStreamSynchronize(h->stream));
+        return SF_OK;
+    };
+    auto labels = [&](const float4* src, size_t cnt) -> int {
+        int r;
+        if ((r = need(cnt * 8))) return r;
+        if ((r = fetch4(h, src, cnt, v4))) return r;
+        uint64_t* d = (uint64_t*)dst;
+        for (size_t i = 0; i < cnt; ++i) d[i] = v4[i].w < 0.f ? 1u : 0u;
+        return SF_OK;
+    };
+    switch (field) {
+    case SF_FIELD_POS: return put3(h->pm[h->cur].p, n, -1);
+    case SF_FIELD_VEL: return put3(h->vr[h->cur].p, n, -1);
+    case SF_FIELD_ACC: return put3(h->acc.p, n, -1);
+    case SF_FIELD_COL: return put3(h->cs[h->cur_cs].p, n, -1);
+    case SF_FIELD_LABEL: return labels(h->pm[h->cur].p, n);
+    case SF_FIELD_MASS: return put1w(h->pm[h->cur].p, n, true);
+    case SF_FIELD_RADIUS: return put1w(h->vr[h->cur].p, n, false);
+    case SF_FIELD_DENSITY: return put2(h->rp.p, n, 0);
+    case SF_FIELD_PRESSURE: return put2(h->rp.p, n, 1);
+    case SF_FIELD_VOIDAGE: return put2(h->es.p, n, 0);
+    case SF_FIELD_SATURATION: return put2(h->es.p, n, 1);
+    case SF_FIELD_MAX_SATURATION: return put1w(h->cs[h->cur_cs].p, n, false);
+    case SF_FIELD_AVG_FLOW_VEL: return put3(h->flow.p, n, -1);
+    case SF_FIELD_AVG_DRAG_FORCE: return put3(h->xf.p, n, 1);
+    case SF_FIELD_ADHESION_FORCE: return put3(h->adh.p, n, 0);
+    case SF_FIELD_AVG_ADHESION_FORCE: return put3(h->xf.p, n, 0);
+    case SF_FIELD_CDA0ASAT: return put3(h->cda[h->cur_aux].p, n, -1);
+    case SF_FIELD_AMCAMCP: {
+        if ((rc = need(n * 8))) return rc;
+        if ((rc = fetch(h, h->amc[h->cur_aux].p, n, v2))) return rc;
+        if (n) std::memcpy(dst, v2.data(), n * 8);
+        return SF_OK;
+    }
+    case SF_FIELD_PARTICLE2CELL: return putu(h->built ? h->keys.p : nullptr, n);
+    case SF_FIELD_SORT_PERM: return putu(h->built ? h->perm : nullptr, n);
+    case SF_FIELD_SORTED_KEYS: return putu(h->built ? h->sorted_keys : nullptr, n);
+    case SF_FIELD_CELL_START: return putu(h->cell.p, (size_t)h->C + 1);
+    case SF_FIELD_NBR_COUNT:
+    case SF_FIELD_NBR_CANDIDATES: {
+        if ((rc = need(n * 4))) return rc;
+        if (!h->built) return set_error(SF_E_STATE, "searcher has not run yet");
+        if (h->nbr_cnt.n < std::max<size_t>(n, 1)) {
+            if ((rc = h->nbr_cnt.alloc(std::max<size_t>(n, 1))) || (rc = h->nbr_cand.alloc(std::max<size_t>(n, 1))))
+                return rc;
+        }
+        // counts refer to the positions the last neighbour build sorted: the
+        // pre-advect state is gone after a step, so count on the current
+        // positions with the current cellStart only directly after
+        // sf_build_neighbors (callers use it that way).
+        launch_nbr_count(h->K, h->pm[h->cur].p, h->cell.p, h->nbr_cnt.p, h->nbr_cand.p, (int)n, h->stream);
+        SF_HIP(hipGetLastError());
+        return putu(field == SF_FIELD_NBR_COUNT ? h->nbr_cnt.p : h->nbr_cand.p, n);
+    }
+    case SF_FIELD_B_POS: return put3(h->bpv.p, nb, -1);
+    case SF_FIELD_B_LABEL: return labels(h->bpv.p, nb);
+    case SF_FIELD_B_VOLUME: return put1w(h->bpv.p, nb, true);
+    case SF_FIELD_B_CELL_START: return putu(h->bcell.p, (size_t)h->C + 1);
+    default: return set_error(SF_E_NOTFOUND, "unknown field id " + std::to_string(field));
+    }
+}
+
+int sf_device_buffer(sf_handle h, int field, void** dptr, size_t* stride) {
+    int rc;
+    if ((rc = check_handle(h))) return rc;
+    if (!dptr || !stride) return set_error(SF_E_INVALID, "null output");
+    void* p = nullptr;
+    size_t s = 16;
+    switch (field) {
+    case SF_FIELD_POS: case SF_FIELD_MASS: case SF_FIELD_LABEL: p = h->pm[h->cur].p; break;
+    case SF_FIELD_VEL: case SF_FIELD_RADIUS: p = h->vr[h->cur].p; break;
+    case SF_FIELD_ACC: p = h->acc.p; break;
+    case SF_FIELD_COL: case SF_FIELD_MAX_SATURATION: p = h->cs[h->cur_cs].p; break;
+    case SF_FIELD_DENSITY: case SF_FIELD_PRESSURE: p = h->rp.p; s = 8; break;
+    case SF_FIELD_VOIDAGE: case SF_FIELD_SATURATION: p = h->es.p; s = 8; break;
+    case SF_FIELD_AVG_FLOW_VEL: p = h->flow.p; break;
+    case SF_FIELD_AVG_DRAG_FORCE: case SF_FIELD_AVG_ADHESION_FORCE: p = h->xf.p; break;
+    case SF_FIELD_ADHESION_FORCE: p = h->adh.p; break;
+    case SF_FIELD_CDA0ASAT: p = h->cda[h->cur_aux].p; break;
+    case SF_FIELD_AMCAMCP: p = h->amc[h->cur_aux].p; s = 8; break;
+    case SF_FIELD_PARTICLE2CELL: p = h->keys.p; s = 4; break;
+    case SF_FIELD_SORT_PERM: p = (void*)h->perm; s = 4; break;
+    case SF_FIELD_SORTED_KEYS: p = (void*)h->sorted_keys; s = 4; break;
+    case SF_FIELD_CELL_START: p = h->cell.p; s = 4; break;
+    case SF_FIELD_B_POS: case SF_FIELD_B_VOLUME: case SF_FIELD_B_LABEL: p = h->bpv.p; break;
+    case SF_FIELD_B_CELL_START: p = h->bcell.p; s = 4; break;
+    default: return set_error(SF_E_NOTFOUND, "field has no device buffer");
+    }
+    *dptr = p;
+    *stride = s;
+    return SF_OK;
+}
+
+int sf_export_bgeo(sf_handle h, const char* folder, const char* name) {
+    int rc;
+    if ((rc = check_handle(h))) return rc;
+    if (!folder || !name) return set_error(SF_E_INVALID, "null folder or name");
+    const size_t n = h->n;
+    std::vector<float> pos(3 * n), col(3 * n), rad(n);
+    std::vector<uint64_t> lab(n);
+    if ((rc = sf_download(h, SF_FIELD_POS, pos.data(), pos.size() * 4))) return rc;
+    if ((rc = sf_download(h, SF_FIELD_COL, col.data(), col.size() * 4))) return rc;
+    if ((rc = sf_download(h, SF_FIELD_RADIUS, rad.data(), rad.size() * 4))) return rc;
+    if ((rc = sf_download(h, SF_FIELD_LABEL, lab.data(), lab.size() * 8))) return rc;
+    if ((rc = make_dirs(folder))) return rc;
+    std::string path = std::string(folder) + "/" + name + ".bgeo";
+    return sf_write_bgeo(path.c_str(), n, pos.data(), col.data(), rad.data(), lab.data());
+}
+
+}  // extern "C"

@@ -1,0 +1,189 @@
+// sf_device.h -- device-side math for the gfx950 seepage kernels.
+//
+// Storage layout (all arrays in the particles' current, cell-sorted order):
+//   pm  float4  (x, y, z, +mass for water / -mass for sand)
+//   vr  float4  (vx, vy, vz, radius)
+//   cs  float4  (r, g, b, maxSaturation)
+//   rp  float2  (density, pressure)
+//   es  float2  (voidage, saturation)
+//   flow/adh/xf/acc float4 (xyz used); xf holds avgAdhesion on water slots
+//   and avgDrag on sand slots (each is only ever read from the other phase).
+//   bpv float4  (x, y, z, +volume for boundary label 0 / -volume for label 1)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SF_DEV __device__ __forceinline__
+
+namespace sf {
+
+constexpr float kEps = 1e-6f;  // KIRI_EPSILON (kiri_pbs_pch.cuh:52)
+constexpr int kBlock = 256;    // 4 wave64s per workgroup
+
+struct v3 {
+    float x, y, z;
+};
+SF_DEV v3 mk3(float x, float y, float z) { return {x, y, z}; }
+SF_DEV v3 xyz(float4 a) { return {a.x, a.y, a.z}; }
+SF_DEV v3 operator+(v3 a, v3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+SF_DEV v3 operator-(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+SF_DEV v3 operator-(v3 a) { return {-a.x, -a.y, -a.z}; }
+SF_DEV v3 operator*(v3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+SF_DEV v3 operator*(float s, v3 a) { return {a.x * s, a.y * s, a.z * s}; }
+SF_DEV v3& operator+=(v3& a, v3 b) {
+    a.x += b.x; a.y += b.y; a.z += b.z;
+    return a;
+}
+SF_DEV float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+SF_DEV float4 f4(v3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+
+// Per-scene constants, precomputed on the host in FP32 the way the reference's
+// functor constructors compute them (cuda_sph_kernel.cuh:79-118,189-203,
+// cuda_sf_utils.cuh:36-57, cuda_dem_solver_common_gpu.cuh:63-65).
+struct Bez {
+    float py0, py1, px1, py2, b, c, d, e;
+};
+struct SFConst {
+    // grid (CudaBoundaryParams)
+    float lox, loy, loz, h;
+    float hix, hiy, hiz, h2;
+    int gx, gy, gz;
+    uint32_t C;  // gx*gy*gz
+    // SPH kernels
+    float w_coef, w0, g_coef, adh_coef;
+    // water
+    float rho0, stiff, neg_scale, nu, bnu, rw;
+    // sand
+    float rho_s, R, kn, ks, tan2, tanphi, damping, young, poisson, cd;
+    // capillary bridge
+    float cap_vb, cap_srup, cap_vb_piR, cap_cos, cap_sin, cap_pi;
+    Bez gcap, gadh;
+    // gravity, colours
+    float gvx, gvy, gvz, grav_abs;
+    float dryr, dryg, dryb, wetr, wetg, wetb;
+    int tait;      // WCSPH Tait EOS (1) or linear EOS (0)
+    int multi;     // MULTI_SF: per-particle cd, fixed adhesion curve
+    int blabel1;   // number of boundary particles with label 1 (0 = skip those loops)
+};
+
+// ---- grid hash (helper_thrust.cuh:56-63), FP32 and exactly as written -----
+SF_DEV int cell_coord(float q, int g) {
+    if (!(q >= 0.f)) return 0;
+    if (q >= (float)g) return g - 1;
+    return (int)q;
+}
+SF_DEV void cell_of(const SFConst& k, float px, float py, float pz, int& cx, int& cy, int& cz) {
+#pragma clang fp contract(off)
+    float rx = px - k.lox, ry = py - k.loy, rz = pz - k.loz;
+    cx = cell_coord(rx / k.h, k.gx);
+    cy = cell_coord(ry / k.h, k.gy);
+    cz = cell_coord(rz / k.h, k.gz);
+}
+SF_DEV uint32_t cell_key(const SFConst& k, int cx, int cy, int cz) {
+    return (uint32_t)((cx * k.gy + cy) * k.gz + cz);
+}
+
+// The 27-cell neighbourhood as 9 (dx, dy) columns; in each column the cells
+// z-1, z, z+1 have consecutive keys, so their particles form one contiguous
+// range [cs[first], cs[last + 1]) of the sorted arrays.  Out-of-grid cells are
+// skipped, as the reference's sentinel hash does (helper_thrust.cuh:123-128).
+template <class F>
+SF_DEV void for_columns(const SFConst& k, int cx, int cy, int cz, F&& f) {
+    const int zlo = cz > 0 ? cz - 1 : 0;
+    const int zhi = cz + 1 < k.gz ? cz + 1 : k.gz - 1;
+    for (int dx = -1; dx <= 1; ++dx) {
+        const int x = cx + dx;
+        if (x < 0 || x >= k.gx) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = cy + dy;
+            if (y < 0 || y >= k.gy) continue;
+            const uint32_t base = (uint32_t)((x * k.gy + y) * k.gz);
+            f(base + (uint32_t)zlo, base + (uint32_t)zhi + 1u);
+        }
+    }
+}
+
+// ---- SPH kernels on r^2 (Poly6 cuda_sph_kernel.cuh:79-98, Spiky :100-118) ----
+SF_DEV float poly6_r2(const SFConst& k, float r2) {
+    if (r2 >= k.h2) return 0.f;
+    float d = k.h2 - r2;
+    return k.w_coef * d * d * d;
+}
+// returns the scalar s with gradW(r) = s * r
+SF_DEV float spiky_s(const SFConst& k, float r) {
+    if (r >= k.h || r < kEps) return 0.f;
+    float d = k.h - r;
+    return k.g_coef * d * d / r;
+}
+// AdhesionAkinci13 (:189-203)
+SF_DEV float adhesion_A(const SFConst& k, float r) {
+    if (2.f * r > k.h && r <= k.h) return k.adh_coef * powf(-4.f * r * r / k.h + 6.f * r - 2.f * k.h, 0.25f);
+    return 0.f;
+}
+// QuadraticBezierCoeff (cuda_sf_utils.cuh:36-82)
+SF_DEV float bezier(const Bez& g, float sr) {
+    if (sr < 0.f) return g.py0;
+    if (sr >= 1.f) return g.py2;
+    if (sr <= g.px1) {
+        float t = sr / g.px1, omt = 1.f - t;
+        return omt * omt * g.py0 + 2.f * t * omt * g.py1 + t * t * g.py1;
+    }
+    float t = (g.b + sqrtf(g.c + g.d * (g.px1 - sr))) / g.e;
+    float omt = 1.f - t;
+    return omt * omt * g.py1 + 2.f * t * omt * g.py1 + t * t * g.py2;
+}
+
+// ComputeDemForces (cuda_dem_solver_common_gpu.cuh:20-51) for a pair known to
+// overlap (dist < rij); dist = |dij| > 0.
+SF_DEV v3 dem_contact(v3 dij, float dist, v3 vij, float rij, float kn, float ks, float tan2) {
+    float pen = rij - dist;
+    v3 n = dij * (1.f / dist);
+    float de = dot(vij, n);
+    v3 vt = vij - de * n;
+    v3 fn = (kn * pen) * n;
+    v3 fs = (-ks) * vt;
+    float maxfs = dot(fn, fn) * tan2;
+    float fs2 = dot(fs, fs);
+    if (fs2 > maxfs) fs = fs * (sqrtf(maxfs) / sqrtf(fs2));
+    return -fn - fs;
+}
+SF_DEV v3 dem_force(v3 dij, v3 vij, float rij, float kn, float ks, float tan2) {
+    float dist = sqrtf(dot(dij, dij));
+    if (rij - dist > 0.f) return dem_contact(dij, dist, vij, rij, kn, ks, tan2);
+    return mk3(0.f, 0.f, 0.f);
+}
+// ComputeDemCapillaryForces (:53-90) inside the window 0 < H < s_rupture
+SF_DEV v3 capillary(const SFConst& k, v3 dij, float dist, float H, float sr) {
+    v3 N = dij * (1.f / dist);
+    float cc = bezier(k.gcap, sr);
+    float R = k.R;
+    float d = -H + sqrtf(H * H + k.cap_vb_piR);
+    float phi = sqrtf(2.f * H / R * (-1.f + sqrtf(1.f + k.cap_vb_piR / (H * H))));
+    float neck = -2.f * k.cap_pi * cc * R * k.cap_cos / (1.f + H / (2.f * d));
+    float surf = -2.f * k.cap_pi * cc * R * phi * k.cap_sin;
+    return -N * (neck + surf);
+}
+// ComputeDemWorldBoundaryForces (:132-199); z-low trigger 2*rij as written (:191)
+SF_DEV v3 world_walls(const SFConst& k, v3 p, v3 v) {
+    const float rij = k.rw + k.R;
+    v3 f = mk3(0.f, 0.f, 0.f);
+    v3 mv = -v;
+    if (p.x > k.hix - rij) f += dem_force(mk3(1.f, 0.f, 0.f) * fabsf(p.x - k.hix), mv, rij, k.kn, k.ks, k.tan2);
+    if (p.x < k.lox + rij) f += dem_force(mk3(-1.f, 0.f, 0.f) * fabsf(p.x - k.lox), mv, rij, k.kn, k.ks, k.tan2);
+    if (p.y > k.hiy - rij) f += dem_force(mk3(0.f, 1.f, 0.f) * fabsf(p.y - k.hiy), mv, rij, k.kn, k.ks, k.tan2);
+    if (p.y < k.loy + rij) f += dem_force(mk3(0.f, -1.f, 0.f) * fabsf(p.y - k.loy), mv, rij, k.kn, k.ks, k.tan2);
+    if (p.z > k.hiz - rij) f += dem_force(mk3(0.f, 0.f, 1.f) * fabsf(p.z - k.hiz), mv, rij, k.kn, k.ks, k.tan2);
+    if (p.z < k.loz + 2.f * rij) f += dem_force(mk3(0.f, 0.f, -1.f) * fabsf(p.z - k.loz), mv, rij, k.kn, k.ks, k.tan2);
+    return f;
+}
+
+// XCD-aware block index: the 8 XCDs receive workgroups round-robin, so map
+// them to 8 contiguous chunks of the cell-sorted particle range; spatial
+// neighbours then share one XCD's L2.
+SF_DEV uint32_t xcd_block() {
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, l = b >> 3;
+    return x < r ? x * (q + 1u) + l : r * (q + 1u) + (x - r) * q + l;
+}
+
+}  // namespace sf

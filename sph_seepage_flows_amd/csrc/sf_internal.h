@@ -1,0 +1,137 @@
+// sf_internal.h -- host-side structures shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/sf_seepage.h"
+#include "sf_device.h"
+
+namespace sf {
+
+// Thread-local error text behind sf_last_error().
+int set_error(int code, const std::string& msg);
+int hip_error(hipError_t e, const char* what);
+#define SF_HIP(call)                                              \
+    do {                                                          \
+        hipError_t e_ = (call);                                   \
+        if (e_ != hipSuccess) return ::sf::hip_error(e_, #call);  \
+    } while (0)
+
+// Owning device buffer (the role of CudaArray, cuda_array.cuh:19-61).
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return hip_error(e, "hipMalloc");
+        }
+        n = count;
+        return SF_OK;
+    }
+};
+
+// LSD radix sort plan over keys in [0, C): balanced digit widths <= 8 bits.
+struct RadixPlan {
+    int passes = 1;
+    int shift[4] = {0, 0, 0, 0};
+    int bits[4] = {1, 0, 0, 0};
+};
+RadixPlan make_radix_plan(uint32_t num_cells);
+
+constexpr int kRsThreads = 256;
+constexpr int kRsItems = 16;
+constexpr int kRsTile = kRsThreads * kRsItems;
+
+struct SortBuffers {
+    uint32_t* kA;
+    uint32_t* vA;
+    uint32_t* kB;
+    uint32_t* vB;
+    uint32_t* hist;  // 256 * ceil(n / kRsTile) entries
+};
+// Stable sort of keys[0, n): on return *sorted_keys / *perm point into the
+// ping-pong buffers (perm[new] = old index).
+int radix_sort(const uint32_t* keys, int n, const RadixPlan& plan, const SortBuffers& b, const uint32_t** sorted_keys,
+               const uint32_t** perm, hipStream_t s);
+
+// ---- searcher launchers (sf_searcher.hip) ----
+void launch_hash(const SFConst& k, const float4* pm, uint32_t* keys, int n, hipStream_t s);
+void launch_cell_start(const uint32_t* sorted_keys, int n, uint32_t C, uint32_t* cell_start, hipStream_t s);
+struct GatherArgs {
+    const uint32_t* perm;
+    const float4 *pm_in, *vr_in, *cs_in, *cda_in;
+    const float2* amc_in;
+    float4 *pm_out, *vr_out, *cs_out, *cda_out;
+    float2* amc_out;
+    int multi;
+};
+void launch_gather(const GatherArgs& a, int n, hipStream_t s);
+void launch_boundary_gather(const uint32_t* perm, const float4* bp_in, float4* bp_out, int nb, hipStream_t s);
+void launch_boundary_volume(const SFConst& k, float4* bpv, const uint32_t* bcs, int nb, hipStream_t s);
+void launch_nbr_count(const SFConst& k, const float4* pm, const uint32_t* cs, uint32_t* cnt, uint32_t* cand, int n,
+                      hipStream_t s);
+void launch_hash_boundary(const SFConst& k, const float4* bp, uint32_t* keys, int nb, hipStream_t s);
+
+// ---- solver launchers (sf_solver.hip) ----
+struct Fields {
+    const float4* pm;
+    const float4* vr;
+    float4* cs;
+    const float4* cda;
+    const uint32_t* cell;
+    const float4* bpv;
+    const uint32_t* bcell;
+    float2* rp;
+    float2* es;
+    float4* flow;
+    float4* adh;
+    float4* xf;
+    float4* acc;
+};
+void launch_density(const SFConst& k, const Fields& f, int n, hipStream_t s);
+void launch_avg_flow(const SFConst& k, const Fields& f, int n, hipStream_t s);
+void launch_voidage_adhesion(const SFConst& k, const Fields& f, int n, hipStream_t s);
+void launch_sand_momentum(const SFConst& k, const Fields& f, int n, hipStream_t s);
+void launch_water_momentum(const SFConst& k, const Fields& f, int n, hipStream_t s);
+// CFL reduction: partial maxima of |acc| -> finalize writes CflOut
+struct CflOut {
+    float dt;
+    uint32_t nsub;
+    float amax;
+    uint32_t pad;
+};
+int cfl_partials(int n);
+void launch_cfl(const SFConst& k, const float4* acc, int n, float* partials, CflOut* out, float render_interval,
+                float sph_mass, float speed_of_sound, float scale, float params_dt, hipStream_t s);
+struct AdvectArgs {
+    const float4* pm_in;
+    const float4* vr_in;
+    float4* acc;
+    float4* pm_out;
+    float4* vr_out;
+    const float4* bpv;
+    const uint32_t* bcell;
+    float dt;
+    const CflOut* dt_dev;  // when non-null, dt = render_interval / nsub from the device
+    float render_interval;
+};
+void launch_advect(const SFConst& k, const AdvectArgs& a, int n, hipStream_t s);
+
+}  // namespace sf

@@ -1,0 +1,331 @@
+// sf_searcher.hip -- uniform-grid neighbour searcher for gfx950.
+//
+// Replaces CudaGNBaseSearcher::BuildGNSearcher / CudaGNSearcher::SortData
+// (cuda_neighbor_searcher.cu:36-116): FP32 cell hash, a stable LSD radix sort
+// of 32-bit cell keys (wave64 ballot matching for in-wave ranks, LDS digit
+// counters per wave, a scanned per-block digit histogram), cellStart from the
+// sorted keys, and one gather of the persistent particle state.
+#include "sf_internal.h"
+
+namespace sf {
+
+// ---------------------------------------------------------------------------
+// hash: keys[i] = cell of pm[i]  (Pos2GridHash, helper_thrust.cuh:65-89)
+__global__ __launch_bounds__(kBlock) void k_hash(SFConst k, const float4* __restrict__ pm, uint32_t* __restrict__ keys,
+                                                 int n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pm[i];
+    int cx, cy, cz;
+    cell_of(k, p.x, p.y, p.z, cx, cy, cz);
+    keys[i] = cell_key(k, cx, cy, cz);
+}
+void launch_hash(const SFConst& k, const float4* pm, uint32_t* keys, int n, hipStream_t s) {
+    if (n <= 0) return;
+    k_hash<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, pm, keys, n);
+}
+void launch_hash_boundary(const SFConst& k, const float4* bp, uint32_t* keys, int nb, hipStream_t s) {
+    launch_hash(k, bp, keys, nb, s);
+}
+
+// ---------------------------------------------------------------------------
+// radix sort
+RadixPlan make_radix_plan(uint32_t num_cells) {
+    RadixPlan p;
+    int K = 0;
+    if (num_cells > 1) {
+        uint32_t m = num_cells - 1;
+        while (m) {
+            ++K;
+            m >>= 1;
+        }
+    }
+    if (K < 1) K = 1;
+    p.passes = (K + 7) / 8;
+    int sh = 0;
+    for (int i = 0; i < p.passes; ++i) {
+        int rem_passes = p.passes - i, rem_bits = K - sh;
+        int b = (rem_bits + rem_passes - 1) / rem_passes;
+        p.shift[i] = sh;
+        p.bits[i] = b;
+        sh += b;
+    }
+    return p;
+}
+
+// lanes of the wave whose digit equals this lane's digit (among valid lanes)
+SF_DEV uint64_t match_digit(uint32_t d, int bits, bool valid) {
+    uint64_t m = __ballot(valid);
+    for (int b = 0; b < bits; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
+    }
+    return m;
+}
+
+// per-block digit counts -> hist[d * nblocks + block]
+__global__ __launch_bounds__(kRsThreads) void k_rs_upsweep(const uint32_t* __restrict__ keys, int n, int shift,
+                                                           int bits, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t cnt[256];
+    const int t = threadIdx.x;
+    cnt[t] = 0;
+    __syncthreads();
+    const uint32_t mask = (1u << bits) - 1u;
+    const int base = blockIdx.x * kRsTile;
+    const uint64_t lt = __lanemask_lt();
+#pragma unroll 4
+    for (int r = 0; r < kRsItems; ++r) {
+        const int idx = base + r * kRsThreads + t;
+        const bool valid = idx < n;
+        const uint32_t d = valid ? (keys[idx] >> shift) & mask : 0u;
+        const uint64_t peers = match_digit(d, bits, valid);
+        if (valid && (peers & lt) == 0) atomicAdd(&cnt[d], (uint32_t)__popcll(peers));
+    }
+    __syncthreads();
+    if (t <= (int)mask) hist[(size_t)t * gridDim.x + blockIdx.x] = cnt[t];
+}
+
+// exclusive scan of hist[0, total) in place, one workgroup of 1024 threads
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ hist, int total) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry_s;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (t == 0) carry_s = 0;
+    __syncthreads();
+    for (int base = 0; base < total; base += 1024 * 4) {
+        uint32_t v[4];
+        uint32_t s = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int idx = base + t * 4 + q;
+            v[q] = idx < total ? hist[idx] : 0u;
+            s += v[q];
+        }
+        // inclusive wave scan of s
+        uint32_t x = s;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        if (w == 0) {
+            uint32_t ws = lane < 16 ? wsum[lane] : 0u;
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                uint32_t y = __shfl_up(ws, o, 64);
+                if (lane >= o) ws += y;
+            }
+            if (lane < 16) wsum[lane] = ws;  // inclusive over waves
+        }
+        __syncthreads();
+        const uint32_t carry = carry_s;
+        uint32_t run = carry + (w > 0 ? wsum[w - 1] : 0u) + x - s;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int idx = base + t * 4 + q;
+            if (idx < total) hist[idx] = run;
+            run += v[q];
+        }
+        __syncthreads();
+        if (t == 1023) carry_s = carry + wsum[15];
+        __syncthreads();
+    }
+}
+
+// stable scatter: each wave ranks a contiguous chunk of 64*kRsItems keys
+__global__ __launch_bounds__(kRsThreads) void k_rs_downsweep(const uint32_t* __restrict__ kin,
+                                                             const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
+                                                             uint32_t* __restrict__ vout, int n, int shift, int bits,
+                                                             const uint32_t* __restrict__ hist) {
+    __shared__ uint32_t wcnt[4][256];
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    for (int q = 0; q < 4; ++q) wcnt[q][t] = 0;
+    __syncthreads();
+    const uint32_t mask = (1u << bits) - 1u;
+    const uint64_t lt = __lanemask_lt();
+    const int base = blockIdx.x * kRsTile + w * (64 * kRsItems);
+    uint32_t key[kRsItems], val[kRsItems], rank[kRsItems];
+    volatile uint32_t* wc = wcnt[w];
+#pragma unroll
+    for (int r = 0; r < kRsItems; ++r) {
+        const int idx = base + r * 64 + lane;
+        const bool valid = idx < n;
+        key[r] = valid ? kin[idx] : 0xffffffffu;
+        val[r] = valid ? (vin ? vin[idx] : (uint32_t)idx) : 0u;
+        const uint32_t d = valid ? (key[r] >> shift) & mask : 0u;
+        const uint64_t peers = match_digit(d, bits, valid);
+        const uint32_t before = (uint32_t)__popcll(peers & lt);
+        const uint32_t c = wc[d];
+        rank[r] = c + before;
+        if (valid && before == 0) wc[d] = c + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    {
+        const uint32_t c0 = wcnt[0][t], c1 = wcnt[1][t], c2 = wcnt[2][t];
+        const uint32_t g = t <= (int)mask ? hist[(size_t)t * gridDim.x + blockIdx.x] : 0u;
+        __syncthreads();
+        wcnt[0][t] = g;
+        wcnt[1][t] = g + c0;
+        wcnt[2][t] = g + c0 + c1;
+        wcnt[3][t] = g + c0 + c1 + c2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRsItems; ++r) {
+        const int idx = base + r * 64 + lane;
+        if (idx < n) {
+            const uint32_t d = (key[r] >> shift) & mask;
+            const uint32_t pos = wcnt[w][d] + rank[r];
+            if (pos < (uint32_t)n) {  // always true for a consistent histogram; keeps a bug in bounds
+                kout[pos] = key[r];
+                vout[pos] = val[r];
+            }
+        }
+    }
+}
+
+int radix_sort(const uint32_t* keys, int n, const RadixPlan& plan, const SortBuffers& b, const uint32_t** sorted_keys,
+               const uint32_t** perm, hipStream_t s) {
+    *sorted_keys = b.kA;
+    *perm = b.vA;
+    if (n <= 0) return SF_OK;
+    const int nblocks = (n + kRsTile - 1) / kRsTile;
+    const uint32_t* kin = keys;
+    const uint32_t* vin = nullptr;  // identity on the first pass
+    uint32_t* ko = b.kA;
+    uint32_t* vo = b.vA;
+    for (int p = 0; p < plan.passes; ++p) {
+        k_rs_upsweep<<<nblocks, kRsThreads, 0, s>>>(kin, n, plan.shift[p], plan.bits[p], b.hist);
+        const int total = nblocks << plan.bits[p];
+        k_rs_scan<<<1, 1024, 0, s>>>(b.hist, total);
+        k_rs_downsweep<<<nblocks, kRsThreads, 0, s>>>(kin, vin, ko, vo, n, plan.shift[p], plan.bits[p], b.hist);
+        *sorted_keys = ko;
+        *perm = vo;
+        kin = ko;
+        vin = vo;
+        ko = (ko == b.kA) ? b.kB : b.kA;
+        vo = (vo == b.vA) ? b.vB : b.vA;
+    }
+    SF_HIP(hipGetLastError());
+    return SF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// cellStart[c] = number of particles with key < c, c in [0, C]
+// (CountingInCell_CUDA + exclusive_scan, cuda_neighbor_searcher.cu:45-47)
+__global__ __launch_bounds__(kBlock) void k_cell_start(const uint32_t* __restrict__ sk, int n, uint32_t C,
+                                                       uint32_t* __restrict__ cs) {
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c > C) return;
+    uint32_t lo = 0, hi = (uint32_t)n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sk[mid] < c) lo = mid + 1;
+        else hi = mid;
+    }
+    cs[c] = lo;
+}
+void launch_cell_start(const uint32_t* sorted_keys, int n, uint32_t C, uint32_t* cell_start, hipStream_t s) {
+    const uint32_t total = C + 1;
+    k_cell_start<<<(total + kBlock - 1) / kBlock, kBlock, 0, s>>>(sorted_keys, n, C, cell_start);
+}
+
+// ---------------------------------------------------------------------------
+// reorder of the persistent state (the zipped payload of sort_by_key,
+// cuda_neighbor_searcher.cu:90-111)
+__global__ __launch_bounds__(kBlock) void k_gather(GatherArgs a, int n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t p = a.perm[i];
+    if (p >= (uint32_t)n) p = (uint32_t)i;  // a permutation of [0, n) never takes this branch
+    a.pm_out[i] = a.pm_in[p];
+    a.vr_out[i] = a.vr_in[p];
+    a.cs_out[i] = a.cs_in[p];
+    if (a.multi) {
+        a.cda_out[i] = a.cda_in[p];
+        a.amc_out[i] = a.amc_in[p];
+    }
+}
+void launch_gather(const GatherArgs& a, int n, hipStream_t s) {
+    if (n <= 0) return;
+    k_gather<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a, n);
+}
+
+__global__ __launch_bounds__(kBlock) void k_boundary_gather(const uint32_t* __restrict__ perm,
+                                                            const float4* __restrict__ in, float4* __restrict__ out,
+                                                            int nb) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nb) return;
+    uint32_t p = perm[i];
+    if (p >= (uint32_t)nb) p = (uint32_t)i;
+    out[i] = in[p];
+}
+void launch_boundary_gather(const uint32_t* perm, const float4* bp_in, float4* bp_out, int nb, hipStream_t s) {
+    if (nb <= 0) return;
+    k_boundary_gather<<<(nb + kBlock - 1) / kBlock, kBlock, 0, s>>>(perm, bp_in, bp_out, nb);
+}
+
+// ComputeBoundaryVolume_CUDA (cuda_sph_system_gpu.cuh:18-66): bpv.w enters as
+// +-1 (label sign) and leaves as +-V_b.
+__global__ __launch_bounds__(kBlock) void k_boundary_volume(SFConst k, float4* __restrict__ bpv,
+                                                            const uint32_t* __restrict__ bcs, int nb) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nb) return;
+    const float4 pi = bpv[i];
+    int cx, cy, cz;
+    cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
+    float sum = 0.f;
+    for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+        const uint32_t e = bcs[c1];
+        for (uint32_t j = bcs[c0]; j < e; ++j) {
+            if ((int)j == i) continue;
+            const float4 pj = bpv[j];
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            sum += poly6_r2(k, dx * dx + dy * dy + dz * dz);
+        }
+    });
+    const float v = 1.f / fmaxf(sum, kEps);
+    // every thread reads only .xyz of other slots, so writing .w is race-free
+    bpv[i].w = pi.w < 0.f ? -v : v;
+}
+void launch_boundary_volume(const SFConst& k, float4* bpv, const uint32_t* bcs, int nb, hipStream_t s) {
+    if (nb <= 0) return;
+    k_boundary_volume<<<(nb + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, bpv, bcs, nb);
+}
+
+// in-range neighbour counts: r^2 = dx*dx + dy*dy + dz*dz in FP32, left to
+// right without contraction, compared with h^2
+__global__ __launch_bounds__(kBlock) void k_nbr_count(SFConst k, const float4* __restrict__ pm,
+                                                      const uint32_t* __restrict__ cs, uint32_t* __restrict__ cnt,
+                                                      uint32_t* __restrict__ cand, int n) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 pi = pm[i];
+    int cx, cy, cz;
+    cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
+    uint32_t c = 0, cc = 0;
+    for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+        const uint32_t s0 = cs[c0], e = cs[c1];
+        cc += e - s0;
+        for (uint32_t j = s0; j < e; ++j) {
+            const float4 pj = pm[j];
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float xx = __fmul_rn(dx, dx), yy = __fmul_rn(dy, dy), zz = __fmul_rn(dz, dz);
+            const float r2 = __fadd_rn(__fadd_rn(xx, yy), zz);
+            c += r2 < k.h2 ? 1u : 0u;
+        }
+    });
+    cnt[i] = c;
+    cand[i] = cc;
+}
+void launch_nbr_count(const SFConst& k, const float4* pm, const uint32_t* cs, uint32_t* cnt, uint32_t* cand, int n,
+                      hipStream_t s) {
+    if (n <= 0) return;
+    k_nbr_count<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, pm, cs, cnt, cand, n);
+}
+
+}  // namespace sf

@@ -1,0 +1,477 @@
+// sf_solver.hip -- the per-step solver passes of the coupled SPH-DEM seepage
+// step on gfx950 (one thread per particle over the cell-sorted arrays; each
+// thread walks its 27-cell neighbourhood as 9 contiguous z-ranges).
+//
+// Pass structure (dependencies of cuda_wcsph_sf_solver.cpp:44-177):
+//   P1 density + EOS            <- ComputeSFDensity_CUDA, ComputeSFPressure(ByTait)_CUDA
+//   P2 avg flow / voidage / sat <- ComputeSFAvgFlow_CUDA
+//   P3 sand voidage | water adhesion
+//                               <- ComputeSFSandVoidage_CUDA, Compute(Multi)SFWaterAdhesionForces_CUDA
+//   P4 sand momentum + colour   <- Compute(Multi)SFSandLinearMomentum_CUDA, ComputeSFWetSandColor_CUDA
+//   P5 water momentum           <- ComputeSFWaterLinearMomentum_CUDA
+//   CFL (optional), advect      <- ComputeSubTimeStepsByCFL, CudaSFParticles::Advect,
+//                                  SFWaterBoundaryConstrain_CUDA
+// ExtraForces (acc = g) is folded into P4/P5, which write the final acc.
+#include "sf_internal.h"
+
+namespace sf {
+
+// Tait EOS B((rho/rho0)^7 - 1) (cuda_wcsph_sf_solver_gpu.cuh:34) evaluated as
+// B * u * (1 + x + ... + x^6) with u = (rho - rho0) / rho0, x = rho / rho0,
+// which avoids the cancellation of x^7 - 1 near rest density.
+SF_DEV float eos(const SFConst& k, float rho) {
+    float p;
+    if (k.tait) {
+        const float x = rho / k.rho0;
+        const float u = (rho - k.rho0) / k.rho0;
+        const float s = 1.f + x * (1.f + x * (1.f + x * (1.f + x * (1.f + x * (1.f + x)))));
+        p = k.stiff * u * s;
+    } else {
+        p = k.stiff * (rho - k.rho0);  // ComputeSFPressure_CUDA (cuda_sph_sf_solver_gpu.cuh:497)
+    }
+    if (p < 0.f) p *= k.neg_scale;
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// P1: water density (self counted twice, :466 + loop) and pressure; sand gets
+// rho_sand and p = 0.
+__global__ __launch_bounds__(kBlock) void k_density(SFConst k, Fields f, int n) {
+    const int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 pi = f.pm[i];
+    if (pi.w < 0.f) {
+        f.rp[i] = make_float2(k.rho_s, 0.f);
+        return;
+    }
+    int cx, cy, cz;
+    cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
+    float sf = 0.f, sb = 0.f;
+    for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+        const uint32_t e = f.cell[c1];
+        for (uint32_t j = f.cell[c0]; j < e; ++j) {
+            const float4 pj = f.pm[j];
+            if (pj.w > 0.f) {
+                const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 < k.h2) {
+                    const float d = k.h2 - r2;
+                    sf += pj.w * d * d * d;
+                }
+            }
+        }
+        const uint32_t be = f.bcell[c1];
+        for (uint32_t j = f.bcell[c0]; j < be; ++j) {
+            const float4 b = f.bpv[j];
+            const float dx = pi.x - b.x, dy = pi.y - b.y, dz = pi.z - b.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < k.h2) {
+                const float d = k.h2 - r2;
+                sb += fabsf(b.w) * d * d * d;
+            }
+        }
+    });
+    const float rho = pi.w * k.w0 + k.w_coef * (sf + k.rho0 * sb);
+    f.rp[i] = make_float2(rho, eos(k, rho));
+}
+void launch_density(const SFConst& k, const Fields& f, int n, hipStream_t s) {
+    if (n <= 0) return;
+    k_density<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, f, n);
+}
+
+// ---------------------------------------------------------------------------
+// P2: sand volume fraction, average inter-phase flow velocity, saturation.
+__global__ __launch_bounds__(kBlock) void k_avg_flow(SFConst k, Fields f, int n) {
+    const int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 pi = f.pm[i];
+    const bool sand_i = pi.w < 0.f;
+    int cx, cy, cz;
+    cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
+    float sandv = 0.f, vsum = 0.f;
+    v3 av = mk3(0.f, 0.f, 0.f);
+    const float inv_rho_s = k.rho_s != 0.f ? 1.f / k.rho_s : 0.f;
+    for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+        const uint32_t e = f.cell[c1];
+        for (uint32_t j = f.cell[c0]; j < e; ++j) {
+            const float4 pj = f.pm[j];
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 >= k.h2) continue;
+            const float d = k.h2 - r2;
+            const float w = k.w_coef * d * d * d;
+            if (pj.w < 0.f) {  // sand j: ComputeSFSandVolume + water-side flow
+                const float vw = (-pj.w * inv_rho_s) * w;
+                sandv += vw;
+                if (!sand_i) {
+                    const float4 vj = f.vr[j];
+                    av += xyz(vj) * vw;
+                    vsum += vw;
+                }
+            } else if (sand_i) {  // water j: sand-side flow and saturation
+                const float vw = (pj.w / f.rp[j].x) * w;
+                const float4 vj = f.vr[j];
+                av += xyz(vj) * vw;
+                vsum += vw;
+            }
+        }
+    });
+    const v3 fl = vsum != 0.f ? av * (1.f / vsum) : mk3(0.f, 0.f, 0.f);
+    f.flow[i] = f4(fl, 0.f);
+    f.es[i] = make_float2(1.f - sandv, sand_i ? vsum : 0.f);
+}
+void launch_avg_flow(const SFConst& k, const Fields& f, int n, hipStream_t s) {
+    if (n <= 0) return;
+    k_avg_flow<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, f, n);
+}
+
+// ---------------------------------------------------------------------------
+// P3: sand i -> voidage from water neighbours; water i -> adhesion force.
+__global__ __launch_bounds__(kBlock) void k_voidage_adhesion(SFConst k, Fields f, int n) {
+    const int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 pi = f.pm[i];
+    const bool sand_i = pi.w < 0.f;
+    int cx, cy, cz;
+    cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
+    if (sand_i) {
+        float num = 0.f, den = 0.f;
+        for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+            const uint32_t e = f.cell[c1];
+            for (uint32_t j = f.cell[c0]; j < e; ++j) {
+                const float4 pj = f.pm[j];
+                if (pj.w < 0.f) continue;
+                const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 >= k.h2) continue;
+                const float d = k.h2 - r2;
+                const float vw = (pj.w / f.rp[j].x) * (k.w_coef * d * d * d);
+                num += f.es[j].x * vw;
+                den += vw;
+            }
+        });
+        f.es[i].x = den != 0.f ? num / den : 0.f;
+        f.adh[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    const float mi = pi.w;
+    const float inv_rho_s = k.rho_s != 0.f ? 1.f / k.rho_s : 0.f;
+    float vs = 0.f;
+    v3 F = mk3(0.f, 0.f, 0.f);
+    for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+        const uint32_t e = f.cell[c1];
+        for (uint32_t j = f.cell[c0]; j < e; ++j) {
+            const float4 pj = f.pm[j];
+            if (pj.w >= 0.f) continue;
+            const v3 dp = mk3(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
+            const float r2 = dot(dp, dp);
+            if (r2 >= k.h2) continue;
+            const float mj = -pj.w;
+            const float d = k.h2 - r2;
+            vs += (mj * inv_rho_s) * (k.w_coef * d * d * d);
+            const float l = sqrtf(r2);
+            const float A = adhesion_A(k, l);
+            if (A != 0.f) {
+                const float G = bezier(k.gadh, f.es[j].y);
+                F += dp * (G * mi * mj * A / fmaxf(kEps, l));
+            }
+        }
+    });
+    f.adh[i] = f4(F, 0.f);
+    f.xf[i] = vs != 0.f ? f4(F * (1.f / vs), 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+void launch_voidage_adhesion(const SFConst& k, const Fields& f, int n, hipStream_t s) {
+    if (n <= 0) return;
+    k_voidage_adhesion<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, f, n);
+}
+
+// ---------------------------------------------------------------------------
+// P4: sand linear momentum (DEM + capillary + buoyancy + adhesion + boundary
+// DEM + walls + drag), acc = g + 2 (f + df) / m, then the wet-sand colour.
+__global__ __launch_bounds__(kBlock) void k_sand_momentum(SFConst k, Fields f, int n) {
+    const int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 pi = f.pm[i];
+    if (pi.w >= 0.f) return;
+    const float mi = -pi.w;
+    const v3 xi = xyz(pi);
+    const v3 vi = xyz(f.vr[i]);
+    const float rho_i = f.rp[i].x;
+    const float2 esi = f.es[i];
+    const float sat_i = esi.y;
+    const float twoR = 2.f * k.R;
+    const float cap_r = twoR + k.cap_srup;
+    const float cap_r2 = cap_r * cap_r;
+    const float vol_i = mi / fmaxf(kEps, rho_i);
+    int cx, cy, cz;
+    cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
+    v3 fsum = mk3(0.f, 0.f, 0.f);
+    float vw = 0.f;
+    for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+        const uint32_t e = f.cell[c1];
+        for (uint32_t j = f.cell[c0]; j < e; ++j) {
+            const float4 pj = f.pm[j];
+            const v3 dji = mk3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+            const float r2 = dot(dji, dji);
+            if (pj.w < 0.f) {  // sand j
+                if ((int)j == i || r2 >= cap_r2) continue;
+                const float dist = sqrtf(r2);
+                if (dist < twoR) {
+                    const v3 vj = xyz(f.vr[j]);
+                    fsum += dem_contact(dji, dist, vj - vi, twoR, k.kn, k.ks, k.tan2);
+                } else {
+                    const float H = dist - twoR;
+                    if (H > 0.f && H < k.cap_srup)
+                        fsum += capillary(k, dji, dist, H, (sat_i + f.es[j].y) * 0.5f);
+                }
+            } else {  // water j
+                if (r2 >= k.h2) continue;
+                const float d = k.h2 - r2;
+                const float w = k.w_coef * d * d * d;
+                const float eps_j = f.es[j].x;
+                if (eps_j > 0.f && eps_j <= 1.f) {  // ComputeSFSandBuoyancyForces
+                    const float2 rpj = f.rp[j];
+                    const float mj = pj.w;
+                    const float dj = rpj.x / fmaxf(kEps, eps_j);
+                    vw += mj / dj * w;
+                    const float gs = spiky_s(k, sqrtf(r2));
+                    // gradW(x_i - x_j) = gs * (x_i - x_j) = -gs * dji
+                    fsum += dji * (mi * mj * rpj.y / fmaxf(kEps, rho_i * dj) * gs);
+                }
+                const float4 aj = f.xf[j];  // avgAdhesion of water j (ComputeSFSandAdhesionTerm)
+                fsum += xyz(aj) * (w * vol_i);
+            }
+        }
+        if (k.blabel1) {  // ComputeSFUniRadiusDemBoundaryForces: boundary label 1 only
+            const uint32_t be = f.bcell[c1];
+            for (uint32_t j = f.bcell[c0]; j < be; ++j) {
+                const float4 b = f.bpv[j];
+                if (b.w >= 0.f) continue;
+                fsum += dem_force(xi - xyz(b), -vi, twoR, k.kn, k.ks, k.tan2);
+            }
+        }
+    });
+    fsum += world_walls(k, xi, vi);
+    v3 df = mk3(0.f, 0.f, 0.f);
+    const float eps_i = esi.x;
+    if (eps_i < 1.f && eps_i > 0.f) {
+        const float cd = k.multi ? f.cda[i].x : k.cd;
+        const float ome = 1.f - eps_i;
+        const float dc = cd * (eps_i * eps_i * eps_i) / (ome * ome);
+        if (dc != 0.f) {
+            const v3 fl = xyz(f.flow[i]);
+            df = (fl - vi) * (k.grav_abs * k.rho0 * (eps_i * eps_i) / dc * mi / rho_i);
+        }
+    }
+    f.xf[i] = vw != 0.f ? f4(df * (1.f / vw), 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const v3 a = mk3(k.gvx, k.gvy, k.gvz) + (fsum + df) * (2.f / mi);
+    f.acc[i] = f4(a, 0.f);
+    // ComputeSFWetSandColor_CUDA (:952-977)
+    float4 c = f.cs[i];
+    if (c.w < sat_i) {
+        c.x = k.dryr + (k.wetr - k.dryr) * sat_i;
+        c.y = k.dryg + (k.wetg - k.dryg) * sat_i;
+        c.z = k.dryb + (k.wetb - k.dryb) * sat_i;
+        c.w = sat_i;
+        f.cs[i] = c;
+    }
+}
+void launch_sand_momentum(const SFConst& k, const Fields& f, int n, hipStream_t s) {
+    if (n <= 0) return;
+    k_sand_momentum<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, f, n);
+}
+
+// ---------------------------------------------------------------------------
+// P5: water linear momentum (pressure + artificial viscosity + seepage +
+// boundary pressure/viscosity), acc = g + a - F_adh / m.
+__global__ __launch_bounds__(kBlock) void k_water_momentum(SFConst k, Fields f, int n) {
+    const int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 pi = f.pm[i];
+    if (pi.w < 0.f) return;
+    const float mi = pi.w;
+    const v3 vi = xyz(f.vr[i]);
+    const float2 rpi = f.rp[i];
+    const float eps_i = f.es[i].x;
+    const float rt_i = rpi.x / fmaxf(kEps, eps_i);
+    const float pterm_i = rpi.y / fmaxf(kEps, rt_i * rt_i);
+    const float inv_rt_i = 1.f / rt_i;
+    const float bpress = -k.rho0 * (rpi.y / fmaxf(kEps, k.rho0 * k.rho0));
+    const float bvisc = -k.bnu / (2.f * k.rho0);
+    int cx, cy, cz;
+    cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
+    v3 a = mk3(0.f, 0.f, 0.f);
+    for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+        const uint32_t e = f.cell[c1];
+        for (uint32_t j = f.cell[c0]; j < e; ++j) {
+            const float4 pj = f.pm[j];
+            const v3 d = mk3(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
+            const float r2 = dot(d, d);
+            if (r2 >= k.h2) continue;
+            const float gs = spiky_s(k, sqrtf(r2));
+            if (pj.w > 0.f) {  // water j
+                if ((int)j == i) continue;
+                const float2 rpj = f.rp[j];
+                const float rt_j = rpj.x / fmaxf(kEps, f.es[j].x);
+                float s = -pj.w * (pterm_i + rpj.y / fmaxf(kEps, rt_j * rt_j));
+                const v3 dv = vi - xyz(f.vr[j]);
+                const float dd = dot(dv, d);
+                if (dd < 0.f) {  // ComputeSFWaterArtificialViscosity
+                    const float pij = -k.nu / (rt_i + rt_j) * (dd / (r2 + kEps));
+                    s += -pj.w * pij;
+                }
+                a += d * (s * gs);
+            } else {  // sand j: ComputeSFWaterSeepageTerm
+                const float mj = -pj.w;
+                const float rho_j = f.rp[j].x;
+                const float dk = k.h2 - r2;
+                const float w = k.w_coef * dk * dk * dk;
+                a += d * (-mj * rpi.y / fmaxf(kEps, rt_i * rho_j) * gs);
+                a += xyz(f.xf[j]) * (-w * inv_rt_i);
+            }
+        }
+        const uint32_t be = f.bcell[c1];
+        for (uint32_t j = f.bcell[c0]; j < be; ++j) {
+            const float4 b = f.bpv[j];
+            const v3 d = mk3(pi.x - b.x, pi.y - b.y, pi.z - b.z);
+            const float r2 = dot(d, d);
+            if (r2 >= k.h2) continue;
+            const float gs = spiky_s(k, sqrtf(r2));
+            const float vb = fabsf(b.w);
+            float s = vb * bpress;  // ComputeBoundaryPressure (densityi = rho0, :944)
+            const float dd = dot(vi, d);
+            if (dd < 0.f) s += -vb * k.rho0 * (bvisc * (dd / (r2 + kEps)));  // ComputeBoundaryViscosity
+            a += d * (s * gs);
+        }
+    });
+    const v3 ad = xyz(f.adh[i]);
+    const v3 acc = mk3(k.gvx, k.gvy, k.gvz) + a - ad * (1.f / mi);
+    f.acc[i] = f4(acc, 0.f);
+}
+void launch_water_momentum(const SFConst& k, const Fields& f, int n, hipStream_t s) {
+    if (n <= 0) return;
+    k_water_momentum<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, f, n);
+}
+
+// ---------------------------------------------------------------------------
+// CFL sub-step count (ComputeSubTimeStepsByCFL, cuda_wcsph_sf_solver.cu:35-58),
+// computed on the device so the step needs no host round trip.
+constexpr int kCflBlocks = 1024;
+int cfl_partials(int) { return kCflBlocks; }
+
+__global__ __launch_bounds__(kBlock) void k_cfl_partial(const float4* __restrict__ acc, int n, float* __restrict__ part) {
+#pragma clang fp contract(off)
+    __shared__ float sm[kBlock / 64];
+    float m = 0.f;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const float4 a = acc[i];
+        const float xx = a.x * a.x, yy = a.y * a.y, zz = a.z * a.z;
+        m = fmaxf(m, (xx + yy) + zz);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_down(m, o, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = sm[0];
+        for (int q = 1; q < kBlock / 64; ++q) r = fmaxf(r, sm[q]);
+        part[blockIdx.x] = r;
+    }
+}
+__global__ __launch_bounds__(kBlock) void k_cfl_final(const float* __restrict__ part, int np, CflOut* out, float h,
+                                                      float render_interval, float m, float c, float scale,
+                                                      float params_dt) {
+    __shared__ float sm[kBlock / 64];
+    float r = 0.f;
+    for (int q = threadIdx.x; q < np; q += kBlock) r = fmaxf(r, part[q]);
+    for (int o = 32; o > 0; o >>= 1) r = fmaxf(r, __shfl_down(r, o, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float mr = sm[0];
+        for (int q = 1; q < kBlock / 64; ++q) mr = fmaxf(mr, sm[q]);
+        const float amax = sqrtf(mr);
+        const float fmax_ = amax * m;
+        const float dts = 0.4f * h / c;
+        const float dtf = 0.25f * sqrtf(h * m / fmax_);
+        const float inner = dtf < dts ? dtf : dts;
+        const float lim = scale * inner;
+        const float dt = params_dt < lim ? params_dt : lim;
+        const float q = render_interval / dt;
+        uint32_t k = q >= 4294967295.f ? 0xffffffffu : (q >= 1.f ? (uint32_t)q : 1u);
+        CflOut o;
+        o.dt = dt;
+        o.nsub = k;
+        o.amax = amax;
+        o.pad = 0;
+        *out = o;
+    }
+}
+void launch_cfl(const SFConst& k, const float4* acc, int n, float* partials, CflOut* out, float render_interval,
+                float sph_mass, float speed_of_sound, float scale, float params_dt, hipStream_t s) {
+    int nb = (n + kBlock - 1) / kBlock;
+    if (nb > kCflBlocks) nb = kCflBlocks;
+    if (nb < 1) nb = 1;
+    k_cfl_partial<<<nb, kBlock, 0, s>>>(acc, n, partials);
+    k_cfl_final<<<1, kBlock, 0, s>>>(partials, nb, out, k.h, render_interval, sph_mass, speed_of_sound, scale,
+                                      params_dt);
+}
+
+// ---------------------------------------------------------------------------
+// advect: sand damping (AccDampingForSand), symplectic Euler, then the water
+// box clamp and label-1 boundary push (SFWaterBoundaryConstrain_CUDA).
+SF_DEV float sgnf(float v) { return (float)((0.f < v) - (v < 0.f)); }
+__global__ __launch_bounds__(kBlock) void k_advect(SFConst k, AdvectArgs a, int n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float dt = a.dt_dev ? a.render_interval / (float)a.dt_dev->nsub : a.dt;
+    const float4 p = a.pm_in[i];
+    const float4 v = a.vr_in[i];
+    float4 ac = a.acc[i];
+    if (p.w < 0.f) {
+        const float qx = ac.x * (v.x + 0.5f * dt * ac.x);
+        const float qy = ac.y * (v.y + 0.5f * dt * ac.y);
+        const float qz = ac.z * (v.z + 0.5f * dt * ac.z);
+        ac.x *= 1.f - k.damping * sgnf(qx);
+        ac.y *= 1.f - k.damping * sgnf(qy);
+        ac.z *= 1.f - k.damping * sgnf(qz);
+        a.acc[i] = ac;  // the reference stores the damped acceleration
+    }
+    v3 nv = mk3(v.x + dt * ac.x, v.y + dt * ac.y, v.z + dt * ac.z);
+    v3 np = mk3(p.x + dt * nv.x, p.y + dt * nv.y, p.z + dt * nv.z);
+    if (p.w > 0.f) {
+        const v3 pre = np;
+        const float r2 = 2.f * k.rw;
+        if (np.x > k.hix - r2) { np.x = k.hix - r2; nv.x = fminf(nv.x, 0.f); }
+        if (np.x < k.lox + r2) { np.x = k.lox + r2; nv.x = fmaxf(nv.x, 0.f); }
+        if (np.y > k.hiy - r2) { np.y = k.hiy - r2; nv.y = fminf(nv.y, 0.f); }
+        if (np.y < k.loy + r2) { np.y = k.loy + r2; nv.y = fmaxf(nv.y, 0.f); }
+        if (np.z > k.hiz - r2) { np.z = k.hiz - r2; nv.z = fminf(nv.z, 0.f); }
+        if (np.z < k.loz + r2) { np.z = k.loz + r2; nv.z = fmaxf(nv.z, 0.f); }
+        if (k.blabel1) {
+            int cx, cy, cz;
+            cell_of(k, pre.x, pre.y, pre.z, cx, cy, cz);
+            for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+                const uint32_t be = a.bcell[c1];
+                for (uint32_t j = a.bcell[c0]; j < be; ++j) {
+                    const float4 b = a.bpv[j];
+                    if (b.w >= 0.f) continue;
+                    const v3 dp = pre - xyz(b);
+                    const float dl = sqrtf(dot(dp, dp));
+                    const float ov = dl - r2;
+                    if (ov < 0.f) {
+                        nv = nv * 0.f;
+                        np += dp * (1.f / dl) * (-ov);
+                    }
+                }
+            });
+        }
+    }
+    a.pm_out[i] = f4(np, p.w);
+    a.vr_out[i] = f4(nv, v.w);
+}
+void launch_advect(const SFConst& k, const AdvectArgs& a, int n, hipStream_t s) {
+    if (n <= 0) return;
+    k_advect<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, a, n);
+}
+
+}  // namespace sf

@@ -1,0 +1,181 @@
+"""Deterministic synthetic scenes (numpy, FP32) for the seepage step.
+
+Lattices are filled in (i, j, k) order with i (x) outermost, as
+CudaVolumeEmitter::BuildSphVolume does (cuda_volume_emitter.cpp:15-34), at a
+spacing of 2r from `origin`, and truncated to the requested count.  Jitter is
+per component U(-a, a) from a counter-based SplitMix64 of (seed, stream,
+particle, component), so the same scene comes out on every host and thread
+count.  (The reference's own jitter draws from std::random_device and is not
+reproducible, cuda_volume_emitter.cpp:41-54.)
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import params as P
+from ._abi import SF_TYPE_MULTI_SF, check, lib, ptr, sf_float3
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    x = x.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        z = x
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed: int, stream: int, count: int, comps: int) -> np.ndarray:
+    """U[0,1) float32 array (count, comps) from (seed, stream, index, component)."""
+    idx = np.arange(count * comps, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        key = (np.uint64(seed) * np.uint64(0xD1B54A32D192ED03)) ^ (np.uint64(stream) << np.uint64(40)) ^ idx
+    u = splitmix64(key) >> np.uint64(40)  # 24 random bits
+    return (u.astype(np.float32) * np.float32(1.0 / (1 << 24))).reshape(count, comps)
+
+
+def normal(seed: int, stream: int, count: int, comps: int) -> np.ndarray:
+    u1 = uniform(seed, stream, count, comps).astype(np.float64)
+    u2 = uniform(seed, stream + 1000003, count, comps).astype(np.float64)
+    return (np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2 * np.pi * u2)).astype(np.float32)
+
+
+def lattice(origin, spacing, counts, n, jitter=0.0, seed=0, stream=0) -> np.ndarray:
+    nx, ny, nz = counts
+    if nx * ny * nz < n:
+        raise ValueError(f"lattice {counts} holds {nx * ny * nz} < {n} points")
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ijk = np.stack([i.ravel(), j.ravel(), k.ravel()], 1)[:n].astype(np.float32)
+    o = np.asarray(origin, np.float32)
+    p = o + ijk * np.float32(spacing)
+    if jitter:
+        p = p + (uniform(seed, stream, n, 3) * np.float32(2.0) - np.float32(1.0)) * np.float32(jitter)
+    return p.astype(np.float32)
+
+
+@dataclass
+class Scene:
+    name: str
+    physics: P.Physics
+    pos: np.ndarray
+    label: np.ndarray
+    mass: np.ndarray
+    radius: np.ndarray
+    col: np.ndarray
+    vel: np.ndarray | None = None
+    cda0asat: np.ndarray | None = None
+    amcamcp: np.ndarray | None = None
+    bpos: np.ndarray | None = None
+    blabel: np.ndarray | None = None
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.pos.shape[0])
+
+    @property
+    def nb(self) -> int:
+        return 0 if self.bpos is None else int(self.bpos.shape[0])
+
+    def params(self):
+        return P.make_params(self.physics)
+
+
+def world_boundary(lo, hi, r) -> tuple[np.ndarray, np.ndarray]:
+    """CudaBoundaryEmitter::BuildWorldBoundary through the C-ABI host helper."""
+    L = lib()
+    import ctypes as C
+    n = C.c_size_t(0)
+    check(L.sf_build_world_boundary(sf_float3.of(lo), sf_float3.of(hi), float(np.float32(r)), 0, None, None, n))
+    pos = np.zeros((n.value, 3), np.float32)
+    lab = np.zeros(n.value, np.uint64)
+    check(L.sf_build_world_boundary(sf_float3.of(lo), sf_float3.of(hi), float(np.float32(r)), n.value, ptr(pos),
+                                    ptr(lab), n))
+    return pos, lab
+
+
+def _assemble(name, ph, blocks, vel=None, boundary=True, extra_b=None, meta=None):
+    """blocks: list of (positions, label)."""
+    p, prm = ph, P.make_params(ph)[0]
+    pos = np.concatenate([b[0] for b in blocks]).astype(np.float32)
+    label = np.concatenate([np.full(len(b[0]), b[1], np.uint64) for b in blocks])
+    water = label == 0
+    mass = np.where(water, np.float32(prm.sph_mass), np.float32(prm.dem_mass)).astype(np.float32)
+    radius = np.where(water, np.float32(prm.sph_particle_radius), np.float32(prm.dem_particle_radius)).astype(np.float32)
+    col = np.where(water[:, None], np.array(P.WATER_COLOR, np.float32), np.array(P.DRY_SAND, np.float32)).astype(np.float32)
+    cda = np.tile(np.array([p.sf_cd, p.sf_a0, p.sf_asat], np.float32), (len(pos), 1))
+    amc = np.tile(np.array([p.sf_amc, p.sf_amc_p], np.float32), (len(pos), 1))
+    bpos = blab = None
+    if boundary:
+        bpos, blab = world_boundary(p.lo, p.hi, p.r_water)
+        if extra_b is not None:
+            bpos = np.concatenate([bpos, extra_b[0]]).astype(np.float32)
+            blab = np.concatenate([blab, extra_b[1]]).astype(np.uint64)
+    return Scene(name, ph, np.ascontiguousarray(pos), label, mass, radius, np.ascontiguousarray(col),
+                 None if vel is None else np.ascontiguousarray(vel, np.float32), cda, amc, bpos, blab, meta or {})
+
+
+def small_block(seed: int = 0, sf_type: int = 0) -> Scene:
+    """Plain method, small: 16,384 water + 8,192 sand in a 1 m box (r = 0.01,
+    h = 4r, dt = 1e-4).  Water: 15 x 25 x 44 lattice at x < 0.3, y < 0.5
+    (no jitter).  Sand: 15 x 13 x 44 lattice at x in [0.5, 0.8], jitter
+    U(-0.1r, 0.1r) from `seed`."""
+    ph = P.Physics(sf_type=sf_type)
+    r = ph.r_water
+    water = lattice((r, r, r), 2 * r, (15, 25, 44), 16384)
+    rs = ph.r_sand
+    sand = lattice((0.5 + rs, rs, rs), 2 * rs, (15, 13, 44), 8192, jitter=0.1 * rs, seed=seed, stream=1)
+    return _assemble("small_block_24k", ph, [(water, 0), (sand, 1)], meta={"seed": seed})
+
+
+def drop_in_1m(seed: int = 1, n_water: int = 600_000, n_sand: int = 400_000) -> Scene:
+    """Plain GPU drop-in at scale: 600k water + 400k sand in a 2 x 1 x 1 m box.
+    The stated volumes hold only 60k + 20k grains at r = 0.01, so the radii
+    shrink until the lattices fit: water column 0.6 x 0.8 x 1 m at 2r = 0.0092
+    (65 x 86 x 108), sand pile 0.4 x 0.4 x 1 m at x in [0.8, 1.2] with
+    2r = 0.0072 (55 x 55 x 138); h = 4 r_water = 0.0184.  Both lattices get
+    U(-0.1r, 0.1r) jitter from `seed`."""
+    rw, rs = 0.0046, 0.0036
+    ph = P.Physics(hi=(2.0, 1.0, 1.0), r_water=rw, r_sand=rs)
+    water = lattice((rw, rw, rw), 2 * rw, (65, 86, 108), n_water, jitter=0.1 * rw, seed=seed, stream=0)
+    sand = lattice((0.8 + rs, rs, rs), 2 * rs, (55, 55, 138), n_sand, jitter=0.1 * rs, seed=seed, stream=1)
+    return _assemble("drop_in_1m", ph, [(water, 0), (sand, 1)], meta={"seed": seed})
+
+
+def mixed(seed: int = 7, sf_type: int = 0, n_water: int = 1440, n_sand: int = 1152, collider: bool = True,
+          vel_sigma: float = 0.2, box: float = 0.5) -> Scene:
+    """Parity scene that exercises every coupling term in one step: a water
+    block and a sand block interpenetrate (the sand lattice is offset by r),
+    both touch the walls, all particles carry random velocities, and a small
+    sphere of label-1 collider boundary particles sits in the overlap."""
+    ph = P.Physics(hi=(box, box, box), sf_type=sf_type)
+    r = ph.r_water
+    water = lattice((0.05, 2 * r, 2 * r), 2 * r, (12, 10, 12), n_water, jitter=0.2 * r, seed=seed, stream=0)
+    sand = lattice((0.2 + r, r, r), 2 * r, (12, 8, 12), n_sand, jitter=0.1 * r, seed=seed, stream=1)
+    vel = normal(seed, 7, n_water + n_sand, 3) * np.float32(vel_sigma)
+    extra = None
+    if collider:
+        # Fibonacci sphere of 120 label-1 boundary particles, radius 0.04, centre in the overlap
+        k = np.arange(120, dtype=np.float64) + 0.5
+        phi = np.arccos(1 - 2 * k / 120)
+        th = np.pi * (1 + 5 ** 0.5) * k
+        c = np.array([0.27, 0.12, 0.2])
+        sp = c + 0.04 * np.stack([np.cos(th) * np.sin(phi), np.cos(phi), np.sin(th) * np.sin(phi)], 1)
+        extra = (sp.astype(np.float32), np.ones(120, np.uint64))
+    sc = _assemble(f"mixed_{n_water + n_sand}", ph, [(water, 0), (sand, 1)], vel=vel, extra_b=extra,
+                   meta={"seed": seed})
+    if sf_type == SF_TYPE_MULTI_SF:
+        u = uniform(seed, 11, sc.n, 1)[:, 0]
+        sc.cda0asat[:, 0] = np.float32(0.2) + u  # per-particle drag coefficient
+    # nonzero saturation history so the colour / maxSat branch is exercised both ways
+    return sc
+
+
+SCENES = {"small_block": small_block, "drop_in_1m": drop_in_1m, "mixed": mixed}
