@@ -39,13 +39,13 @@ def pass_bytes(n_w: int, n_s: int, nb: int, cells: int, passes: int, multi: bool
     return {
         "hash": n * (16 + 4),
         "sort": n * (4 + 16) * passes - n * 4,          # upsweep key read + key/value read+write per pass
-        "cell_start": 4 * (cells + 1) + 4 * n,
-        "gather": n * (4 + 2 * (48 + aux)),
-        "density": n * (16 + 8) + nb * 16 + 4 * 2 * (cells + 1),
-        "avg_flow": n * (16 + 16 + 8 + 16 + 8) + 4 * (cells + 1),
-        "voidage_adhesion": n * (16 + 8 + 8) + n_s * 4 + n_w * 32 + n_s * 16 + 4 * (cells + 1),
-        "sand_momentum": n * (16 + 16 + 8 + 8) + n_w * 16 + n_s * (16 + 16 + 16 + 16 + 16) + 4 * (cells + 1),
-        "water_momentum": n * (16 + 16 + 8 + 8) + n_s * 16 + n_w * (16 + 16) + nb * 16 + 4 * 2 * (cells + 1),
+        "gather": n * (4 + 2 * (48 + aux)) + n * 16 * 2 + 4 * (n + 1),  # + water-prefix scan (2 reads of pm.w)
+        "cell_start": 4 * (cells + 1) * 3 + 4 * n,                          # cellStart + (start, prefix) table
+        "density": n * (16 + 8) + nb * 16 + 8 * (cells + 1) + 4 * (cells + 1),
+        "avg_flow": n * (16 + 16 + 8 + 16 + 8) + n_w * 8 + 8 * (cells + 1),
+        "voidage_adhesion": n * (16 + 8 + 8) + n_s * 4 + n_w * 32 + n_s * 16 + 8 * (cells + 1),
+        "sand_momentum": n * (16 + 16 + 8 + 8) + n_w * 16 + n_s * (16 + 16 + 16 + 16 + 16) + 8 * (cells + 1),
+        "water_momentum": n * (16 + 16 + 8) + n_s * 16 + n_w * (8 + 16 + 16) + nb * 16 + 8 * (cells + 1) + 4 * (cells + 1),
         "advect": n * (16 * 3 + 16 * 2) + n_s * 16,
     }
 
@@ -77,7 +77,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="drop_in_1m", choices=["drop_in_1m", "small_block"])
     args = ap.parse_args()

@@ -201,7 +201,7 @@ int sf_update_system(sf_handle h, float render_interval, float* ms_out);
 int sf_num_sub_steps(sf_handle h, size_t* out);
 /* Passes of one step, in launch order, for sf_step_profiled. */
 enum sf_pass {
-    SF_PASS_HASH = 0, SF_PASS_SORT = 1, SF_PASS_CELL_START = 2, SF_PASS_GATHER = 3,
+    SF_PASS_HASH = 0, SF_PASS_SORT = 1, SF_PASS_GATHER = 2, SF_PASS_CELL_START = 3,
     SF_PASS_DENSITY = 4, SF_PASS_AVG_FLOW = 5, SF_PASS_VOIDAGE_ADHESION = 6,
     SF_PASS_SAND_MOMENTUM = 7, SF_PASS_WATER_MOMENTUM = 8, SF_PASS_ADVECT = 9,
     SF_PASS_COUNT = 10

@@ -39,7 +39,7 @@ FIELDS = {
 }
 
 
-PASSES = ["hash", "sort", "cell_start", "gather", "density", "avg_flow", "voidage_adhesion", "sand_momentum",
+PASSES = ["hash", "sort", "gather", "cell_start", "density", "avg_flow", "voidage_adhesion", "sand_momentum",
           "water_momentum", "advect"]
 
 

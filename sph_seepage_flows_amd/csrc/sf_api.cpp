@@ -130,12 +130,13 @@ struct sf_system {
     int cur_cs = 0;   // cs
     int cur_aux = 0;  // cda / amc
     // searcher
-    DevBuf<uint32_t> keys, kA, kB, vA, vB, hist, cell, bcell;
+    DevBuf<uint32_t> keys, kA, kB, vA, vB, hist, cell, bcell, wpre, wpart;
+    DevBuf<uint2> ct;
     size_t sort_cap = 0;
     const uint32_t* sorted_keys = nullptr;
     const uint32_t* perm = nullptr;
     // transient fields
-    DevBuf<float2> rp, es;
+    DevBuf<float2> rp, es, wt;
     DevBuf<float4> flow, adh, xf, acc;
     // boundary
     DevBuf<float4> bpv;
@@ -156,11 +157,12 @@ struct sf_system {
         f.vr = vr[cur].p;
         f.cs = cs[cur_cs].p;
         f.cda = cda[cur_aux].p;
-        f.cell = cell.p;
+        f.ct = ct.p;
         f.bpv = bpv.p;
         f.bcell = bcell.p;
         f.rp = rp.p;
         f.es = es.p;
+        f.wt = wt.p;
         f.flow = flow.p;
         f.adh = adh.p;
         f.xf = xf.p;
@@ -180,6 +182,7 @@ int ensure_sort_capacity(sf_system* h, size_t m) {
         return rc;
     size_t nblocks = (c + kRsTile - 1) / kRsTile;
     if ((rc = h->hist.alloc(256 * nblocks))) return rc;
+    if ((rc = h->wpre.alloc(c + 1)) || (rc = h->wpart.alloc((size_t)wprefix_blocks((int)c + 1)))) return rc;
     h->sort_cap = c;
     return SF_OK;
 }
@@ -203,8 +206,6 @@ int build_neighbors(sf_system* h, hipEvent_t* ev = nullptr) {
     SF_MARK(ev, SF_PASS_SORT);
     int rc = radix_sort(h->keys.p, n, h->plan, sb, &h->sorted_keys, &h->perm, s);
     if (rc) return rc;
-    SF_MARK(ev, SF_PASS_CELL_START);
-    launch_cell_start(h->sorted_keys, n, h->C, h->cell.p, s);
     GatherArgs g;
     g.perm = h->perm;
     g.pm_in = h->pm[h->cur].p;
@@ -220,6 +221,9 @@ int build_neighbors(sf_system* h, hipEvent_t* ev = nullptr) {
     g.multi = h->K.multi;
     SF_MARK(ev, SF_PASS_GATHER);
     launch_gather(g, n, s);
+    launch_wprefix(g.pm_out, n, h->wpart.p, h->wpre.p, s);
+    SF_MARK(ev, SF_PASS_CELL_START);
+    launch_cell_start(h->sorted_keys, n, h->C, h->wpre.p, h->cell.p, h->ct.p, s);
     h->cur ^= 1;
     h->cur_cs ^= 1;
     if (h->K.multi) h->cur_aux ^= 1;
@@ -310,7 +314,7 @@ int sf_create(const sf_params* p, const sf_boundary_params* b, const sf_solver_o
     *out = nullptr;
     int rc = validate(p, b);
     if (rc) return rc;
-    if (max_num > (size_t)0x7fffffff - kRsTile) return set_error(SF_E_CAPACITY, "max_num above 2^31");
+    if (max_num > ((size_t)1 << 27)) return set_error(SF_E_CAPACITY, "max_num above 2^27 particles");
     sf_system* h = new sf_system();
     h->P = *p;
     h->B = *b;
@@ -346,10 +350,12 @@ int sf_create(const sf_params* p, const sf_boundary_params* b, const sf_solver_o
             (rc = h->cda[q].alloc(c)) || (rc = h->amc[q].alloc(c)))
             return bail(rc);
     }
-    if ((rc = h->rp.alloc(c)) || (rc = h->es.alloc(c)) || (rc = h->flow.alloc(c)) || (rc = h->adh.alloc(c)) ||
+    if ((rc = h->rp.alloc(c)) || (rc = h->es.alloc(c)) || (rc = h->wt.alloc(c)) || (rc = h->flow.alloc(c)) || (rc = h->adh.alloc(c)) ||
         (rc = h->xf.alloc(c)) || (rc = h->acc.alloc(c)))
         return bail(rc);
-    if ((rc = h->cell.alloc((size_t)h->C + 1)) || (rc = h->bcell.alloc((size_t)h->C + 1))) return bail(rc);
+    if ((rc = h->cell.alloc((size_t)h->C + 1)) || (rc = h->bcell.alloc((size_t)h->C + 1)) ||
+        (rc = h->ct.alloc((size_t)h->C + 1)))
+        return bail(rc);
     if ((rc = h->cfl_part.alloc(cfl_partials(0))) || (rc = h->cfl_out.alloc(1))) return bail(rc);
     if ((rc = ensure_sort_capacity(h, c))) return bail(rc);
     if ((rc = h->bpv.alloc(1))) return bail(rc);
@@ -361,6 +367,7 @@ int sf_create(const sf_params* p, const sf_boundary_params* b, const sf_solver_o
     }
     if ((rc = zero(h, h->rp)) || (rc = zero(h, h->es)) || (rc = zero(h, h->flow)) || (rc = zero(h, h->adh)) ||
         (rc = zero(h, h->xf)) || (rc = zero(h, h->acc)) || (rc = zero(h, h->cell)) || (rc = zero(h, h->bcell)) ||
+        (rc = zero(h, h->ct)) || (rc = zero(h, h->wt)) || (rc = zero(h, h->wpre)) ||
         (rc = zero(h, h->keys)) || (rc = zero(h, h->bpv)))
         return bail(rc);
     {
@@ -448,7 +455,7 @@ int sf_upload_boundary(sf_handle h, size_t nb, const float* pos3, const uint64_t
     int rc;
     if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
     if (nb > 0 && !pos3) return set_error(SF_E_INVALID, "null boundary positions");
-    if (nb > (size_t)0x7fffffff - kRsTile) return set_error(SF_E_CAPACITY, "too many boundary particles");
+    if (nb > ((size_t)1 << 27)) return set_error(SF_E_CAPACITY, "more than 2^27 boundary particles");
     std::vector<float4> bp(nb);
     int l1 = 0;
     for (size_t i = 0; i < nb; ++i) {
@@ -467,7 +474,7 @@ int sf_upload_boundary(sf_handle h, size_t nb, const float* pos3, const uint64_t
     SortBuffers sb{h->kA.p, h->vA.p, h->kB.p, h->vB.p, h->hist.p};
     const uint32_t *sk = nullptr, *bperm = nullptr;
     if ((rc = radix_sort(h->keys.p, (int)nb, h->plan, sb, &sk, &bperm, h->stream))) return rc;
-    launch_cell_start(sk, (int)nb, h->C, h->bcell.p, h->stream);
+    launch_cell_start(sk, (int)nb, h->C, nullptr, h->bcell.p, nullptr, h->stream);
     launch_boundary_gather(bperm, tmp.p, h->bpv.p, (int)nb, h->stream);
     launch_boundary_volume(h->K, h->bpv.p, h->bcell.p, (int)nb, h->stream);
     SF_HIP(hipGetLastError());

@@ -37,6 +37,20 @@ SF_DEV v3& operator+=(v3& a, v3 b) {
 SF_DEV float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 SF_DEV float4 f4(v3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
 
+// Loads through a 32-bit byte offset from a uniform base, so the compiler can
+// use the SGPR-base + VGPR-offset addressing form (no 64-bit VALU address
+// math per candidate).  Particle indices stay below 2^27 (checked at create).
+template <class T>
+SF_DEV T ldo(const T* __restrict__ base, uint32_t j) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (j * (uint32_t)sizeof(T)));
+}
+
+// Fast reciprocal / square root (1 ulp hardware ops) for the physics; the
+// grid hash keeps IEEE division.
+SF_DEV float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+SF_DEV float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+SF_DEV float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // Per-scene constants, precomputed on the host in FP32 the way the reference's
 // functor constructors compute them (cuda_sph_kernel.cuh:79-118,189-203,
 // cuda_sf_utils.cuh:36-57, cuda_dem_solver_common_gpu.cuh:63-65).

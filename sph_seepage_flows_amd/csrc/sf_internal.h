@@ -73,7 +73,9 @@ int radix_sort(const uint32_t* keys, int n, const RadixPlan& plan, const SortBuf
 
 // ---- searcher launchers (sf_searcher.hip) ----
 void launch_hash(const SFConst& k, const float4* pm, uint32_t* keys, int n, hipStream_t s);
-void launch_cell_start(const uint32_t* sorted_keys, int n, uint32_t C, uint32_t* cell_start, hipStream_t s);
+// cellStart[c] for c in [0, C] plus the cell table ct[c] = (cellStart[c], wpre[cellStart[c]])
+void launch_cell_start(const uint32_t* sorted_keys, int n, uint32_t C, const uint32_t* wpre, uint32_t* cell_start,
+                       uint2* ct, hipStream_t s);
 struct GatherArgs {
     const uint32_t* perm;
     const float4 *pm_in, *vr_in, *cs_in, *cda_in;
@@ -88,6 +90,9 @@ void launch_boundary_volume(const SFConst& k, float4* bpv, const uint32_t* bcs, 
 void launch_nbr_count(const SFConst& k, const float4* pm, const uint32_t* cs, uint32_t* cnt, uint32_t* cand, int n,
                       hipStream_t s);
 void launch_hash_boundary(const SFConst& k, const float4* bp, uint32_t* keys, int nb, hipStream_t s);
+// wpre[0..n]: water count among sorted slots [0, i); part holds wprefix_blocks(n + 1) entries
+int wprefix_blocks(int n);
+void launch_wprefix(const float4* pm, int n, uint32_t* part, uint32_t* wpre, hipStream_t s);
 
 // ---- solver launchers (sf_solver.hip) ----
 struct Fields {
@@ -95,11 +100,12 @@ struct Fields {
     const float4* vr;
     float4* cs;
     const float4* cda;
-    const uint32_t* cell;
+    const uint2* ct;  // (cellStart[c], water prefix at cellStart[c]), c in [0, C]
     const float4* bpv;
     const uint32_t* bcell;
     float2* rp;
     float2* es;
+    float2* wt;  // water: (rho / max(eps, voidage), p / max(eps, (rho / max(eps, voidage))^2))
     float4* flow;
     float4* adh;
     float4* xf;

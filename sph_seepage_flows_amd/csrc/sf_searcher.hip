@@ -217,7 +217,8 @@ int radix_sort(const uint32_t* keys, int n, const RadixPlan& plan, const SortBuf
 // cellStart[c] = number of particles with key < c, c in [0, C]
 // (CountingInCell_CUDA + exclusive_scan, cuda_neighbor_searcher.cu:45-47)
 __global__ __launch_bounds__(kBlock) void k_cell_start(const uint32_t* __restrict__ sk, int n, uint32_t C,
-                                                       uint32_t* __restrict__ cs) {
+                                                       const uint32_t* __restrict__ wpre, uint32_t* __restrict__ cs,
+                                                       uint2* __restrict__ ct) {
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c > C) return;
     uint32_t lo = 0, hi = (uint32_t)n;
@@ -227,10 +228,12 @@ __global__ __launch_bounds__(kBlock) void k_cell_start(const uint32_t* __restric
         else hi = mid;
     }
     cs[c] = lo;
+    if (ct) ct[c] = make_uint2(lo, wpre ? wpre[lo] : 0u);
 }
-void launch_cell_start(const uint32_t* sorted_keys, int n, uint32_t C, uint32_t* cell_start, hipStream_t s) {
+void launch_cell_start(const uint32_t* sorted_keys, int n, uint32_t C, const uint32_t* wpre, uint32_t* cell_start,
+                       uint2* ct, hipStream_t s) {
     const uint32_t total = C + 1;
-    k_cell_start<<<(total + kBlock - 1) / kBlock, kBlock, 0, s>>>(sorted_keys, n, C, cell_start);
+    k_cell_start<<<(total + kBlock - 1) / kBlock, kBlock, 0, s>>>(sorted_keys, n, C, wpre, cell_start, ct);
 }
 
 // ---------------------------------------------------------------------------
@@ -294,6 +297,75 @@ __global__ __launch_bounds__(kBlock) void k_boundary_volume(SFConst k, float4* _
 void launch_boundary_volume(const SFConst& k, float4* bpv, const uint32_t* bcs, int nb, hipStream_t s) {
     if (nb <= 0) return;
     k_boundary_volume<<<(nb + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, bpv, bcs, nb);
+}
+
+// ---------------------------------------------------------------------------
+// wpre[i] = number of water particles among sorted slots [0, i), i in [0, n]:
+// lets a neighbour loop see how many water / sand particles a contiguous
+// cell range holds (wpre[e] - wpre[s]) and skip ranges without the phase it
+// needs.  Two kernels around a tiny scan of per-block counts.
+constexpr int kWpItems = 8;
+constexpr int kWpTile = kBlock * kWpItems;
+int wprefix_blocks(int n) { return (n + kWpTile - 1) / kWpTile; }
+
+__global__ __launch_bounds__(kBlock) void k_wcount(const float4* __restrict__ pm, int n, uint32_t* __restrict__ part) {
+    __shared__ uint32_t ws[kBlock / 64];
+    const int t = threadIdx.x;
+    const int base = blockIdx.x * kWpTile;
+    uint32_t c = 0;
+#pragma unroll
+    for (int q = 0; q < kWpItems; ++q) {
+        const int i = base + q * kBlock + t;
+        c += (i < n && pm[i].w > 0.f) ? 1u : 0u;
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+    if ((t & 63) == 0) ws[t >> 6] = c;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t s = 0;
+        for (int q = 0; q < kBlock / 64; ++q) s += ws[q];
+        part[blockIdx.x] = s;
+    }
+}
+// part is exclusive-scanned; each block writes its slots' prefixes (thread t
+// owns kWpItems consecutive slots); slot n gets the total.
+__global__ __launch_bounds__(kBlock) void k_wprefix(const float4* __restrict__ pm, int n,
+                                                    const uint32_t* __restrict__ part, uint32_t* __restrict__ wpre) {
+    __shared__ uint32_t wsum[kBlock / 64];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int base = blockIdx.x * kWpTile + t * kWpItems;
+    uint32_t f[kWpItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int q = 0; q < kWpItems; ++q) {
+        const int i = base + q;
+        f[q] = (i < n && pm[i].w > 0.f) ? 1u : 0u;
+        s += f[q];
+    }
+    uint32_t x = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t run = part[blockIdx.x] + x - s;
+    for (int q = 0; q < w; ++q) run += wsum[q];
+#pragma unroll
+    for (int q = 0; q < kWpItems; ++q) {
+        const int i = base + q;
+        if (i <= n) wpre[i] = run;  // slot n (one past the end) receives the total
+        run += f[q];
+    }
+}
+void launch_wprefix(const float4* pm, int n, uint32_t* part, uint32_t* wpre, hipStream_t s) {
+    // blocks cover slots [0, n]; the block holding slot n (one past the end)
+    // counts nothing and receives the grand total from the scan
+    const int nb = wprefix_blocks(n + 1);
+    k_wcount<<<nb, kBlock, 0, s>>>(pm, n, part);
+    k_rs_scan<<<1, 1024, 0, s>>>(part, nb);
+    k_wprefix<<<nb, kBlock, 0, s>>>(pm, n, part, wpre);
 }
 
 // in-range neighbour counts: r^2 = dx*dx + dy*dy + dz*dz in FP32, left to

@@ -33,13 +33,72 @@ SF_DEV float eos(const SFConst& k, float rho) {
     return p;
 }
 
+// Per-thread candidate lists in LDS.  A neighbour loop first filters the
+// candidates of its cell ranges (cheap: one float4 load, r^2, compare) and
+// appends the survivors to the lane's list; the expensive pair bodies then run
+// over the compacted lists, so a wave no longer executes every body for every
+// candidate that some lane happens to keep.  The flush is decided wave-wide
+// before appending, so every list stays within kCap.
+constexpr int kCap = 24;
+
+struct NList {
+    uint32_t* slot;  // this lane's column of the LDS list: slot[k * kBlock]
+    int cnt;
+};
+#define SF_NLIST(name)                                   \
+    __shared__ uint32_t name##_lds[kCap * kBlock];       \
+    NList name{name##_lds + threadIdx.x, 0}
+
+template <class Body>
+SF_DEV void nl_flush(NList& L, Body&& body) {
+    for (int k = 0; k < L.cnt; ++k) body(L.slot[k * kBlock]);
+    L.cnt = 0;
+}
+template <class Body>
+SF_DEV void nl_push(NList& L, bool keep, uint32_t j, Body&& body) {
+    if (__ballot(L.cnt == kCap)) nl_flush(L, body);
+    L.slot[L.cnt * kBlock] = j;  // cnt < kCap here; a dropped slot is overwritten later
+    L.cnt += keep ? 1 : 0;
+}
+
+// Walks pm[s, e) with four independent loads in flight per lane (the loop
+// is otherwise bound by one dependent global load per candidate).
+template <class F>
+SF_DEV void for_range4(const float4* __restrict__ pm, uint32_t s, uint32_t e, F&& fn) {
+    uint32_t j = s;
+    for (; j + 4u <= e; j += 4u) {
+        const float4 a = ldo(pm, j), b = ldo(pm, j + 1u), c = ldo(pm, j + 2u), d = ldo(pm, j + 3u);
+        fn(j, a);
+        fn(j + 1u, b);
+        fn(j + 2u, c);
+        fn(j + 3u, d);
+    }
+    for (; j < e; ++j) fn(j, ldo(pm, j));
+}
+
+// column ranges plus the number of water particles in each: the cell table
+// holds (cellStart[c], water prefix at cellStart[c]) so one 8-byte load per
+// range end gives both.
+template <class F>
+SF_DEV void for_ranges(const SFConst& k, const Fields& f, int cx, int cy, int cz, F&& fn) {
+    for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+        const uint2 a = ldo(f.ct, c0), b = ldo(f.ct, c1);
+        if (a.x == b.x) return;
+        const uint32_t nw = b.y - a.y;
+        fn(a.x, b.x, nw, (b.x - a.x) - nw);
+    });
+}
+
+SF_DEV float sq3(float d) { return d * d * d; }
+
 // ---------------------------------------------------------------------------
 // P1: water density (self counted twice, :466 + loop) and pressure; sand gets
-// rho_sand and p = 0.
+// rho_sand and p = 0.  Branch-free: max(h^2 - r^2, 0) vanishes outside the
+// support and max(w, 0) drops sand (w = -mass).
 __global__ __launch_bounds__(kBlock) void k_density(SFConst k, Fields f, int n) {
     const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
-    const float4 pi = f.pm[i];
+    const float4 pi = ldo(f.pm, (uint32_t)i);
     if (pi.w < 0.f) {
         f.rp[i] = make_float2(k.rho_s, 0.f);
         return;
@@ -47,28 +106,21 @@ __global__ __launch_bounds__(kBlock) void k_density(SFConst k, Fields f, int n) 
     int cx, cy, cz;
     cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
     float sf = 0.f, sb = 0.f;
+    for_ranges(k, f, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t) {
+        if (nw == 0) return;
+        for_range4(f.pm, s, e, [&](uint32_t, const float4& pj) {
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float d = fmaxf(k.h2 - (dx * dx + dy * dy + dz * dz), 0.f);
+            sf += fmaxf(pj.w, 0.f) * sq3(d);
+        });
+    });
     for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
-        const uint32_t e = f.cell[c1];
-        for (uint32_t j = f.cell[c0]; j < e; ++j) {
-            const float4 pj = f.pm[j];
-            if (pj.w > 0.f) {
-                const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 < k.h2) {
-                    const float d = k.h2 - r2;
-                    sf += pj.w * d * d * d;
-                }
-            }
-        }
-        const uint32_t be = f.bcell[c1];
-        for (uint32_t j = f.bcell[c0]; j < be; ++j) {
-            const float4 b = f.bpv[j];
+        const uint32_t be = ldo(f.bcell, c1);
+        for (uint32_t j = ldo(f.bcell, c0); j < be; ++j) {
+            const float4 b = ldo(f.bpv, j);
             const float dx = pi.x - b.x, dy = pi.y - b.y, dz = pi.z - b.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < k.h2) {
-                const float d = k.h2 - r2;
-                sb += fabsf(b.w) * d * d * d;
-            }
+            const float d = fmaxf(k.h2 - (dx * dx + dy * dy + dz * dz), 0.f);
+            sb += fabsf(b.w) * sq3(d);
         }
     });
     const float rho = pi.w * k.w0 + k.w_coef * (sf + k.rho0 * sb);
@@ -80,45 +132,46 @@ void launch_density(const SFConst& k, const Fields& f, int n, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// P2: sand volume fraction, average inter-phase flow velocity, saturation.
+// P2 (ComputeSFAvgFlow_CUDA): water i -> voidage 1 - sum_sand V W and the
+// average sand velocity; sand i -> average water velocity and saturation.
+// (Sand voidage from this pass is overwritten by P3 before anything reads it,
+// so a sand particle only needs its water neighbours, a water particle only
+// its sand neighbours: ranges without the other phase are skipped.)
+// Water particles also store wt = (rho / max(eps, voidage), p / max(eps, that^2)),
+// the per-particle factors the water momentum pass needs from every neighbour.
 __global__ __launch_bounds__(kBlock) void k_avg_flow(SFConst k, Fields f, int n) {
     const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
-    const float4 pi = f.pm[i];
+    const float4 pi = ldo(f.pm, (uint32_t)i);
     const bool sand_i = pi.w < 0.f;
     int cx, cy, cz;
     cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
     float sandv = 0.f, vsum = 0.f;
     v3 av = mk3(0.f, 0.f, 0.f);
     const float inv_rho_s = k.rho_s != 0.f ? 1.f / k.rho_s : 0.f;
-    for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
-        const uint32_t e = f.cell[c1];
-        for (uint32_t j = f.cell[c0]; j < e; ++j) {
-            const float4 pj = f.pm[j];
+    for_ranges(k, f, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t ns) {
+        if ((sand_i ? nw : ns) == 0) return;
+        for (uint32_t j = s; j < e; ++j) {
+            const float4 pj = ldo(f.pm, j);
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 >= k.h2) continue;
-            const float d = k.h2 - r2;
-            const float w = k.w_coef * d * d * d;
-            if (pj.w < 0.f) {  // sand j: ComputeSFSandVolume + water-side flow
-                const float vw = (-pj.w * inv_rho_s) * w;
-                sandv += vw;
-                if (!sand_i) {
-                    const float4 vj = f.vr[j];
-                    av += xyz(vj) * vw;
-                    vsum += vw;
-                }
-            } else if (sand_i) {  // water j: sand-side flow and saturation
-                const float vw = (pj.w / f.rp[j].x) * w;
-                const float4 vj = f.vr[j];
-                av += xyz(vj) * vw;
-                vsum += vw;
-            }
+            if (r2 >= k.h2 || (pj.w < 0.f) == sand_i) continue;
+            const float w = k.w_coef * sq3(k.h2 - r2);
+            const float vw = (pj.w < 0.f ? -pj.w * inv_rho_s : pj.w / ldo(f.rp, j).x) * w;
+            if (!sand_i) sandv += vw;
+            av += xyz(ldo(f.vr, j)) * vw;
+            vsum += vw;
         }
     });
     const v3 fl = vsum != 0.f ? av * (1.f / vsum) : mk3(0.f, 0.f, 0.f);
     f.flow[i] = f4(fl, 0.f);
-    f.es[i] = make_float2(1.f - sandv, sand_i ? vsum : 0.f);
+    const float eps_i = 1.f - sandv;
+    f.es[i] = make_float2(eps_i, sand_i ? vsum : 0.f);
+    if (!sand_i) {
+        const float2 rpi = ldo(f.rp, (uint32_t)i);
+        const float rt = rpi.x / fmaxf(kEps, eps_i);
+        f.wt[i] = make_float2(rt, rpi.y / fmaxf(kEps, rt * rt));
+    }
 }
 void launch_avg_flow(const SFConst& k, const Fields& f, int n, hipStream_t s) {
     if (n <= 0) return;
@@ -130,23 +183,21 @@ void launch_avg_flow(const SFConst& k, const Fields& f, int n, hipStream_t s) {
 __global__ __launch_bounds__(kBlock) void k_voidage_adhesion(SFConst k, Fields f, int n) {
     const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
-    const float4 pi = f.pm[i];
+    const float4 pi = ldo(f.pm, (uint32_t)i);
     const bool sand_i = pi.w < 0.f;
     int cx, cy, cz;
     cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
     if (sand_i) {
         float num = 0.f, den = 0.f;
-        for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
-            const uint32_t e = f.cell[c1];
-            for (uint32_t j = f.cell[c0]; j < e; ++j) {
-                const float4 pj = f.pm[j];
-                if (pj.w < 0.f) continue;
+        for_ranges(k, f, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t) {
+            if (nw == 0) return;
+            for (uint32_t j = s; j < e; ++j) {
+                const float4 pj = ldo(f.pm, j);
                 const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                 const float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 >= k.h2) continue;
-                const float d = k.h2 - r2;
-                const float vw = (pj.w / f.rp[j].x) * (k.w_coef * d * d * d);
-                num += f.es[j].x * vw;
+                if (pj.w < 0.f || r2 >= k.h2) continue;
+                const float vw = (pj.w / ldo(f.rp, j).x) * (k.w_coef * sq3(k.h2 - r2));
+                num += ldo(f.es, j).x * vw;
                 den += vw;
             }
         });
@@ -158,21 +209,19 @@ __global__ __launch_bounds__(kBlock) void k_voidage_adhesion(SFConst k, Fields f
     const float inv_rho_s = k.rho_s != 0.f ? 1.f / k.rho_s : 0.f;
     float vs = 0.f;
     v3 F = mk3(0.f, 0.f, 0.f);
-    for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
-        const uint32_t e = f.cell[c1];
-        for (uint32_t j = f.cell[c0]; j < e; ++j) {
-            const float4 pj = f.pm[j];
-            if (pj.w >= 0.f) continue;
+    for_ranges(k, f, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t, uint32_t ns) {
+        if (ns == 0) return;
+        for (uint32_t j = s; j < e; ++j) {
+            const float4 pj = ldo(f.pm, j);
             const v3 dp = mk3(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
             const float r2 = dot(dp, dp);
-            if (r2 >= k.h2) continue;
+            if (pj.w >= 0.f || r2 >= k.h2) continue;
             const float mj = -pj.w;
-            const float d = k.h2 - r2;
-            vs += (mj * inv_rho_s) * (k.w_coef * d * d * d);
+            vs += (mj * inv_rho_s) * (k.w_coef * sq3(k.h2 - r2));
             const float l = sqrtf(r2);
             const float A = adhesion_A(k, l);
             if (A != 0.f) {
-                const float G = bezier(k.gadh, f.es[j].y);
+                const float G = bezier(k.gadh, ldo(f.es, j).y);
                 F += dp * (G * mi * mj * A / fmaxf(kEps, l));
             }
         }
@@ -188,16 +237,28 @@ void launch_voidage_adhesion(const SFConst& k, const Fields& f, int n, hipStream
 // ---------------------------------------------------------------------------
 // P4: sand linear momentum (DEM + capillary + buoyancy + adhesion + boundary
 // DEM + walls + drag), acc = g + 2 (f + df) / m, then the wet-sand colour.
+//
+// Sand-sand forces vanish beyond r_c = 2R + s_rupture (DEM needs overlap,
+// the bridge H < s_rupture), and r_c < h / 2, so along each axis a grain only
+// reaches its own cell and the one neighbour on its nearer side: the contact
+// search walks 2 x 2 x 2 cells instead of 27 (same non-zero terms).
+SF_DEV void near_span(float p, float lo, float h, int c, int g, float rc, int& a, int& b) {
+    const float cl = lo + (float)c * h;
+    a = (c > 0 && p - cl < rc) ? c - 1 : c;
+    b = (c < g - 1 && cl + h - p < rc) ? c + 1 : c;
+}
+
 __global__ __launch_bounds__(kBlock) void k_sand_momentum(SFConst k, Fields f, int n) {
+    SF_NLIST(L);
     const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
-    const float4 pi = f.pm[i];
+    const float4 pi = ldo(f.pm, (uint32_t)i);
     if (pi.w >= 0.f) return;
     const float mi = -pi.w;
     const v3 xi = xyz(pi);
-    const v3 vi = xyz(f.vr[i]);
-    const float rho_i = f.rp[i].x;
-    const float2 esi = f.es[i];
+    const v3 vi = xyz(ldo(f.vr, (uint32_t)i));
+    const float rho_i = ldo(f.rp, (uint32_t)i).x;
+    const float2 esi = ldo(f.es, (uint32_t)i);
     const float sat_i = esi.y;
     const float twoR = 2.f * k.R;
     const float cap_r = twoR + k.cap_srup;
@@ -207,59 +268,84 @@ __global__ __launch_bounds__(kBlock) void k_sand_momentum(SFConst k, Fields f, i
     cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
     v3 fsum = mk3(0.f, 0.f, 0.f);
     float vw = 0.f;
-    for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
-        const uint32_t e = f.cell[c1];
-        for (uint32_t j = f.cell[c0]; j < e; ++j) {
-            const float4 pj = f.pm[j];
-            const v3 dji = mk3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-            const float r2 = dot(dji, dji);
-            if (pj.w < 0.f) {  // sand j
-                if ((int)j == i || r2 >= cap_r2) continue;
-                const float dist = sqrtf(r2);
-                if (dist < twoR) {
-                    const v3 vj = xyz(f.vr[j]);
-                    fsum += dem_contact(dji, dist, vj - vi, twoR, k.kn, k.ks, k.tan2);
-                } else {
-                    const float H = dist - twoR;
-                    if (H > 0.f && H < k.cap_srup)
-                        fsum += capillary(k, dji, dist, H, (sat_i + f.es[j].y) * 0.5f);
-                }
-            } else {  // water j
-                if (r2 >= k.h2) continue;
-                const float d = k.h2 - r2;
-                const float w = k.w_coef * d * d * d;
-                const float eps_j = f.es[j].x;
-                if (eps_j > 0.f && eps_j <= 1.f) {  // ComputeSFSandBuoyancyForces
-                    const float2 rpj = f.rp[j];
-                    const float mj = pj.w;
-                    const float dj = rpj.x / fmaxf(kEps, eps_j);
-                    vw += mj / dj * w;
-                    const float gs = spiky_s(k, sqrtf(r2));
-                    // gradW(x_i - x_j) = gs * (x_i - x_j) = -gs * dji
-                    fsum += dji * (mi * mj * rpj.y / fmaxf(kEps, rho_i * dj) * gs);
-                }
-                const float4 aj = f.xf[j];  // avgAdhesion of water j (ComputeSFSandAdhesionTerm)
-                fsum += xyz(aj) * (w * vol_i);
-            }
+
+    // sand-sand: DEM contact or liquid bridge
+    auto contact = [&](uint32_t j) {
+        const float4 pj = ldo(f.pm, j);
+        const v3 dji = mk3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+        const float dist = sqrtf(dot(dji, dji));
+        if (dist < twoR) {
+            fsum += dem_contact(dji, dist, xyz(ldo(f.vr, j)) - vi, twoR, k.kn, k.ks, k.tan2);
+        } else {
+            const float H = dist - twoR;
+            if (H > 0.f && H < k.cap_srup) fsum += capillary(k, dji, dist, H, (sat_i + ldo(f.es, j).y) * 0.5f);
         }
-        if (k.blabel1) {  // ComputeSFUniRadiusDemBoundaryForces: boundary label 1 only
-            const uint32_t be = f.bcell[c1];
-            for (uint32_t j = f.bcell[c0]; j < be; ++j) {
-                const float4 b = f.bpv[j];
+    };
+    {
+        const float rcm = cap_r + 1e-3f * k.h;  // conservative against FP32 cell edges
+        int xa, xb, ya, yb, za, zb;
+        near_span(pi.x, k.lox, k.h, cx, k.gx, rcm, xa, xb);
+        near_span(pi.y, k.loy, k.h, cy, k.gy, rcm, ya, yb);
+        near_span(pi.z, k.loz, k.h, cz, k.gz, rcm, za, zb);
+        for (int x = xa; x <= xb; ++x)
+            for (int y = ya; y <= yb; ++y) {
+                const uint32_t base = (uint32_t)((x * k.gy + y) * k.gz);
+                const uint32_t s0 = ldo(f.ct, base + (uint32_t)za).x, e0 = ldo(f.ct, base + (uint32_t)zb + 1u).x;
+                for_range4(f.pm, s0, e0, [&](uint32_t j, const float4& pj) {
+                    const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    nl_push(L, pj.w < 0.f && (int)j != i && r2 < cap_r2, j, contact);
+                });
+            }
+        nl_flush(L, contact);
+    }
+    // water neighbours: buoyancy (ComputeSFSandBuoyancyForces) + adhesion
+    // (ComputeSFSandAdhesionTerm)
+    auto water = [&](uint32_t j) {
+        const float4 pj = ldo(f.pm, j);
+        const v3 dji = mk3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+        const float r2 = dot(dji, dji);
+        const float w = k.w_coef * sq3(k.h2 - r2);
+        const float eps_j = ldo(f.es, j).x;
+        if (eps_j > 0.f && eps_j <= 1.f) {
+            const float2 rpj = ldo(f.rp, j);
+            const float mj = pj.w;
+            const float dj = rpj.x / fmaxf(kEps, eps_j);
+            vw += mj / dj * w;
+            const float gs = spiky_s(k, sqrtf(r2));
+            // gradW(x_i - x_j) = gs * (x_i - x_j) = -gs * dji
+            fsum += dji * (mi * mj * rpj.y / fmaxf(kEps, rho_i * dj) * gs);
+        }
+        fsum += xyz(ldo(f.xf, j)) * (w * vol_i);
+    };
+    for_ranges(k, f, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t) {
+        if (nw == 0) return;
+        for_range4(f.pm, s, e, [&](uint32_t j, const float4& pj) {
+            const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            nl_push(L, pj.w > 0.f && r2 < k.h2, j, water);
+        });
+    });
+    nl_flush(L, water);
+    if (k.blabel1) {  // ComputeSFUniRadiusDemBoundaryForces: boundary label 1 only
+        for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+            const uint32_t be = ldo(f.bcell, c1);
+            for (uint32_t j = ldo(f.bcell, c0); j < be; ++j) {
+                const float4 b = ldo(f.bpv, j);
                 if (b.w >= 0.f) continue;
                 fsum += dem_force(xi - xyz(b), -vi, twoR, k.kn, k.ks, k.tan2);
             }
-        }
-    });
+        });
+    }
     fsum += world_walls(k, xi, vi);
     v3 df = mk3(0.f, 0.f, 0.f);
     const float eps_i = esi.x;
     if (eps_i < 1.f && eps_i > 0.f) {
-        const float cd = k.multi ? f.cda[i].x : k.cd;
+        const float cd = k.multi ? ldo(f.cda, (uint32_t)i).x : k.cd;
         const float ome = 1.f - eps_i;
         const float dc = cd * (eps_i * eps_i * eps_i) / (ome * ome);
         if (dc != 0.f) {
-            const v3 fl = xyz(f.flow[i]);
+            const v3 fl = xyz(ldo(f.flow, (uint32_t)i));
             df = (fl - vi) * (k.grav_abs * k.rho0 * (eps_i * eps_i) / dc * mi / rho_i);
         }
     }
@@ -267,7 +353,7 @@ __global__ __launch_bounds__(kBlock) void k_sand_momentum(SFConst k, Fields f, i
     const v3 a = mk3(k.gvx, k.gvy, k.gvz) + (fsum + df) * (2.f / mi);
     f.acc[i] = f4(a, 0.f);
     // ComputeSFWetSandColor_CUDA (:952-977)
-    float4 c = f.cs[i];
+    float4 c = ldo((const float4*)f.cs, (uint32_t)i);
     if (c.w < sat_i) {
         c.x = k.dryr + (k.wetr - k.dryr) * sat_i;
         c.y = k.dryg + (k.wetg - k.dryg) * sat_i;
@@ -285,66 +371,73 @@ void launch_sand_momentum(const SFConst& k, const Fields& f, int n, hipStream_t 
 // P5: water linear momentum (pressure + artificial viscosity + seepage +
 // boundary pressure/viscosity), acc = g + a - F_adh / m.
 __global__ __launch_bounds__(kBlock) void k_water_momentum(SFConst k, Fields f, int n) {
+    SF_NLIST(L);
     const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
-    const float4 pi = f.pm[i];
+    const float4 pi = ldo(f.pm, (uint32_t)i);
     if (pi.w < 0.f) return;
     const float mi = pi.w;
-    const v3 vi = xyz(f.vr[i]);
-    const float2 rpi = f.rp[i];
-    const float eps_i = f.es[i].x;
-    const float rt_i = rpi.x / fmaxf(kEps, eps_i);
-    const float pterm_i = rpi.y / fmaxf(kEps, rt_i * rt_i);
-    const float inv_rt_i = 1.f / rt_i;
-    const float bpress = -k.rho0 * (rpi.y / fmaxf(kEps, k.rho0 * k.rho0));
+    const v3 vi = xyz(ldo(f.vr, (uint32_t)i));
+    const float2 wti = ldo(f.wt, (uint32_t)i);  // (rho~_i, p_i / rho~_i^2)
+    const float p_i = ldo(f.rp, (uint32_t)i).y;
+    const float rt_i = wti.x;
+    const float inv_rt_i = frcp(rt_i);
+    const float bpress = -k.rho0 * (p_i / fmaxf(kEps, k.rho0 * k.rho0));
     const float bvisc = -k.bnu / (2.f * k.rho0);
     int cx, cy, cz;
     cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
     v3 a = mk3(0.f, 0.f, 0.f);
-    for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
-        const uint32_t e = f.cell[c1];
-        for (uint32_t j = f.cell[c0]; j < e; ++j) {
-            const float4 pj = f.pm[j];
-            const v3 d = mk3(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
-            const float r2 = dot(d, d);
-            if (r2 >= k.h2) continue;
-            const float gs = spiky_s(k, sqrtf(r2));
-            if (pj.w > 0.f) {  // water j
-                if ((int)j == i) continue;
-                const float2 rpj = f.rp[j];
-                const float rt_j = rpj.x / fmaxf(kEps, f.es[j].x);
-                float s = -pj.w * (pterm_i + rpj.y / fmaxf(kEps, rt_j * rt_j));
-                const v3 dv = vi - xyz(f.vr[j]);
-                const float dd = dot(dv, d);
-                if (dd < 0.f) {  // ComputeSFWaterArtificialViscosity
-                    const float pij = -k.nu / (rt_i + rt_j) * (dd / (r2 + kEps));
-                    s += -pj.w * pij;
-                }
-                a += d * (s * gs);
-            } else {  // sand j: ComputeSFWaterSeepageTerm
-                const float mj = -pj.w;
-                const float rho_j = f.rp[j].x;
-                const float dk = k.h2 - r2;
-                const float w = k.w_coef * dk * dk * dk;
-                a += d * (-mj * rpi.y / fmaxf(kEps, rt_i * rho_j) * gs);
-                a += xyz(f.xf[j]) * (-w * inv_rt_i);
-            }
+    auto pair = [&](uint32_t j) {
+        const float4 pj = ldo(f.pm, j);
+        const v3 d = mk3(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
+        const float r2 = dot(d, d);
+        const float rs = frsq(r2);
+        const float r = r2 * rs;
+        const float t = k.h - r;
+        const float gs = r >= kEps ? k.g_coef * t * t * rs : 0.f;  // spiky gradient / r
+        if (pj.w > 0.f) {  // water j: ComputeSFWaterPressureTerm + ArtificialViscosity
+            const float2 wtj = ldo(f.wt, j);
+            float s = -pj.w * (wti.y + wtj.y);
+            const v3 dv = vi - xyz(ldo(f.vr, j));
+            const float dd = dot(dv, d);
+            const float pij = -k.nu * frcp(rt_i + wtj.x) * (dd * frcp(r2 + kEps));
+            s += dd < 0.f ? -pj.w * pij : 0.f;
+            a += d * (s * gs);
+        } else {  // sand j: ComputeSFWaterSeepageTerm
+            const float mj = -pj.w;
+            const float rho_j = ldo(f.rp, j).x;
+            const float w = k.w_coef * sq3(k.h2 - r2);
+            a += d * (-mj * p_i * frcp(fmaxf(kEps, rt_i * rho_j)) * gs);
+            a += xyz(ldo(f.xf, j)) * (-w * inv_rt_i);
         }
-        const uint32_t be = f.bcell[c1];
-        for (uint32_t j = f.bcell[c0]; j < be; ++j) {
-            const float4 b = f.bpv[j];
+    };
+    for_ranges(k, f, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t, uint32_t) {
+        for_range4(f.pm, s, e, [&](uint32_t j, const float4& pj) {
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            nl_push(L, r2 < k.h2 && (int)j != i, j, pair);
+        });
+    });
+    nl_flush(L, pair);
+    for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+        const uint32_t be = ldo(f.bcell, c1);
+        for (uint32_t j = ldo(f.bcell, c0); j < be; ++j) {
+            const float4 b = ldo(f.bpv, j);
             const v3 d = mk3(pi.x - b.x, pi.y - b.y, pi.z - b.z);
             const float r2 = dot(d, d);
             if (r2 >= k.h2) continue;
-            const float gs = spiky_s(k, sqrtf(r2));
+            const float rs = frsq(r2);
+            const float r = r2 * rs;
+            const float t = k.h - r;
+            const float gs = r >= kEps ? k.g_coef * t * t * rs : 0.f;
             const float vb = fabsf(b.w);
             float s = vb * bpress;  // ComputeBoundaryPressure (densityi = rho0, :944)
             const float dd = dot(vi, d);
-            if (dd < 0.f) s += -vb * k.rho0 * (bvisc * (dd / (r2 + kEps)));  // ComputeBoundaryViscosity
+            if (dd < 0.f) s += -vb * k.rho0 * (bvisc * (dd * frcp(r2 + kEps)));  // ComputeBoundaryViscosity
             a += d * (s * gs);
         }
     });
-    const v3 ad = xyz(f.adh[i]);
+    const v3 ad = xyz(ldo(f.adh, (uint32_t)i));
     const v3 acc = mk3(k.gvx, k.gvy, k.gvz) + a - ad * (1.f / mi);
     f.acc[i] = f4(acc, 0.f);
 }

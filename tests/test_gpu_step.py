@@ -22,8 +22,11 @@ TOL = {
     "saturation": (1, 1e-5),
     "avg_flow_vel": (None, 1e-5),
     "adhesion_force": (0, 1e-5),
-    "avg_adhesion_force": (0, 1e-5),
-    "avg_drag_force": (1, 1e-5),
+    # F / vS and df / vW: at a phase fringe the denominator holds only
+    # kernel-edge terms c (h^2 - r^2)^3 with r ~ h, whose FP32 relative error
+    # is ~ 6e-8 * 3 h^2 / (h^2 - r^2); the ratio inherits it.
+    "avg_adhesion_force": (0, 5e-5),
+    "avg_drag_force": (1, 5e-5),
     "acc_water": (0, 1e-5),
     "acc_sand": (1, 1e-5),
     "max_saturation": (1, 1e-5),
