@@ -129,9 +129,11 @@ SF_DEV float spiky_s(const SFConst& k, float r) {
     float d = k.h - r;
     return k.g_coef * d * d / r;
 }
-// AdhesionAkinci13 (:189-203)
+// AdhesionAkinci13 (:189-203).  The base -4r^2/h + 6r - 2h vanishes at r = h
+// and can round to a tiny negative there (pow would give NaN); it is clamped
+// at 0, which changes nothing where the reference's value is finite.
 SF_DEV float adhesion_A(const SFConst& k, float r) {
-    if (2.f * r > k.h && r <= k.h) return k.adh_coef * powf(-4.f * r * r / k.h + 6.f * r - 2.f * k.h, 0.25f);
+    if (2.f * r > k.h && r <= k.h) return k.adh_coef * powf(fmaxf(-4.f * r * r / k.h + 6.f * r - 2.f * k.h, 0.f), 0.25f);
     return 0.f;
 }
 // QuadraticBezierCoeff (cuda_sf_utils.cuh:36-82)
