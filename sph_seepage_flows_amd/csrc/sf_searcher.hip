@@ -384,10 +384,13 @@ __global__ __launch_bounds__(kBlock) void k_nbr_count(SFConst k, const float4* _
         const uint32_t s0 = cs[c0], e = cs[c1];
         cc += e - s0;
         for (uint32_t j = s0; j < e; ++j) {
+#pragma clang fp contract(off)
+            // written out here (not via helpers) so the pragma governs every
+            // operation and nothing is fused into an FMA
             const float4 pj = pm[j];
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float xx = __fmul_rn(dx, dx), yy = __fmul_rn(dy, dy), zz = __fmul_rn(dz, dz);
-            const float r2 = __fadd_rn(__fadd_rn(xx, yy), zz);
+            const float xx = dx * dx, yy = dy * dy, zz = dz * dz;
+            const float r2 = (xx + yy) + zz;
             c += r2 < k.h2 ? 1u : 0u;
         }
     });
