@@ -77,6 +77,13 @@ def build(verbose: bool = False) -> Path:
     if r.returncode:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}{r.stderr}")
     os.replace(tmp, LIB)
+    # the main.cpp-style example driver on the drop-in C++ layer
+    drv = LIBDIR / "seepage_main"
+    cmd = [hipcc(), "-O2", "-std=c++17", f"-I{INCLUDE}", str(PKG / "cpp" / "seepage_main.cpp"), f"-L{LIBDIR}",
+           "-lsf_seepage", "-Wl,-rpath,$ORIGIN", "-o", str(drv)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}{r.stderr}")
     return LIB
 
 

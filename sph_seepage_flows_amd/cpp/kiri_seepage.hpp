@@ -1,0 +1,355 @@
+// kiri_seepage.hpp -- header-only C++ layer that re-creates the reference's
+// seepage-flow class API (namespace KIRI) on top of the C-ABI in
+// include/sf_seepage.h, so a main.cpp-style caller drops in:
+//
+//   reference                                         here
+//   CudaSeepageflowParams / CUDA_SEEPAGEFLOW_PARAMS   = sf_params, inline global
+//     (seepage_flows_cuda/include/kiri_pbs_cuda/data/cuda_seepageflow_params.h:25-84)
+//   CudaBoundaryParams / CUDA_BOUNDARY_PARAMS         = sf_boundary_params, inline global
+//     (.../data/cuda_boundary_params.h:19-29)
+//   CudaBoundaryEmitter::BuildWorldBoundary           -> sf_build_world_boundary
+//     (src/kiri_pbs_cuda/emitter/cuda_boundary_emitter.cpp:12-80)
+//   CudaSFParticles(maxNum, pos, col, label, mass, radius[, cda0asat, amcamcp])
+//     (.../particle/cuda_sf_particles.cuh:138-235)
+//   CudaBoundaryParticles(pos, label)                 (.../particle/cuda_boundary_particles.cuh:21-28)
+//   CudaGNSearcher / CudaGNBoundarySearcher           (.../searcher/cuda_neighbor_searcher.cuh:56-98)
+//   CudaWCSphSFSolver(num, negativeScale, timeStepLimitScale, speedOfSound)
+//     (.../solver/seepageflow/cuda_wcsph_sf_solver.cuh:19-68)
+//   CudaSFSystem(...), UpdateSystem(renderInterval), GetNumOfSubTimeSteps()
+//     (.../system/cuda_sf_system.cuh:15-50, cuda_base_system.cuh:17-44)
+//   ExportBgeoFileCUDA(folder, name, ...)             (seepage_flows/include/kiri_utils.h:364-448)
+//
+// Differences a caller sees: Get*Ptr() return device pointers to this
+// library's float4 storage (stride 16 bytes, see sf_device_buffer); the bgeo
+// export takes the particle object instead of four device pointers; the
+// inflow emitter is not part of the seepage step and must stay disabled.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <hip/hip_vector_types.h>
+
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sf_seepage.h"
+
+namespace KIRI {
+
+template <class T>
+using SharedPtr = std::shared_ptr<T>;
+using Vec_Float = std::vector<float>;
+using Vec_Float2 = std::vector<float2>;
+using Vec_Float3 = std::vector<float3>;
+using Vec_SizeT = std::vector<size_t>;
+
+enum CudaSphSolverType { SPH_SOLVER = SF_SPH_SOLVER, WCSPH_SOLVER = SF_WCSPH_SOLVER, IISPH_SOLVER = 2 };
+enum CudaSeepageflowType { SF = SF_TYPE_SF, MULTI_SF = SF_TYPE_MULTI_SF };
+enum SearcherParticleType { SEEPAGE = 0, SEEPAGE_MULTI = 1 };
+
+using CudaSeepageflowParams = sf_params;
+using CudaBoundaryParams = sf_boundary_params;
+
+struct CudaSeepageflowAppParams {
+    size_t max_num = 100000;
+    bool run = false;
+    bool run_offline = false;
+    int scene_data_idx = 0;
+    char bgeo_export_folder[320] = "default";
+    bool bgeo_export = false;
+};
+
+inline CudaSeepageflowParams CUDA_SEEPAGEFLOW_PARAMS{};
+inline CudaBoundaryParams CUDA_BOUNDARY_PARAMS{};
+inline CudaSeepageflowAppParams CUDA_SEEPAGEFLOW_APP_PARAMS{};
+
+inline sf_float3 to_sf(float3 v) { return sf_float3{v.x, v.y, v.z}; }
+inline sf_int3 to_sf(int3 v) { return sf_int3{v.x, v.y, v.z}; }
+
+// The reference prints failures and carries on (kiri_pbs_pch.cuh:35-50); a
+// handle that could not be created cannot carry on, so that case throws.
+inline void sf_check(int rc, const char* what) {
+    if (rc != SF_OK) {
+        std::string msg = std::string(what) + ": " + sf_last_error();
+        std::fprintf(stderr, "%s\n", msg.c_str());
+        throw std::runtime_error(msg);
+    }
+}
+
+// int3((highest - lowest) / kernelRadius) (main.cpp:104)
+inline int3 GridSize(float3 lo, float3 hi, float h) {
+    sf_int3 g{};
+    sf_check(sf_grid_size(to_sf(lo), to_sf(hi), h, &g), "sf_grid_size");
+    return make_int3(g.x, g.y, g.z);
+}
+
+struct BoundaryData {
+    Vec_Float3 pos;
+    Vec_SizeT label;
+};
+
+class CudaBoundaryEmitter {
+  public:
+    explicit CudaBoundaryEmitter(bool enable = true) : bEnable(enable) {}
+    void BuildWorldBoundary(BoundaryData& data, const float3& lowest, const float3& highest, float particleRadius) {
+        if (!bEnable) return;
+        size_t n = 0;
+        sf_check(sf_build_world_boundary(to_sf(lowest), to_sf(highest), particleRadius, 0, nullptr, nullptr, &n),
+                 "sf_build_world_boundary");
+        std::vector<float> p(3 * n);
+        std::vector<uint64_t> l(n);
+        sf_check(sf_build_world_boundary(to_sf(lowest), to_sf(highest), particleRadius, n, p.data(), l.data(), &n),
+                 "sf_build_world_boundary");
+        for (size_t i = 0; i < n; ++i) {
+            data.pos.push_back(make_float3(p[3 * i], p[3 * i + 1], p[3 * i + 2]));
+            data.label.push_back((size_t)l[i]);
+        }
+    }
+    void BuildBoundaryShapeVolume(BoundaryData& data, const std::vector<float4>& shape) {
+        if (!bEnable) return;
+        for (const float4& s : shape) {
+            data.pos.push_back(make_float3(s.x, s.y, s.z));
+            data.label.push_back(1);
+        }
+    }
+
+  private:
+    bool bEnable;
+};
+
+class CudaSFSystem;
+
+class CudaSFParticles {
+  public:
+    CudaSFParticles(size_t maxNum, const Vec_Float3& p, const Vec_Float3& col, const Vec_SizeT& label,
+                    const Vec_Float& mass, const Vec_Float& radius, const Vec_Float3& cda0asat = {},
+                    const Vec_Float2& amcamcp = {})
+        : mMax(maxNum), mPos(p), mCol(col), mLabel(label), mMass(mass), mRadius(radius), mCda(cda0asat),
+          mAmc(amcamcp) {}
+    CudaSFParticles(const CudaSFParticles&) = delete;
+    CudaSFParticles& operator=(const CudaSFParticles&) = delete;
+
+    size_t Size() const {
+        if (!mHandle) return mPos.size();
+        size_t n = 0;
+        sf_size(mHandle, &n, nullptr, nullptr);
+        return n;
+    }
+    size_t MaxSize() const { return mMax; }
+    // device pointers into the library's storage (float4 elements, stride 16)
+    float4* GetPosPtr() const { return (float4*)ptr(SF_FIELD_POS); }
+    float4* GetVelPtr() const { return (float4*)ptr(SF_FIELD_VEL); }
+    float4* GetAccPtr() const { return (float4*)ptr(SF_FIELD_ACC); }
+    float4* GetColPtr() const { return (float4*)ptr(SF_FIELD_COL); }
+    float2* GetDensityPressurePtr() const { return (float2*)ptr(SF_FIELD_DENSITY); }
+    float2* GetVoidageSaturationPtr() const { return (float2*)ptr(SF_FIELD_VOIDAGE); }
+
+    // host copies of any field (packed float3 / float / size_t as the reference stores them)
+    Vec_Float3 DownloadFloat3(int field) const {
+        Vec_Float3 out(Size());
+        if (!out.empty()) sf_check(sf_download(mHandle, field, out.data(), out.size() * sizeof(float3)), "sf_download");
+        return out;
+    }
+    Vec_Float DownloadFloat(int field) const {
+        Vec_Float out(Size());
+        if (!out.empty()) sf_check(sf_download(mHandle, field, out.data(), out.size() * sizeof(float)), "sf_download");
+        return out;
+    }
+    Vec_SizeT DownloadLabel() const {
+        std::vector<uint64_t> l(Size());
+        if (!l.empty()) sf_check(sf_download(mHandle, SF_FIELD_LABEL, l.data(), l.size() * 8), "sf_download");
+        return Vec_SizeT(l.begin(), l.end());
+    }
+    sf_handle Handle() const { return mHandle; }
+
+  private:
+    friend class CudaSFSystem;
+    void* ptr(int field) const {
+        void* p = nullptr;
+        size_t s = 0;
+        if (!mHandle) return nullptr;
+        sf_check(sf_device_buffer(mHandle, field, &p, &s), "sf_device_buffer");
+        return p;
+    }
+    void Attach(sf_handle h, bool multi) {
+        const size_t n = mPos.size();
+        std::vector<float> pos(3 * n), col(3 * n), cda, amc;
+        std::vector<uint64_t> lab(n);
+        for (size_t i = 0; i < n; ++i) {
+            pos[3 * i] = mPos[i].x; pos[3 * i + 1] = mPos[i].y; pos[3 * i + 2] = mPos[i].z;
+            if (i < mCol.size()) { col[3 * i] = mCol[i].x; col[3 * i + 1] = mCol[i].y; col[3 * i + 2] = mCol[i].z; }
+            lab[i] = mLabel[i];
+        }
+        if (multi && mCda.size() == n && mAmc.size() == n) {
+            cda.resize(3 * n);
+            amc.resize(2 * n);
+            for (size_t i = 0; i < n; ++i) {
+                cda[3 * i] = mCda[i].x; cda[3 * i + 1] = mCda[i].y; cda[3 * i + 2] = mCda[i].z;
+                amc[2 * i] = mAmc[i].x; amc[2 * i + 1] = mAmc[i].y;
+            }
+        }
+        sf_check(sf_upload_particles(h, n, pos.data(), nullptr, col.data(), lab.data(), mMass.data(),
+                                     mRadius.empty() ? nullptr : mRadius.data(), cda.empty() ? nullptr : cda.data(),
+                                     amc.empty() ? nullptr : amc.data()),
+                 "sf_upload_particles");
+        mHandle = h;
+    }
+    size_t mMax;
+    Vec_Float3 mPos, mCol;
+    Vec_SizeT mLabel;
+    Vec_Float mMass, mRadius;
+    Vec_Float3 mCda;
+    Vec_Float2 mAmc;
+    sf_handle mHandle = nullptr;
+};
+typedef SharedPtr<CudaSFParticles> CudaSFParticlesPtr;
+
+class CudaBoundaryParticles {
+  public:
+    CudaBoundaryParticles(const Vec_Float3& p, const Vec_SizeT& label = {}) : mPos(p), mLabel(label) {}
+    size_t Size() const { return mPos.size(); }
+    size_t MaxSize() const { return mPos.size(); }
+
+  private:
+    friend class CudaSFSystem;
+    Vec_Float3 mPos;
+    Vec_SizeT mLabel;
+};
+typedef SharedPtr<CudaBoundaryParticles> CudaBoundaryParticlesPtr;
+
+// Searchers are configuration here: the grid they describe is built inside
+// every step of the system (cuda_neighbor_searcher.cu:19-34).
+class CudaGNBaseSearcher {
+  public:
+    CudaGNBaseSearcher(float3 lo, float3 hi, size_t num, float cellSize)
+        : mLowestPoint(lo), mHighestPoint(hi), mCellSize(cellSize), mGridSize(GridSize(lo, hi, cellSize)),
+          mMaxNumOfParticles(num) {}
+    float3 GetLowestPoint() const { return mLowestPoint; }
+    float3 GetHighestPoint() const { return mHighestPoint; }
+    float GetCellSize() const { return mCellSize; }
+    int3 GetGridSize() const { return mGridSize; }
+
+  protected:
+    float3 mLowestPoint, mHighestPoint;
+    float mCellSize;
+    int3 mGridSize;
+    size_t mMaxNumOfParticles;
+};
+class CudaGNSearcher final : public CudaGNBaseSearcher {
+  public:
+    CudaGNSearcher(float3 lo, float3 hi, size_t num, float cellSize, SearcherParticleType type)
+        : CudaGNBaseSearcher(lo, hi, num, cellSize), mType(type) {}
+    SearcherParticleType GetSearcherType() const { return mType; }
+
+  private:
+    SearcherParticleType mType;
+};
+class CudaGNBoundarySearcher final : public CudaGNBaseSearcher {
+  public:
+    using CudaGNBaseSearcher::CudaGNBaseSearcher;
+};
+typedef SharedPtr<CudaGNSearcher> CudaGNSearcherPtr;
+typedef SharedPtr<CudaGNBoundarySearcher> CudaGNBoundarySearcherPtr;
+
+class CudaWCSphSFSolver {
+  public:
+    explicit CudaWCSphSFSolver(size_t num, float negativeScale = 0.f, float timeStepLimitScale = 3.f,
+                               float speedOfSound = 100.f)
+        : mNum(num), mOpt{negativeScale, timeStepLimitScale, speedOfSound} {}
+    float GetSpeedOfSound() const { return mOpt.speed_of_sound; }
+    float GetTimeStepLimitScale() const { return mOpt.time_step_limit_scale; }
+    void SetSpeedOfSound(float c) { mOpt.speed_of_sound = c > 1e-6f ? c : 1e-6f; }        // cuda_wcsph_sf_solver.cpp:19-22
+    void SetTimeStepLimitScale(float s) { mOpt.time_step_limit_scale = s > 0.f ? s : 0.f; } // :29-32
+    const sf_solver_options& Options() const { return mOpt; }
+
+  private:
+    size_t mNum;
+    sf_solver_options mOpt;
+};
+typedef SharedPtr<CudaWCSphSFSolver> CudaSphSFSolverPtr;
+
+// The inflow emitter is outside the seepage step; only a disabled one is accepted.
+struct CudaEmitter {
+    bool enable = false;
+};
+typedef SharedPtr<CudaEmitter> CudaEmitterPtr;
+
+class CudaSFSystem {
+  public:
+    CudaSFSystem(CudaSFParticlesPtr& particles, CudaBoundaryParticlesPtr& boundaryParticles,
+                 CudaSphSFSolverPtr& solver, CudaGNSearcherPtr& searcher, CudaGNBoundarySearcherPtr& boundarySearcher,
+                 CudaEmitterPtr& emitter, bool adaptiveSubTimeStep = false)
+        : mParticles(std::move(particles)), mBoundaries(std::move(boundaryParticles)), mSolver(std::move(solver)),
+          mSearcher(std::move(searcher)), mBoundarySearcher(std::move(boundarySearcher)),
+          bAdaptiveSubTimeStep(adaptiveSubTimeStep) {
+        if (emitter && emitter->enable) throw std::runtime_error("the inflow emitter is not part of this library");
+        const bool multi = CUDA_SEEPAGEFLOW_PARAMS.sf_type == SF_TYPE_MULTI_SF;
+        sf_check(sf_create(&CUDA_SEEPAGEFLOW_PARAMS, &CUDA_BOUNDARY_PARAMS, &mSolver->Options(),
+                           mParticles->MaxSize(), -1, &mHandle),
+                 "sf_create");
+        mParticles->Attach(mHandle, multi);
+        std::vector<float> bp(3 * mBoundaries->mPos.size());
+        std::vector<uint64_t> bl(mBoundaries->mPos.size(), 0);
+        for (size_t i = 0; i < mBoundaries->mPos.size(); ++i) {
+            bp[3 * i] = mBoundaries->mPos[i].x;
+            bp[3 * i + 1] = mBoundaries->mPos[i].y;
+            bp[3 * i + 2] = mBoundaries->mPos[i].z;
+            if (i < mBoundaries->mLabel.size()) bl[i] = mBoundaries->mLabel[i];
+        }
+        sf_check(sf_upload_boundary(mHandle, mBoundaries->mPos.size(), bp.data(), bl.data()), "sf_upload_boundary");
+    }
+    CudaSFSystem(const CudaSFSystem&) = delete;
+    CudaSFSystem& operator=(const CudaSFSystem&) = delete;
+    ~CudaSFSystem() { sf_destroy(mHandle); }
+
+    // CudaBaseSystem::UpdateSystem (cuda_base_system.cu:29-56): the global
+    // parameter structs are passed by value to every step, failures are
+    // printed and the step returns its measured time in ms.
+    float UpdateSystem(float renderInterval) {
+        float ms = 0.f;
+        if (sf_set_params(mHandle, &CUDA_SEEPAGEFLOW_PARAMS) != SF_OK ||
+            sf_update_system(mHandle, renderInterval, &ms) != SF_OK)
+            std::printf("UpdateSystem failed: %s\n", sf_last_error());
+        return ms;
+    }
+    size_t GetNumOfSubTimeSteps() const {
+        size_t k = 1;
+        sf_num_sub_steps(mHandle, &k);
+        return k;
+    }
+    bool GetAdaptiveSubTimeStep() const { return bAdaptiveSubTimeStep; }
+    size_t NumOfParticles() const { return mParticles->Size(); }
+    size_t MaxNumOfParticles() const { return mParticles->MaxSize(); }
+    CudaSFParticlesPtr GetSFParticles() const { return mParticles; }
+    sf_handle Handle() const { return mHandle; }
+
+  private:
+    CudaSFParticlesPtr mParticles;
+    CudaBoundaryParticlesPtr mBoundaries;
+    CudaSphSFSolverPtr mSolver;
+    CudaGNSearcherPtr mSearcher;
+    CudaGNBoundarySearcherPtr mBoundarySearcher;
+    bool bAdaptiveSubTimeStep;
+    sf_handle mHandle = nullptr;
+};
+typedef SharedPtr<CudaSFSystem> CudaSFSystemPtr;
+
+// UInt2Str4Digit (kiri_utils.h:36-42)
+inline std::string UInt2Str4Digit(unsigned input) {
+    char output[16];
+    std::snprintf(output, sizeof(output), "%04u", input);
+    return std::string(output);
+}
+
+// ExportBgeoFileCUDA (kiri_utils.h:364-448): folder/file_name.bgeo with
+// position, Cd, pscale and label of the particles' current state.
+inline void ExportBgeoFileCUDA(const std::string& folder, const std::string& file_name,
+                               const CudaSFParticlesPtr& particles) {
+    if (sf_export_bgeo(particles->Handle(), folder.c_str(), file_name.c_str()) != SF_OK)
+        std::printf("ExportBgeoFileCUDA failed: %s\n", sf_last_error());
+}
+
+}  // namespace KIRI

@@ -90,7 +90,7 @@ def make_params(ph: Physics):
     p.sph_bnu = f32(np.float32(ph.boundary_friction) * h * cs)
     p.dem_density = f32(ph.rho_sand)
     p.dem_particle_radius = f32(rs)
-    p.dem_mass = f32(np.float32(ph.rho_sand) * (np.float32(4.0 / 3.0) * np.float32(3.1415926) * rs ** 3))
+    p.dem_mass = f32(np.float32(ph.rho_sand) * (np.float32(4.0 / 3.0) * np.float32(3.1415926) * (rs * rs * rs)))
     p.dem_kernel_radius = f32(np.float32(4.0) * rs)
     p.dem_young = f32(ph.young)
     p.dem_poisson = f32(ph.poisson)
