@@ -39,25 +39,26 @@ SF_DEV float eos(const SFConst& k, float rho) {
 // over the compacted lists, so a wave no longer executes every body for every
 // candidate that some lane happens to keep.  The flush is decided wave-wide
 // before appending, so every list stays within kCap.
-constexpr int kCap = 24;
+constexpr int kCap = 16;
+constexpr int kNB = kBlock;  // threads per workgroup of the neighbour passes
 
 struct NList {
-    uint32_t* slot;  // this lane's column of the LDS list: slot[k * kBlock]
+    uint32_t* slot;  // this lane's column of the LDS list: slot[k * kNB]
     int cnt;
 };
 #define SF_NLIST(name)                                   \
-    __shared__ uint32_t name##_lds[kCap * kBlock];       \
+    __shared__ uint32_t name##_lds[kCap * kNB];          \
     NList name{name##_lds + threadIdx.x, 0}
 
 template <class Body>
 SF_DEV void nl_flush(NList& L, Body&& body) {
-    for (int k = 0; k < L.cnt; ++k) body(L.slot[k * kBlock]);
+    for (int k = 0; k < L.cnt; ++k) body(L.slot[k * kNB]);
     L.cnt = 0;
 }
 template <class Body>
 SF_DEV void nl_push(NList& L, bool keep, uint32_t j, Body&& body) {
     if (__ballot(L.cnt == kCap)) nl_flush(L, body);
-    L.slot[L.cnt * kBlock] = j;  // cnt < kCap here; a dropped slot is overwritten later
+    L.slot[L.cnt * kNB] = j;  // cnt < kCap here; a dropped slot is overwritten later
     L.cnt += keep ? 1 : 0;
 }
 
@@ -95,8 +96,8 @@ SF_DEV float sq3(float d) { return d * d * d; }
 // P1: water density (self counted twice, :466 + loop) and pressure; sand gets
 // rho_sand and p = 0.  Branch-free: max(h^2 - r^2, 0) vanishes outside the
 // support and max(w, 0) drops sand (w = -mass).
-__global__ __launch_bounds__(kBlock) void k_density(SFConst k, Fields f, int n) {
-    const int i = xcd_block() * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kNB) void k_density(SFConst k, Fields f, int n) {
+    const int i = xcd_block() * kNB + threadIdx.x;
     if (i >= n) return;
     const float4 pi = ldo(f.pm, (uint32_t)i);
     if (pi.w < 0.f) {
@@ -128,7 +129,7 @@ __global__ __launch_bounds__(kBlock) void k_density(SFConst k, Fields f, int n) 
 }
 void launch_density(const SFConst& k, const Fields& f, int n, hipStream_t s) {
     if (n <= 0) return;
-    k_density<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, f, n);
+    k_density<<<(n + kNB - 1) / kNB, kNB, 0, s>>>(k, f, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -248,9 +249,9 @@ SF_DEV void near_span(float p, float lo, float h, int c, int g, float rc, int& a
     b = (c < g - 1 && cl + h - p < rc) ? c + 1 : c;
 }
 
-__global__ __launch_bounds__(kBlock) void k_sand_momentum(SFConst k, Fields f, int n) {
+__global__ __launch_bounds__(kNB) void k_sand_momentum(SFConst k, Fields f, int n) {
     SF_NLIST(L);
-    const int i = xcd_block() * kBlock + threadIdx.x;
+    const int i = xcd_block() * kNB + threadIdx.x;
     if (i >= n) return;
     const float4 pi = ldo(f.pm, (uint32_t)i);
     if (pi.w >= 0.f) return;
@@ -364,15 +365,15 @@ __global__ __launch_bounds__(kBlock) void k_sand_momentum(SFConst k, Fields f, i
 }
 void launch_sand_momentum(const SFConst& k, const Fields& f, int n, hipStream_t s) {
     if (n <= 0) return;
-    k_sand_momentum<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, f, n);
+    k_sand_momentum<<<(n + kNB - 1) / kNB, kNB, 0, s>>>(k, f, n);
 }
 
 // ---------------------------------------------------------------------------
 // P5: water linear momentum (pressure + artificial viscosity + seepage +
 // boundary pressure/viscosity), acc = g + a - F_adh / m.
-__global__ __launch_bounds__(kBlock) void k_water_momentum(SFConst k, Fields f, int n) {
+__global__ __launch_bounds__(kNB) void k_water_momentum(SFConst k, Fields f, int n) {
     SF_NLIST(L);
-    const int i = xcd_block() * kBlock + threadIdx.x;
+    const int i = xcd_block() * kNB + threadIdx.x;
     if (i >= n) return;
     const float4 pi = ldo(f.pm, (uint32_t)i);
     if (pi.w < 0.f) return;
@@ -443,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void k_water_momentum(SFConst k, Fields f, 
 }
 void launch_water_momentum(const SFConst& k, const Fields& f, int n, hipStream_t s) {
     if (n <= 0) return;
-    k_water_momentum<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, f, n);
+    k_water_momentum<<<(n + kNB - 1) / kNB, kNB, 0, s>>>(k, f, n);
 }
 
 // ---------------------------------------------------------------------------
