@@ -27,6 +27,7 @@
  *   sf_write_bgeo        <- Partio::writeBGEO (uncompressed V5)  extlib/include/partio/io/BGEO.cpp:311-476
  *   sf_build_world_boundary <- CudaBoundaryEmitter::BuildWorldBoundary  cuda_boundary_emitter.cpp:12-80
  *   sf_grid_size         <- make_int3((highest - lowest) / kernel_radius)  main.cpp:104, cuda_neighbor_searcher.cu:26
+ *   sf_batch_*, sf_step_graph, sf_*_launch_config, sf_autotune: no counterpart (see below)
  *
  * Threading: a handle is single-threaded; all its GPU work is ordered on the
  * handle's own HIP stream (sf_set_stream can replace it).
@@ -41,7 +42,7 @@
 extern "C" {
 #endif
 
-#define SF_ABI_VERSION 1
+#define SF_ABI_VERSION 2
 
 /* ---- error codes ------------------------------------------------------- */
 #define SF_OK 0
@@ -222,6 +223,46 @@ int sf_download(sf_handle h, int field, void* dst, size_t dst_bytes);
 int sf_device_buffer(sf_handle h, int field, void** dptr, size_t* stride_bytes);
 /* Writes folder/name.bgeo from the current state (creates folder). */
 int sf_export_bgeo(sf_handle h, const char* folder, const char* name);
+
+/* ---- batched scenes ------------------------------------------------------
+ * No counterpart in the reference (one CudaSFSystem per process, main.cpp:106-221):
+ * many independent scenes of uneven particle counts in one handle, every pass
+ * one launch over all of them.  Scenes share the box, grid, boundary particles
+ * and every sf_params field except these, whose names follow sf_params. */
+typedef struct sf_scene_params {
+    float dem_tan_friction_angle;
+    float sf_c0, sf_cmc, sf_cmc_p, sf_csat; /* capillary Bezier controls */
+    float sf_cd;                           /* drag coefficient (SF mode)  */
+} sf_scene_params;
+/* counts[s] particles for scene s; the handle's capacity is their sum.  Upload
+ * with sf_upload_particles, scene-major (scene 0's particles first).  Scene s
+ * stays in the slot range [off[s], off[s+1]) of every field for the handle's
+ * lifetime (off from sf_batch_scene_offsets); within it the particles are in
+ * cell order, exactly as a single-scene handle of that scene keeps them.
+ * Batched handles step with a fixed dt (sf_step*, sf_step_graph). */
+int sf_batch_create(const sf_params* p, const sf_boundary_params* b, const sf_solver_options* opts,
+                    size_t n_scenes, const size_t* counts, const sf_scene_params* scenes, int device,
+                    sf_handle* out);
+int sf_batch_set_scene_params(sf_handle h, const sf_scene_params* scenes);
+/* offsets: n_scenes + 1 entries; *n_scenes may be NULL. */
+int sf_batch_scene_offsets(sf_handle h, size_t* n_scenes, size_t* offsets, size_t capacity);
+
+/* ---- graph replay and launch tuning -------------------------------------
+ * No counterpart in the reference (which synchronises after every pass,
+ * kiri_pbs_pch.cuh:35-50).  sf_step_graph captures two fixed-dt steps into a
+ * HIP graph once (again after any change of dt, parameters, uploads or launch
+ * shapes) and replays it; results are identical to sf_step_n. */
+int sf_step_graph(sf_handle h, float dt, int n_steps);
+/* Launch shape of a neighbour pass (SF_PASS_DENSITY .. SF_PASS_WATER_MOMENTUM):
+ * workgroup size nb in {64, 128, 256}; LDS candidate-list depth cap in {8, 16}
+ * (sand / water momentum only).  Results do not depend on the shape. */
+int sf_set_launch_config(sf_handle h, int pass, int nb, int cap);
+int sf_get_launch_config(sf_handle h, int pass, int* nb, int* cap);
+/* Times every shape of every neighbour pass on the handle's current particles
+ * (reps launches each, after one neighbour build) and keeps the fastest.
+ * best_ms / default_ms (5 floats each, may be NULL) receive per pass the
+ * chosen and the default (256, 16) shape's mean time. */
+int sf_autotune(sf_handle h, int reps, float* best_ms, float* default_ms);
 
 #ifdef __cplusplus
 }

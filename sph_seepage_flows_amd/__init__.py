@@ -9,10 +9,10 @@ from ._abi import (SF_SPH_SOLVER, SF_TYPE_MULTI_SF, SF_TYPE_SF, SF_WCSPH_SOLVER,
                    sf_params, sf_solver_options)
 from .params import Physics, make_params
 from .scenes import Scene, drop_in_1m, mixed, small_block, world_boundary
-from .system import SFSystem, write_bgeo
+from .system import BatchSystem, SFSystem, write_bgeo
 
 __all__ = [
-    "SFError", "SFSystem", "Scene", "Physics", "make_params", "small_block", "drop_in_1m", "mixed",
+    "SFError", "SFSystem", "BatchSystem", "Scene", "Physics", "make_params", "small_block", "drop_in_1m", "mixed",
     "world_boundary", "write_bgeo", "lib", "sf_params", "sf_boundary_params", "sf_solver_options",
     "SF_SPH_SOLVER", "SF_WCSPH_SOLVER", "SF_TYPE_SF", "SF_TYPE_MULTI_SF",
 ]

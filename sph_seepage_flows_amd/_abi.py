@@ -93,6 +93,12 @@ class sf_solver_options(C.Structure):
                 ("speed_of_sound", C.c_float)]
 
 
+class sf_scene_params(C.Structure):
+    """Per-scene overrides of a batched handle (names as in sf_params)."""
+    _fields_ = [(n, C.c_float) for n in ("dem_tan_friction_angle", "sf_c0", "sf_cmc", "sf_cmc_p", "sf_csat",
+                                         "sf_cd")]
+
+
 _lib = None
 
 _VP = C.c_void_p
@@ -122,6 +128,14 @@ _SIG = {
     "sf_download": (C.c_int, [_VP, C.c_int, _VP, C.c_size_t]),
     "sf_device_buffer": (C.c_int, [_VP, C.c_int, C.POINTER(_VP), C.POINTER(C.c_size_t)]),
     "sf_export_bgeo": (C.c_int, [_VP, C.c_char_p, C.c_char_p]),
+    "sf_batch_create": (C.c_int, [C.POINTER(sf_params), C.POINTER(sf_boundary_params), C.POINTER(sf_solver_options),
+                                  C.c_size_t, _VP, _VP, C.c_int, C.POINTER(_VP)]),
+    "sf_batch_set_scene_params": (C.c_int, [_VP, _VP]),
+    "sf_batch_scene_offsets": (C.c_int, [_VP, C.POINTER(C.c_size_t), _VP, C.c_size_t]),
+    "sf_step_graph": (C.c_int, [_VP, C.c_float, C.c_int]),
+    "sf_set_launch_config": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int]),
+    "sf_get_launch_config": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "sf_autotune": (C.c_int, [_VP, C.c_int, _VP, _VP]),
 }
 
 

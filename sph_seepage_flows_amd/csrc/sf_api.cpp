@@ -144,6 +144,20 @@ struct sf_system {
     DevBuf<float> cfl_part;
     DevBuf<CflOut> cfl_out;
     size_t nsub = 1;
+    // batched scenes (nscenes == 0: one scene)
+    size_t nscenes = 0;
+    std::vector<size_t> soff;
+    std::vector<sf_scene_params> sp;
+    DevBuf<uint16_t> scene;
+    DevBuf<SceneConst> sct;
+    // launch shapes of the neighbour passes (SF_PASS_DENSITY ..)
+    PassCfg cfg[5];
+    // graph of two fixed-dt steps
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    float graph_dt = 0.f;
+    int graph_parity = -1;
+    bool graph_dirty = true;
     // diagnostics
     DevBuf<uint32_t> nbr_cnt, nbr_cand;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -167,7 +181,18 @@ struct sf_system {
         f.adh = adh.p;
         f.xf = xf.p;
         f.acc = acc.p;
+        f.scene = nscenes ? scene.p : nullptr;
+        f.sct = nscenes ? sct.p : nullptr;
         return f;
+    }
+    uint32_t local_cells() const { return K.C; }
+    int parity() const { return cur | (cur_cs << 1) | (cur_aux << 2); }
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+        graph_dirty = true;
     }
 };
 
@@ -201,7 +226,7 @@ int build_neighbors(sf_system* h, hipEvent_t* ev = nullptr) {
     const int n = (int)h->n;
     hipStream_t s = h->stream;
     SF_MARK(ev, SF_PASS_HASH);
-    launch_hash(h->K, h->pm[h->cur].p, h->keys.p, n, s);
+    launch_hash(h->K, h->pm[h->cur].p, h->nscenes ? h->scene.p : nullptr, h->keys.p, n, s);
     SortBuffers sb{h->kA.p, h->vA.p, h->kB.p, h->vB.p, h->hist.p};
     SF_MARK(ev, SF_PASS_SORT);
     int rc = radix_sort(h->keys.p, n, h->plan, sb, &h->sorted_keys, &h->perm, s);
@@ -237,15 +262,15 @@ int solve(sf_system* h, hipEvent_t* ev = nullptr) {
     hipStream_t s = h->stream;
     Fields f = h->fields();
     SF_MARK(ev, SF_PASS_DENSITY);
-    launch_density(h->K, f, n, s);
+    launch_density(h->K, f, n, h->cfg[0], s);
     SF_MARK(ev, SF_PASS_AVG_FLOW);
-    launch_avg_flow(h->K, f, n, s);
+    launch_avg_flow(h->K, f, n, h->cfg[1], s);
     SF_MARK(ev, SF_PASS_VOIDAGE_ADHESION);
-    launch_voidage_adhesion(h->K, f, n, s);
+    launch_voidage_adhesion(h->K, f, n, h->cfg[2], s);
     SF_MARK(ev, SF_PASS_SAND_MOMENTUM);
-    launch_sand_momentum(h->K, f, n, s);
+    launch_sand_momentum(h->K, f, n, h->cfg[3], s);
     SF_MARK(ev, SF_PASS_WATER_MOMENTUM);
-    launch_water_momentum(h->K, f, n, s);
+    launch_water_momentum(h->K, f, n, h->cfg[4], s);
     SF_HIP(hipGetLastError());
     return SF_OK;
 }
@@ -304,6 +329,52 @@ int fetch(sf_system* h, const T* src, size_t n, std::vector<T>& out) {
     return SF_OK;
 }
 
+// One neighbour pass with an explicit launch shape (autotuner).
+void launch_pass(sf_system* h, int q, PassCfg c) {
+    const Fields f = h->fields();
+    const int n = (int)h->n;
+    switch (q) {
+    case 0: launch_density(h->K, f, n, c, h->stream); break;
+    case 1: launch_avg_flow(h->K, f, n, c, h->stream); break;
+    case 2: launch_voidage_adhesion(h->K, f, n, c, h->stream); break;
+    case 3: launch_sand_momentum(h->K, f, n, c, h->stream); break;
+    default: launch_water_momentum(h->K, f, n, c, h->stream); break;
+    }
+}
+
+// sf_params of scene s: the handle's parameters with the scene's overrides.
+sf_params scene_params(const sf_params& P, const sf_scene_params& q) {
+    sf_params r = P;
+    r.dem_tan_friction_angle = q.dem_tan_friction_angle;
+    r.sf_c0 = q.sf_c0;
+    r.sf_cmc = q.sf_cmc;
+    r.sf_cmc_p = q.sf_cmc_p;
+    r.sf_csat = q.sf_csat;
+    r.sf_cd = q.sf_cd;
+    return r;
+}
+
+// Per-scene constants from make_const of each scene's own sf_params, so a
+// scene computes exactly what a single-scene handle with those parameters does.
+int upload_scene_table(sf_system* h) {
+    if (!h->nscenes) return SF_OK;
+    std::vector<SceneConst> t(h->nscenes);
+    for (size_t q = 0; q < h->nscenes; ++q) {
+        const sf_params ps = scene_params(h->P, h->sp[q]);
+        int rc = validate(&ps, &h->B);
+        if (rc) return set_error(SF_E_INVALID, "scene " + std::to_string(q) + ": " + sf_last_error());
+        const SFConst k = make_const(ps, h->B, h->O, h->blabel1);
+        t[q].tan2 = k.tan2;
+        t[q].tanphi = k.tanphi;
+        t[q].cd = k.cd;
+        t[q].pad = 0.f;
+        t[q].gcap = k.gcap;
+    }
+    SF_HIP(hipMemcpyAsync(h->sct.p, t.data(), t.size() * sizeof(SceneConst), hipMemcpyHostToDevice, h->stream));
+    SF_HIP(hipStreamSynchronize(h->stream));
+    return SF_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -353,7 +424,7 @@ int sf_create(const sf_params* p, const sf_boundary_params* b, const sf_solver_o
     if ((rc = h->rp.alloc(c)) || (rc = h->es.alloc(c)) || (rc = h->wt.alloc(c)) || (rc = h->flow.alloc(c)) || (rc = h->adh.alloc(c)) ||
         (rc = h->xf.alloc(c)) || (rc = h->acc.alloc(c)))
         return bail(rc);
-    if ((rc = h->cell.alloc((size_t)h->C + 1)) || (rc = h->bcell.alloc((size_t)h->C + 1)) ||
+    if ((rc = h->cell.alloc((size_t)h->C + 1)) || (rc = h->bcell.alloc((size_t)h->local_cells() + 1)) ||
         (rc = h->ct.alloc((size_t)h->C + 1)))
         return bail(rc);
     if ((rc = h->cfl_part.alloc(cfl_partials(0))) || (rc = h->cfl_out.alloc(1))) return bail(rc);
@@ -382,6 +453,7 @@ int sf_destroy(sf_handle h) {
     if (!h) return SF_OK;
     (void)hipSetDevice(h->device);
     if (h->own_stream) (void)hipStreamSynchronize(h->own_stream);
+    h->drop_graph();
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
@@ -396,8 +468,15 @@ int sf_set_params(sf_handle h, const sf_params* p) {
     if ((rc = validate(p, &h->B))) return rc;
     if (p->sf_type != h->P.sf_type && h->uploaded)
         return set_error(SF_E_STATE, "sf_type cannot change after particles are uploaded");
+    const sf_params old = h->P;
     h->P = *p;
     h->K = make_const(h->P, h->B, h->O, h->blabel1);
+    if ((rc = upload_scene_table(h))) {
+        h->P = old;
+        h->K = make_const(h->P, h->B, h->O, h->blabel1);
+        return rc;
+    }
+    h->drop_graph();
     return SF_OK;
 }
 
@@ -421,6 +500,8 @@ int sf_upload_particles(sf_handle h, size_t n, const float* pos3, const float* v
     int rc;
     if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
     if (n > h->cap) return set_error(SF_E_CAPACITY, "particle count exceeds the handle's max_num");
+    if (h->nscenes && n != h->cap)
+        return set_error(SF_E_INVALID, "a batched handle takes exactly the sum of its scene counts");
     if (n > 0 && (!pos3 || !label || !mass)) return set_error(SF_E_INVALID, "pos3, label and mass are required");
     std::vector<float4> pm(n), vr(n), cs(n), cda(n);
     std::vector<float2> amc(n);
@@ -448,6 +529,7 @@ int sf_upload_particles(sf_handle h, size_t n, const float* pos3, const float* v
     h->n = n;
     h->uploaded = true;
     h->built = false;
+    h->drop_graph();
     return SF_OK;
 }
 
@@ -474,7 +556,7 @@ int sf_upload_boundary(sf_handle h, size_t nb, const float* pos3, const uint64_t
     SortBuffers sb{h->kA.p, h->vA.p, h->kB.p, h->vB.p, h->hist.p};
     const uint32_t *sk = nullptr, *bperm = nullptr;
     if ((rc = radix_sort(h->keys.p, (int)nb, h->plan, sb, &sk, &bperm, h->stream))) return rc;
-    launch_cell_start(sk, (int)nb, h->C, nullptr, h->bcell.p, nullptr, h->stream);
+    launch_cell_start(sk, (int)nb, h->local_cells(), nullptr, h->bcell.p, nullptr, h->stream);
     launch_boundary_gather(bperm, tmp.p, h->bpv.p, (int)nb, h->stream);
     launch_boundary_volume(h->K, h->bpv.p, h->bcell.p, (int)nb, h->stream);
     SF_HIP(hipGetLastError());
@@ -483,7 +565,8 @@ int sf_upload_boundary(sf_handle h, size_t nb, const float* pos3, const uint64_t
     h->blabel1 = l1;
     h->K = make_const(h->P, h->B, h->O, h->blabel1);
     h->built = false;  // the particle searcher buffers were reused
-    return SF_OK;
+    h->drop_graph();
+    return upload_scene_table(h);
 }
 
 int sf_build_neighbors(sf_handle h) {
@@ -550,6 +633,7 @@ int sf_update_system(sf_handle h, float render_interval, float* ms_out) {
     if (!h->uploaded) return set_error(SF_E_STATE, "no particles uploaded");
     if (!std::isfinite(render_interval) || !(render_interval > 0.f))
         return set_error(SF_E_INVALID, "render interval must be finite and > 0");
+    if (h->nscenes) return set_error(SF_E_STATE, "batched handles step with a fixed dt (sf_step, sf_step_graph)");
     SF_HIP(hipEventRecord(h->ev0, h->stream));
     if ((rc = build_neighbors(h))) return rc;
     if (h->P.solver_type == SF_WCSPH_SOLVER) {
@@ -696,14 +780,15 @@ int sf_download(sf_handle h, int field, void* dst, size_t dst_bytes) {
         // pre-advect state is gone after a step, so count on the current
         // positions with the current cellStart only directly after
         // sf_build_neighbors (callers use it that way).
-        launch_nbr_count(h->K, h->pm[h->cur].p, h->cell.p, h->nbr_cnt.p, h->nbr_cand.p, (int)n, h->stream);
+        launch_nbr_count(h->K, h->pm[h->cur].p, h->nscenes ? h->scene.p : nullptr, h->cell.p, h->nbr_cnt.p,
+                         h->nbr_cand.p, (int)n, h->stream);
         SF_HIP(hipGetLastError());
         return putu(field == SF_FIELD_NBR_COUNT ? h->nbr_cnt.p : h->nbr_cand.p, n);
     }
     case SF_FIELD_B_POS: return put3(h->bpv.p, nb, -1);
     case SF_FIELD_B_LABEL: return labels(h->bpv.p, nb);
     case SF_FIELD_B_VOLUME: return put1w(h->bpv.p, nb, true);
-    case SF_FIELD_B_CELL_START: return putu(h->bcell.p, (size_t)h->C + 1);
+    case SF_FIELD_B_CELL_START: return putu(h->bcell.p, (size_t)h->local_cells() + 1);
     default: return set_error(SF_E_NOTFOUND, "unknown field id " + std::to_string(field));
     }
 }
@@ -753,6 +838,171 @@ int sf_export_bgeo(sf_handle h, const char* folder, const char* name) {
     if ((rc = make_dirs(folder))) return rc;
     std::string path = std::string(folder) + "/" + name + ".bgeo";
     return sf_write_bgeo(path.c_str(), n, pos.data(), col.data(), rad.data(), lab.data());
+}
+
+int sf_batch_create(const sf_params* p, const sf_boundary_params* b, const sf_solver_options* opts, size_t n_scenes,
+                    const size_t* counts, const sf_scene_params* scenes, int device, sf_handle* out) {
+    if (!out) return set_error(SF_E_INVALID, "null output handle");
+    *out = nullptr;
+    if (n_scenes == 0 || n_scenes > 65535) return set_error(SF_E_INVALID, "n_scenes must be in [1, 65535]");
+    if (!counts || !scenes) return set_error(SF_E_INVALID, "null counts or scene parameters");
+    int rc = validate(p, b);
+    if (rc) return rc;
+    std::vector<size_t> off(n_scenes + 1, 0);
+    for (size_t q = 0; q < n_scenes; ++q) off[q + 1] = off[q] + counts[q];
+    const uint64_t cells = (uint64_t)b->grid_size.x * b->grid_size.y * b->grid_size.z;
+    if (cells * n_scenes >= ((uint64_t)1 << 31)) return set_error(SF_E_CAPACITY, "scenes x cells above 2^31");
+    sf_handle h = nullptr;
+    if ((rc = sf_create(p, b, opts, off.back(), device, &h))) return rc;
+    h->nscenes = n_scenes;
+    h->soff = off;
+    h->sp.assign(scenes, scenes + n_scenes);
+    h->C = (uint32_t)(cells * n_scenes);
+    h->plan = make_radix_plan(h->C);
+    std::vector<uint16_t> sid(std::max<size_t>(off.back(), 1), 0);
+    for (size_t q = 0; q < n_scenes; ++q)
+        for (size_t i = off[q]; i < off[q + 1]; ++i) sid[i] = (uint16_t)q;
+    auto bail = [&](int code) {
+        sf_destroy(h);
+        return code;
+    };
+    if ((rc = h->cell.alloc((size_t)h->C + 1)) || (rc = h->ct.alloc((size_t)h->C + 1)) ||
+        (rc = h->scene.alloc(sid.size())) || (rc = h->sct.alloc(n_scenes)))
+        return bail(rc);
+    {
+        hipError_t e;
+        if ((e = hipMemsetAsync(h->cell.p, 0, h->cell.n * sizeof(uint32_t), h->stream)) != hipSuccess ||
+            (e = hipMemsetAsync(h->ct.p, 0, h->ct.n * sizeof(uint2), h->stream)) != hipSuccess ||
+            (e = hipMemcpyAsync(h->scene.p, sid.data(), sid.size() * sizeof(uint16_t), hipMemcpyHostToDevice,
+                                h->stream)) != hipSuccess)
+            return bail(hip_error(e, "batch buffers"));
+    }
+    if ((rc = upload_scene_table(h))) return bail(rc);
+    *out = h;
+    return SF_OK;
+}
+
+int sf_batch_set_scene_params(sf_handle h, const sf_scene_params* scenes) {
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!h->nscenes) return set_error(SF_E_STATE, "not a batched handle");
+    if (!scenes) return set_error(SF_E_INVALID, "null scene parameters");
+    std::vector<sf_scene_params> old = h->sp;
+    h->sp.assign(scenes, scenes + h->nscenes);
+    if ((rc = upload_scene_table(h))) {
+        h->sp = old;
+        (void)upload_scene_table(h);
+        return rc;
+    }
+    return SF_OK;
+}
+
+int sf_batch_scene_offsets(sf_handle h, size_t* n_scenes, size_t* offsets, size_t capacity) {
+    int rc;
+    if ((rc = check_handle(h))) return rc;
+    if (n_scenes) *n_scenes = h->nscenes;
+    if (!offsets) return SF_OK;
+    if (!h->nscenes) return set_error(SF_E_STATE, "not a batched handle");
+    if (capacity < h->nscenes + 1) return set_error(SF_E_INVALID, "offsets needs n_scenes + 1 entries");
+    std::copy(h->soff.begin(), h->soff.end(), offsets);
+    return SF_OK;
+}
+
+int sf_step_graph(sf_handle h, float dt, int n_steps) {
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!h->uploaded) return set_error(SF_E_STATE, "no particles uploaded");
+    if (!std::isfinite(dt) || !(dt > 0.f)) return set_error(SF_E_INVALID, "dt must be finite and > 0");
+    if (n_steps < 0) return set_error(SF_E_INVALID, "n_steps must be >= 0");
+    if (n_steps >= 2 && (h->graph_dirty || !h->gexec || h->graph_dt != dt || h->graph_parity != h->parity())) {
+        h->drop_graph();
+        const int parity = h->parity();
+        SF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        rc = one_step(h, dt);
+        if (!rc) rc = one_step(h, dt);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(h->stream, &g);
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : hip_error(e, "hipStreamEndCapture");
+        }
+        h->graph = g;
+        SF_HIP(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+        h->graph_dt = dt;
+        h->graph_parity = parity;
+        h->graph_dirty = false;
+        if (h->parity() != parity) return set_error(SF_E_STATE, "two steps did not restore the buffer parity");
+    }
+    for (int q = 0; q + 1 < n_steps; q += 2) SF_HIP(hipGraphLaunch(h->gexec, h->stream));
+    if (n_steps & 1)
+        if ((rc = one_step(h, dt))) return rc;
+    SF_HIP(hipStreamSynchronize(h->stream));
+    return SF_OK;
+}
+
+int sf_set_launch_config(sf_handle h, int pass, int nb, int cap) {
+    int rc;
+    if ((rc = check_handle(h))) return rc;
+    if (pass < SF_PASS_DENSITY || pass > SF_PASS_WATER_MOMENTUM)
+        return set_error(SF_E_INVALID, "pass must be one of the neighbour passes");
+    if (std::find(std::begin(kNbChoices), std::end(kNbChoices), nb) == std::end(kNbChoices) ||
+        std::find(std::begin(kCapChoices), std::end(kCapChoices), cap) == std::end(kCapChoices))
+        return set_error(SF_E_INVALID, "nb must be 64, 128 or 256 and cap 8 or 16");
+    h->cfg[pass - SF_PASS_DENSITY] = PassCfg{nb, cap};
+    h->drop_graph();
+    return SF_OK;
+}
+
+int sf_get_launch_config(sf_handle h, int pass, int* nb, int* cap) {
+    int rc;
+    if ((rc = check_handle(h))) return rc;
+    if (pass < SF_PASS_DENSITY || pass > SF_PASS_WATER_MOMENTUM)
+        return set_error(SF_E_INVALID, "pass must be one of the neighbour passes");
+    if (nb) *nb = h->cfg[pass - SF_PASS_DENSITY].nb;
+    if (cap) *cap = h->cfg[pass - SF_PASS_DENSITY].cap;
+    return SF_OK;
+}
+
+int sf_autotune(sf_handle h, int reps, float* best_ms, float* default_ms) {
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!h->uploaded) return set_error(SF_E_STATE, "no particles uploaded");
+    if (reps < 1 || reps > 1000) return set_error(SF_E_INVALID, "reps must be in [1, 1000]");
+    // A neighbour build reorders the particles exactly as the next step's
+    // first pass would; every neighbour pass is then idempotent (it rewrites
+    // the same outputs from unchanged inputs), so each is timed in place, in
+    // dependency order, without disturbing the trajectory.
+    if ((rc = build_neighbors(h))) return rc;
+    if ((rc = solve(h))) return rc;
+    for (int q = 0; q < 5; ++q) {
+        const bool lists = q >= 3;
+        float best = 0.f, dflt = 0.f;
+        PassCfg pick{256, 16};
+        for (int nb : kNbChoices)
+            for (int cap : kCapChoices) {
+                if (!lists && cap != 16) continue;
+                const PassCfg c{nb, cap};
+                launch_pass(h, q, c);  // warm-up
+                SF_HIP(hipEventRecord(h->ev0, h->stream));
+                for (int r = 0; r < reps; ++r) launch_pass(h, q, c);
+                SF_HIP(hipEventRecord(h->ev1, h->stream));
+                SF_HIP(hipEventSynchronize(h->ev1));
+                SF_HIP(hipGetLastError());
+                float ms = 0.f;
+                SF_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+                ms /= (float)reps;
+                if (nb == 256 && cap == 16) dflt = ms;
+                if (best == 0.f || ms < best) {
+                    best = ms;
+                    pick = c;
+                }
+            }
+        h->cfg[q] = pick;
+        if (best_ms) best_ms[q] = best;
+        if (default_ms) default_ms[q] = dflt;
+    }
+    h->drop_graph();
+    return SF_OK;
 }
 
 }  // extern "C"

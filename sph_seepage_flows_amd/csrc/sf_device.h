@@ -80,6 +80,15 @@ struct SFConst {
     int blabel1;   // number of boundary particles with label 1 (0 = skip those loops)
 };
 
+// Batched scenes: every scene shares the box, grid and the constants above
+// except these, which it takes from its own row of a per-scene table (the
+// sand-momentum pass is the only reader).  Scene s owns the fixed slot range
+// [off[s], off[s+1]) and the cell keys [s*C, (s+1)*C).
+struct SceneConst {
+    float tan2, tanphi, cd, pad;
+    Bez gcap;
+};
+
 // ---- grid hash (helper_thrust.cuh:56-63), FP32 and exactly as written -----
 SF_DEV int cell_coord(float q, int g) {
     if (!(q >= 0.f)) return 0;

@@ -6,6 +6,7 @@
 
 #include <cerrno>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -40,6 +41,16 @@ int validate(const sf_params* p, const sf_boundary_params* b) {
         return set_error(SF_E_INVALID, "grid_size differs from make_int3((highest - lowest) / kernel_radius)");
     double cells = (double)b->grid_size.x * b->grid_size.y * b->grid_size.z;
     if (cells > 2147483000.0) return set_error(SF_E_INVALID, "grid has more than 2^31 cells");
+    static_assert(offsetof(sf_params, sf_amc_p) == offsetof(sf_params, sph_mass) + 24 * sizeof(float),
+                  "sph_mass .. sf_amc_p must be 25 consecutive floats");
+    {  // the 25 scalar fields sph_mass .. sf_amc_p are consecutive floats
+        const float* f = &p->sph_mass;
+        for (int q = 0; q < 25; ++q)
+            if (!std::isfinite(f[q])) return set_error(SF_E_INVALID, "sf_params has a non-finite field");
+        if (!finite3(p->gravity) || !finite3(p->sf_dry_sand_color) || !finite3(p->sf_wet_sand_color))
+            return set_error(SF_E_INVALID, "sf_params has a non-finite vector field");
+        if (!(p->dt > 0.f) || !std::isfinite(p->dt)) return set_error(SF_E_INVALID, "dt must be finite and > 0");
+    }
     if (!(p->sph_density > 0.f)) return set_error(SF_E_INVALID, "sph_density must be > 0");
     if (!(p->dem_density > 0.f)) return set_error(SF_E_INVALID, "dem_density must be > 0");
     if (!(p->sph_mass > 0.f)) return set_error(SF_E_INVALID, "sph_mass must be > 0");

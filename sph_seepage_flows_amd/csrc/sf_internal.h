@@ -72,7 +72,8 @@ int radix_sort(const uint32_t* keys, int n, const RadixPlan& plan, const SortBuf
                const uint32_t** perm, hipStream_t s);
 
 // ---- searcher launchers (sf_searcher.hip) ----
-void launch_hash(const SFConst& k, const float4* pm, uint32_t* keys, int n, hipStream_t s);
+// scene (batched handles): per-slot scene id, keys offset by scene * C; null for one scene
+void launch_hash(const SFConst& k, const float4* pm, const uint16_t* scene, uint32_t* keys, int n, hipStream_t s);
 // cellStart[c] for c in [0, C] plus the cell table ct[c] = (cellStart[c], wpre[cellStart[c]])
 void launch_cell_start(const uint32_t* sorted_keys, int n, uint32_t C, const uint32_t* wpre, uint32_t* cell_start,
                        uint2* ct, hipStream_t s);
@@ -87,8 +88,8 @@ struct GatherArgs {
 void launch_gather(const GatherArgs& a, int n, hipStream_t s);
 void launch_boundary_gather(const uint32_t* perm, const float4* bp_in, float4* bp_out, int nb, hipStream_t s);
 void launch_boundary_volume(const SFConst& k, float4* bpv, const uint32_t* bcs, int nb, hipStream_t s);
-void launch_nbr_count(const SFConst& k, const float4* pm, const uint32_t* cs, uint32_t* cnt, uint32_t* cand, int n,
-                      hipStream_t s);
+void launch_nbr_count(const SFConst& k, const float4* pm, const uint16_t* scene, const uint32_t* cs, uint32_t* cnt,
+                      uint32_t* cand, int n, hipStream_t s);
 void launch_hash_boundary(const SFConst& k, const float4* bp, uint32_t* keys, int nb, hipStream_t s);
 // wpre[0..n]: water count among sorted slots [0, i); part holds wprefix_blocks(n + 1) entries
 int wprefix_blocks(int n);
@@ -110,12 +111,23 @@ struct Fields {
     float4* adh;
     float4* xf;
     float4* acc;
+    // batched handles only (null otherwise): per-slot scene id, per-scene constants
+    const uint16_t* scene;
+    const SceneConst* sct;
 };
-void launch_density(const SFConst& k, const Fields& f, int n, hipStream_t s);
-void launch_avg_flow(const SFConst& k, const Fields& f, int n, hipStream_t s);
-void launch_voidage_adhesion(const SFConst& k, const Fields& f, int n, hipStream_t s);
-void launch_sand_momentum(const SFConst& k, const Fields& f, int n, hipStream_t s);
-void launch_water_momentum(const SFConst& k, const Fields& f, int n, hipStream_t s);
+// Launch shape of a neighbour pass: workgroup size and the per-lane LDS
+// candidate-list capacity (the latter only for the passes that use one).
+struct PassCfg {
+    int nb = 256;
+    int cap = 16;
+};
+constexpr int kNbChoices[3] = {64, 128, 256};
+constexpr int kCapChoices[2] = {8, 16};
+void launch_density(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);
+void launch_avg_flow(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);
+void launch_voidage_adhesion(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);
+void launch_sand_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);
+void launch_water_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);
 // CFL reduction: partial maxima of |acc| -> finalize writes CflOut
 struct CflOut {
     float dt;

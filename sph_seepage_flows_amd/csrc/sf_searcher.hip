@@ -11,21 +11,28 @@ namespace sf {
 
 // ---------------------------------------------------------------------------
 // hash: keys[i] = cell of pm[i]  (Pos2GridHash, helper_thrust.cuh:65-89)
-__global__ __launch_bounds__(kBlock) void k_hash(SFConst k, const float4* __restrict__ pm, uint32_t* __restrict__ keys,
+template <bool kBatch>
+__global__ __launch_bounds__(kBlock) void k_hash(SFConst k, const float4* __restrict__ pm,
+                                                 const uint16_t* __restrict__ scene, uint32_t* __restrict__ keys,
                                                  int n) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const float4 p = pm[i];
     int cx, cy, cz;
     cell_of(k, p.x, p.y, p.z, cx, cy, cz);
-    keys[i] = cell_key(k, cx, cy, cz);
+    uint32_t key = cell_key(k, cx, cy, cz);
+    if constexpr (kBatch) key += (uint32_t)scene[i] * k.C;
+    keys[i] = key;
 }
-void launch_hash(const SFConst& k, const float4* pm, uint32_t* keys, int n, hipStream_t s) {
+void launch_hash(const SFConst& k, const float4* pm, const uint16_t* scene, uint32_t* keys, int n, hipStream_t s) {
     if (n <= 0) return;
-    k_hash<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, pm, keys, n);
+    if (scene)
+        k_hash<true><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, pm, scene, keys, n);
+    else
+        k_hash<false><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, pm, nullptr, keys, n);
 }
 void launch_hash_boundary(const SFConst& k, const float4* bp, uint32_t* keys, int nb, hipStream_t s) {
-    launch_hash(k, bp, keys, nb, s);
+    launch_hash(k, bp, nullptr, keys, nb, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -371,17 +378,19 @@ void launch_wprefix(const float4* pm, int n, uint32_t* part, uint32_t* wpre, hip
 // in-range neighbour counts: r^2 = dx*dx + dy*dy + dz*dz in FP32, left to
 // right without contraction, compared with h^2
 __global__ __launch_bounds__(kBlock) void k_nbr_count(SFConst k, const float4* __restrict__ pm,
+                                                      const uint16_t* __restrict__ scene,
                                                       const uint32_t* __restrict__ cs, uint32_t* __restrict__ cnt,
                                                       uint32_t* __restrict__ cand, int n) {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const float4 pi = pm[i];
+    const uint32_t cb = scene ? (uint32_t)scene[i] * k.C : 0u;
     int cx, cy, cz;
     cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
     uint32_t c = 0, cc = 0;
     for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
-        const uint32_t s0 = cs[c0], e = cs[c1];
+        const uint32_t s0 = cs[cb + c0], e = cs[cb + c1];
         cc += e - s0;
         for (uint32_t j = s0; j < e; ++j) {
 #pragma clang fp contract(off)
@@ -397,10 +406,10 @@ __global__ __launch_bounds__(kBlock) void k_nbr_count(SFConst k, const float4* _
     cnt[i] = c;
     cand[i] = cc;
 }
-void launch_nbr_count(const SFConst& k, const float4* pm, const uint32_t* cs, uint32_t* cnt, uint32_t* cand, int n,
-                      hipStream_t s) {
+void launch_nbr_count(const SFConst& k, const float4* pm, const uint16_t* scene, const uint32_t* cs, uint32_t* cnt,
+                      uint32_t* cand, int n, hipStream_t s) {
     if (n <= 0) return;
-    k_nbr_count<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, pm, cs, cnt, cand, n);
+    k_nbr_count<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, pm, scene, cs, cnt, cand, n);
 }
 
 }  // namespace sf

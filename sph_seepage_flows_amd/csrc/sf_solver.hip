@@ -12,6 +12,8 @@
 //   CFL (optional), advect      <- ComputeSubTimeStepsByCFL, CudaSFParticles::Advect,
 //                                  SFWaterBoundaryConstrain_CUDA
 // ExtraForces (acc = g) is folded into P4/P5, which write the final acc.
+#include <type_traits>
+
 #include "sf_internal.h"
 
 namespace sf {
@@ -38,28 +40,33 @@ SF_DEV float eos(const SFConst& k, float rho) {
 // appends the survivors to the lane's list; the expensive pair bodies then run
 // over the compacted lists, so a wave no longer executes every body for every
 // candidate that some lane happens to keep.  The flush is decided wave-wide
-// before appending, so every list stays within kCap.
-constexpr int kCap = 16;
-constexpr int kNB = kBlock;  // threads per workgroup of the neighbour passes
-
+// before appending, so every list stays within CAP.  NB is the workgroup size.
+template <int NB, int CAP>
 struct NList {
-    uint32_t* slot;  // this lane's column of the LDS list: slot[k * kNB]
+    uint32_t* slot;  // this lane's column of the LDS list: slot[k * NB]
     int cnt;
 };
 #define SF_NLIST(name)                                   \
-    __shared__ uint32_t name##_lds[kCap * kNB];          \
-    NList name{name##_lds + threadIdx.x, 0}
+    __shared__ uint32_t name##_lds[CAP * NB];            \
+    NList<NB, CAP> name{name##_lds + threadIdx.x, 0}
 
-template <class Body>
-SF_DEV void nl_flush(NList& L, Body&& body) {
-    for (int k = 0; k < L.cnt; ++k) body(L.slot[k * kNB]);
+template <int NB, int CAP, class Body>
+SF_DEV void nl_flush(NList<NB, CAP>& L, Body&& body) {
+    for (int k = 0; k < L.cnt; ++k) body(L.slot[k * NB]);
     L.cnt = 0;
 }
-template <class Body>
-SF_DEV void nl_push(NList& L, bool keep, uint32_t j, Body&& body) {
-    if (__ballot(L.cnt == kCap)) nl_flush(L, body);
-    L.slot[L.cnt * kNB] = j;  // cnt < kCap here; a dropped slot is overwritten later
+template <int NB, int CAP, class Body>
+SF_DEV void nl_push(NList<NB, CAP>& L, bool keep, uint32_t j, Body&& body) {
+    if (__ballot(L.cnt == CAP)) nl_flush(L, body);
+    L.slot[L.cnt * NB] = j;  // cnt < CAP here; a dropped slot is overwritten later
     L.cnt += keep ? 1 : 0;
+}
+
+// First cell key of slot i's scene (batched handles), 0 otherwise.
+template <bool B>
+SF_DEV uint32_t scene_base(const SFConst& k, const Fields& f, int i) {
+    if constexpr (B) return (uint32_t)ldo(f.scene, (uint32_t)i) * k.C;
+    else return 0u;
 }
 
 // Walks pm[s, e) with four independent loads in flight per lane (the loop
@@ -81,9 +88,9 @@ SF_DEV void for_range4(const float4* __restrict__ pm, uint32_t s, uint32_t e, F&
 // holds (cellStart[c], water prefix at cellStart[c]) so one 8-byte load per
 // range end gives both.
 template <class F>
-SF_DEV void for_ranges(const SFConst& k, const Fields& f, int cx, int cy, int cz, F&& fn) {
+SF_DEV void for_ranges(const SFConst& k, const Fields& f, uint32_t cb, int cx, int cy, int cz, F&& fn) {
     for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
-        const uint2 a = ldo(f.ct, c0), b = ldo(f.ct, c1);
+        const uint2 a = ldo(f.ct, cb + c0), b = ldo(f.ct, cb + c1);
         if (a.x == b.x) return;
         const uint32_t nw = b.y - a.y;
         fn(a.x, b.x, nw, (b.x - a.x) - nw);
@@ -92,22 +99,43 @@ SF_DEV void for_ranges(const SFConst& k, const Fields& f, int cx, int cy, int cz
 
 SF_DEV float sq3(float d) { return d * d * d; }
 
+// Host-side launch dispatch over (batched?, workgroup size, list capacity);
+// the choices are kNbChoices x kCapChoices.
+template <int V>
+using IC = std::integral_constant<int, V>;
+template <class L>
+void dispatch(const Fields& f, PassCfg c, L&& l) {
+    auto with_cap = [&](auto B, auto NB) {
+        if (c.cap == 8) l(B, NB, IC<8>{});
+        else l(B, NB, IC<16>{});
+    };
+    auto with_nb = [&](auto B) {
+        if (c.nb == 64) with_cap(B, IC<64>{});
+        else if (c.nb == 128) with_cap(B, IC<128>{});
+        else with_cap(B, IC<256>{});
+    };
+    if (f.scene) with_nb(std::true_type{});
+    else with_nb(std::false_type{});
+}
+
 // ---------------------------------------------------------------------------
 // P1: water density (self counted twice, :466 + loop) and pressure; sand gets
 // rho_sand and p = 0.  Branch-free: max(h^2 - r^2, 0) vanishes outside the
 // support and max(w, 0) drops sand (w = -mass).
-__global__ __launch_bounds__(kNB) void k_density(SFConst k, Fields f, int n) {
-    const int i = xcd_block() * kNB + threadIdx.x;
+template <bool B, int NB>
+__global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
+    const int i = xcd_block() * NB + threadIdx.x;
     if (i >= n) return;
     const float4 pi = ldo(f.pm, (uint32_t)i);
     if (pi.w < 0.f) {
         f.rp[i] = make_float2(k.rho_s, 0.f);
         return;
     }
+    const uint32_t cb = scene_base<B>(k, f, i);
     int cx, cy, cz;
     cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
     float sf = 0.f, sb = 0.f;
-    for_ranges(k, f, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t) {
+    for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t) {
         if (nw == 0) return;
         for_range4(f.pm, s, e, [&](uint32_t, const float4& pj) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -127,9 +155,11 @@ __global__ __launch_bounds__(kNB) void k_density(SFConst k, Fields f, int n) {
     const float rho = pi.w * k.w0 + k.w_coef * (sf + k.rho0 * sb);
     f.rp[i] = make_float2(rho, eos(k, rho));
 }
-void launch_density(const SFConst& k, const Fields& f, int n, hipStream_t s) {
+void launch_density(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
     if (n <= 0) return;
-    k_density<<<(n + kNB - 1) / kNB, kNB, 0, s>>>(k, f, n);
+    dispatch(f, c, [&](auto B, auto NB, auto) {
+        k_density<decltype(B)::value, decltype(NB)::value><<<(n + NB - 1) / NB, (int)NB, 0, s>>>(k, f, n);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -140,17 +170,19 @@ void launch_density(const SFConst& k, const Fields& f, int n, hipStream_t s) {
 // its sand neighbours: ranges without the other phase are skipped.)
 // Water particles also store wt = (rho / max(eps, voidage), p / max(eps, that^2)),
 // the per-particle factors the water momentum pass needs from every neighbour.
-__global__ __launch_bounds__(kBlock) void k_avg_flow(SFConst k, Fields f, int n) {
-    const int i = xcd_block() * kBlock + threadIdx.x;
+template <bool B, int NB>
+__global__ __launch_bounds__(NB) void k_avg_flow(SFConst k, Fields f, int n) {
+    const int i = xcd_block() * NB + threadIdx.x;
     if (i >= n) return;
     const float4 pi = ldo(f.pm, (uint32_t)i);
     const bool sand_i = pi.w < 0.f;
+    const uint32_t cb = scene_base<B>(k, f, i);
     int cx, cy, cz;
     cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
     float sandv = 0.f, vsum = 0.f;
     v3 av = mk3(0.f, 0.f, 0.f);
     const float inv_rho_s = k.rho_s != 0.f ? 1.f / k.rho_s : 0.f;
-    for_ranges(k, f, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t ns) {
+    for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t ns) {
         if ((sand_i ? nw : ns) == 0) return;
         for (uint32_t j = s; j < e; ++j) {
             const float4 pj = ldo(f.pm, j);
@@ -174,23 +206,27 @@ __global__ __launch_bounds__(kBlock) void k_avg_flow(SFConst k, Fields f, int n)
         f.wt[i] = make_float2(rt, rpi.y / fmaxf(kEps, rt * rt));
     }
 }
-void launch_avg_flow(const SFConst& k, const Fields& f, int n, hipStream_t s) {
+void launch_avg_flow(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
     if (n <= 0) return;
-    k_avg_flow<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, f, n);
+    dispatch(f, c, [&](auto B, auto NB, auto) {
+        k_avg_flow<decltype(B)::value, decltype(NB)::value><<<(n + NB - 1) / NB, (int)NB, 0, s>>>(k, f, n);
+    });
 }
 
 // ---------------------------------------------------------------------------
 // P3: sand i -> voidage from water neighbours; water i -> adhesion force.
-__global__ __launch_bounds__(kBlock) void k_voidage_adhesion(SFConst k, Fields f, int n) {
-    const int i = xcd_block() * kBlock + threadIdx.x;
+template <bool B, int NB>
+__global__ __launch_bounds__(NB) void k_voidage_adhesion(SFConst k, Fields f, int n) {
+    const int i = xcd_block() * NB + threadIdx.x;
     if (i >= n) return;
     const float4 pi = ldo(f.pm, (uint32_t)i);
     const bool sand_i = pi.w < 0.f;
+    const uint32_t cb = scene_base<B>(k, f, i);
     int cx, cy, cz;
     cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
     if (sand_i) {
         float num = 0.f, den = 0.f;
-        for_ranges(k, f, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t) {
+        for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t) {
             if (nw == 0) return;
             for (uint32_t j = s; j < e; ++j) {
                 const float4 pj = ldo(f.pm, j);
@@ -210,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_voidage_adhesion(SFConst k, Fields f
     const float inv_rho_s = k.rho_s != 0.f ? 1.f / k.rho_s : 0.f;
     float vs = 0.f;
     v3 F = mk3(0.f, 0.f, 0.f);
-    for_ranges(k, f, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t, uint32_t ns) {
+    for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t, uint32_t ns) {
         if (ns == 0) return;
         for (uint32_t j = s; j < e; ++j) {
             const float4 pj = ldo(f.pm, j);
@@ -230,9 +266,11 @@ __global__ __launch_bounds__(kBlock) void k_voidage_adhesion(SFConst k, Fields f
     f.adh[i] = f4(F, 0.f);
     f.xf[i] = vs != 0.f ? f4(F * (1.f / vs), 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
 }
-void launch_voidage_adhesion(const SFConst& k, const Fields& f, int n, hipStream_t s) {
+void launch_voidage_adhesion(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
     if (n <= 0) return;
-    k_voidage_adhesion<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, f, n);
+    dispatch(f, c, [&](auto B, auto NB, auto) {
+        k_voidage_adhesion<decltype(B)::value, decltype(NB)::value><<<(n + NB - 1) / NB, (int)NB, 0, s>>>(k, f, n);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -249,12 +287,21 @@ SF_DEV void near_span(float p, float lo, float h, int c, int g, float rc, int& a
     b = (c < g - 1 && cl + h - p < rc) ? c + 1 : c;
 }
 
-__global__ __launch_bounds__(kNB) void k_sand_momentum(SFConst k, Fields f, int n) {
+template <bool B, int NB, int CAP>
+__global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n) {
     SF_NLIST(L);
-    const int i = xcd_block() * kNB + threadIdx.x;
+    const int i = xcd_block() * NB + threadIdx.x;
     if (i >= n) return;
     const float4 pi = ldo(f.pm, (uint32_t)i);
     if (pi.w >= 0.f) return;
+    const uint32_t cb = scene_base<B>(k, f, i);
+    if constexpr (B) {  // the scene's friction, capillary curve and drag coefficient
+        const SceneConst sc = f.sct[ldo(f.scene, (uint32_t)i)];
+        k.tan2 = sc.tan2;
+        k.tanphi = sc.tanphi;
+        k.cd = sc.cd;
+        k.gcap = sc.gcap;
+    }
     const float mi = -pi.w;
     const v3 xi = xyz(pi);
     const v3 vi = xyz(ldo(f.vr, (uint32_t)i));
@@ -290,7 +337,7 @@ __global__ __launch_bounds__(kNB) void k_sand_momentum(SFConst k, Fields f, int 
         near_span(pi.z, k.loz, k.h, cz, k.gz, rcm, za, zb);
         for (int x = xa; x <= xb; ++x)
             for (int y = ya; y <= yb; ++y) {
-                const uint32_t base = (uint32_t)((x * k.gy + y) * k.gz);
+                const uint32_t base = cb + (uint32_t)((x * k.gy + y) * k.gz);
                 const uint32_t s0 = ldo(f.ct, base + (uint32_t)za).x, e0 = ldo(f.ct, base + (uint32_t)zb + 1u).x;
                 for_range4(f.pm, s0, e0, [&](uint32_t j, const float4& pj) {
                     const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
@@ -319,7 +366,7 @@ __global__ __launch_bounds__(kNB) void k_sand_momentum(SFConst k, Fields f, int 
         }
         fsum += xyz(ldo(f.xf, j)) * (w * vol_i);
     };
-    for_ranges(k, f, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t) {
+    for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t) {
         if (nw == 0) return;
         for_range4(f.pm, s, e, [&](uint32_t j, const float4& pj) {
             const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
@@ -363,20 +410,25 @@ __global__ __launch_bounds__(kNB) void k_sand_momentum(SFConst k, Fields f, int 
         f.cs[i] = c;
     }
 }
-void launch_sand_momentum(const SFConst& k, const Fields& f, int n, hipStream_t s) {
+void launch_sand_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
     if (n <= 0) return;
-    k_sand_momentum<<<(n + kNB - 1) / kNB, kNB, 0, s>>>(k, f, n);
+    dispatch(f, c, [&](auto B, auto NB, auto CAP) {
+        k_sand_momentum<decltype(B)::value, decltype(NB)::value, decltype(CAP)::value>
+            <<<(n + NB - 1) / NB, (int)NB, 0, s>>>(k, f, n);
+    });
 }
 
 // ---------------------------------------------------------------------------
 // P5: water linear momentum (pressure + artificial viscosity + seepage +
 // boundary pressure/viscosity), acc = g + a - F_adh / m.
-__global__ __launch_bounds__(kNB) void k_water_momentum(SFConst k, Fields f, int n) {
+template <bool B, int NB, int CAP>
+__global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int n) {
     SF_NLIST(L);
-    const int i = xcd_block() * kNB + threadIdx.x;
+    const int i = xcd_block() * NB + threadIdx.x;
     if (i >= n) return;
     const float4 pi = ldo(f.pm, (uint32_t)i);
     if (pi.w < 0.f) return;
+    const uint32_t cb = scene_base<B>(k, f, i);
     const float mi = pi.w;
     const v3 vi = xyz(ldo(f.vr, (uint32_t)i));
     const float2 wti = ldo(f.wt, (uint32_t)i);  // (rho~_i, p_i / rho~_i^2)
@@ -412,7 +464,7 @@ __global__ __launch_bounds__(kNB) void k_water_momentum(SFConst k, Fields f, int
             a += xyz(ldo(f.xf, j)) * (-w * inv_rt_i);
         }
     };
-    for_ranges(k, f, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t, uint32_t) {
+    for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t, uint32_t) {
         for_range4(f.pm, s, e, [&](uint32_t j, const float4& pj) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
@@ -442,9 +494,12 @@ __global__ __launch_bounds__(kNB) void k_water_momentum(SFConst k, Fields f, int
     const v3 acc = mk3(k.gvx, k.gvy, k.gvz) + a - ad * (1.f / mi);
     f.acc[i] = f4(acc, 0.f);
 }
-void launch_water_momentum(const SFConst& k, const Fields& f, int n, hipStream_t s) {
+void launch_water_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
     if (n <= 0) return;
-    k_water_momentum<<<(n + kNB - 1) / kNB, kNB, 0, s>>>(k, f, n);
+    dispatch(f, c, [&](auto B, auto NB, auto CAP) {
+        k_water_momentum<decltype(B)::value, decltype(NB)::value, decltype(CAP)::value>
+            <<<(n + NB - 1) / NB, (int)NB, 0, s>>>(k, f, n);
+    });
 }
 
 // ---------------------------------------------------------------------------

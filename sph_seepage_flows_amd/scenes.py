@@ -178,4 +178,45 @@ def mixed(seed: int = 7, sf_type: int = 0, n_water: int = 1440, n_sand: int = 11
     return sc
 
 
+def _block(n, x0, r, max_rows=50, depth=44, cols=15):
+    """Lattice extents for n particles of the small-block layout: `cols` x
+    rows x `depth` at spacing 2r from (x0 + r, r, r); rows grow up to the box
+    height (`max_rows`), beyond that columns grow along x."""
+    rows = max(1, math.ceil(n / (cols * depth)))
+    if rows > max_rows:
+        rows = max_rows
+        cols = math.ceil(n / (rows * depth))
+    return (cols, rows, depth)
+
+
+def ensemble(seed: int = 3, n_scenes: int = 256, n_min: int = 5_000, n_max: int = 60_000) -> list[Scene]:
+    """Batched ensemble: `n_scenes` water-block + sand-pile scenes of the
+    small-block kind (1 m box, r = 0.01, h = 4r), each with its own total count
+    log-uniform in [n_min, n_max], water fraction U(0.3, 0.7), friction
+    U(0.3, 0.7), capillary coefficient sigma U(0, 0.05) and drag coefficient D
+    log-uniform in [1e3, 1e4], all from `seed`.  Capillary sigma enters the
+    Bezier controls as c0 = 0.7 sigma, cmc = sigma (cmc_p = 0.01, csat = 0, the
+    shape of the reference driver's curve, main.cpp:68-72); the drag enters as
+    cd = g rho0 / D (the reference's drag is g rho0 eps^2 / (cd eps^3 / (1 - eps)^2),
+    cuda_sph_sf_solver_gpu.cuh:858-862).  Water blocks start at x = 0, sand piles at
+    x = 0.5; sand gets U(-0.1r, 0.1r) jitter."""
+    u = uniform(seed, 0, n_scenes, 5).astype(np.float64)
+    out = []
+    for q in range(n_scenes):
+        n = int(round(math.exp(math.log(n_min) + u[q, 0] * (math.log(n_max) - math.log(n_min)))))
+        nw = int(round(n * (0.3 + 0.4 * u[q, 1])))
+        ns = n - nw
+        sigma = 0.05 * u[q, 3]
+        D = math.exp(math.log(1e3) + u[q, 4] * (math.log(1e4) - math.log(1e3)))
+        ph = P.Physics(friction=0.3 + 0.4 * u[q, 2], sf_c0=0.7 * sigma, sf_cmc=sigma, sf_cmc_p=0.01, sf_csat=0.0,
+                       sf_cd=9.81 * 1000.0 / D)
+        r, rs = ph.r_water, ph.r_sand
+        water = lattice((r, r, r), 2 * r, _block(nw, 0.0, r), nw)
+        sand = lattice((0.5 + rs, rs, rs), 2 * rs, _block(ns, 0.5, rs), ns, jitter=0.1 * rs, seed=seed,
+                       stream=100 + q)
+        out.append(_assemble(f"ensemble_{q}", ph, [(water, 0), (sand, 1)],
+                             meta={"seed": seed, "scene": q, "sigma": sigma, "drag_D": D}))
+    return out
+
+
 SCENES = {"small_block": small_block, "drop_in_1m": drop_in_1m, "mixed": mixed}

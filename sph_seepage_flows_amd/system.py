@@ -91,6 +91,32 @@ class SFSystem:
     def step(self, dt: float, n: int = 1) -> None:
         check(lib().sf_step_n(self._h, float(dt), int(n)))
 
+    def step_graph(self, dt: float, n: int) -> None:
+        """n fixed-dt steps replayed from a captured HIP graph (same results as step)."""
+        check(lib().sf_step_graph(self._h, float(dt), int(n)))
+
+    NEIGHBOUR_PASSES = ("density", "avg_flow", "voidage_adhesion", "sand_momentum", "water_momentum")
+
+    def autotune(self, reps: int = 10) -> dict:
+        """Times every launch shape of the neighbour passes on the current
+        particles and keeps the fastest. Returns {pass: (nb, cap, best_ms, default_ms)}."""
+        best = np.zeros(5, np.float32)
+        dflt = np.zeros(5, np.float32)
+        check(lib().sf_autotune(self._h, int(reps), ptr(best), ptr(dflt)))
+        out = {}
+        for q, name in enumerate(self.NEIGHBOUR_PASSES):
+            nb, cap = self.launch_config(name)
+            out[name] = (nb, cap, float(best[q]), float(dflt[q]))
+        return out
+
+    def launch_config(self, name: str) -> tuple[int, int]:
+        nb, cap = C.c_int(), C.c_int()
+        check(lib().sf_get_launch_config(self._h, _abi.PASSES.index(name), C.byref(nb), C.byref(cap)))
+        return nb.value, cap.value
+
+    def set_launch_config(self, name: str, nb: int, cap: int = 16) -> None:
+        check(lib().sf_set_launch_config(self._h, _abi.PASSES.index(name), int(nb), int(cap)))
+
     def step_profiled(self, dt: float, n: int, per_pass: bool = True):
         """n fixed-dt steps timed with HIP events on the handle's stream.
         Returns (total_ms, {pass: mean ms per step} or None)."""
@@ -128,6 +154,9 @@ class SFSystem:
         elif kind == "u32":
             out = np.zeros(cnt, np.uint32)
         elif kind == "u32c":
+            if name == "b_cell_start":  # the boundary grid is one scene's grid
+                g = self._b.grid_size
+                c = g.x * g.y * g.z
             out = np.zeros(c + 1, np.uint32)
         else:
             raise KeyError(name)
@@ -151,3 +180,64 @@ def write_bgeo(path: str, pos, col, radius, label) -> None:
     radius = np.ascontiguousarray(radius, np.float32)
     label = np.ascontiguousarray(label, np.uint64)
     check(lib().sf_write_bgeo(path.encode(), len(pos), ptr(pos), ptr(col), ptr(radius), ptr(label)))
+
+
+class BatchSystem(SFSystem):
+    """Many independent scenes in one handle (sf_batch_create): every pass is
+    one launch over all scenes.  Scenes share the box, grid, boundary and all
+    parameters except friction, the capillary curve and the drag coefficient
+    (sf_scene_params); scene s keeps the slot range offsets[s]:offsets[s+1]."""
+
+    def __init__(self, params, bparams, opts, counts, scene_params, device: int = -1):
+        self._p, self._b, self._o = params, bparams, opts
+        counts = np.ascontiguousarray(counts, np.uint64)
+        sp = (_abi.sf_scene_params * len(counts))(*scene_params)
+        h = C.c_void_p()
+        check(lib().sf_batch_create(C.byref(params), C.byref(bparams), C.byref(opts) if opts is not None else None,
+                                    len(counts), ptr(counts), C.addressof(sp), int(device), C.byref(h)))
+        self._h = h
+        self._sp = sp
+
+    @staticmethod
+    def scene_params_of(physics) -> "_abi.sf_scene_params":
+        p = _abi.sf_scene_params()
+        p.dem_tan_friction_angle = float(np.float32(physics.friction))
+        for n in ("sf_c0", "sf_cmc", "sf_cmc_p", "sf_csat", "sf_cd"):
+            setattr(p, n, float(np.float32(getattr(physics, n))))
+        return p
+
+    @classmethod
+    def from_scenes(cls, scenes, device: int = -1):
+        """scenes: Scene objects with one shared box / grid / radii / boundary;
+        each scene's own friction, capillary controls and drag go to its table row."""
+        p, b, o = scenes[0].params()
+        s = cls(p, b, o, [sc.n for sc in scenes], [cls.scene_params_of(sc.physics) for sc in scenes], device)
+        cat = lambda name, dt=np.float32: None if getattr(scenes[0], name) is None else np.ascontiguousarray(
+            np.concatenate([getattr(sc, name) for sc in scenes]), dt)
+        pos, vel, col = cat("pos"), cat("vel"), cat("col")
+        if vel is None and any(sc.vel is not None for sc in scenes):
+            vel = np.concatenate([sc.vel if sc.vel is not None else np.zeros((sc.n, 3), np.float32)
+                                  for sc in scenes]).astype(np.float32)
+        lab = cat("label", np.uint64)
+        mass, rad, cda, amc = cat("mass"), cat("radius"), cat("cda0asat"), cat("amcamcp")
+        check(lib().sf_upload_particles(s._h, len(pos), ptr(pos), ptr(vel), ptr(col), ptr(lab), ptr(mass),
+                                        ptr(rad), ptr(cda), ptr(amc)))
+        if scenes[0].bpos is not None:
+            s.upload_boundary(scenes[0].bpos, scenes[0].blabel)
+        return s
+
+    def set_scene_params(self, scene_params) -> None:
+        sp = (_abi.sf_scene_params * len(scene_params))(*scene_params)
+        check(lib().sf_batch_set_scene_params(self._h, C.addressof(sp)))
+        self._sp = sp
+
+    def offsets(self) -> np.ndarray:
+        ns = C.c_size_t()
+        check(lib().sf_batch_scene_offsets(self._h, C.byref(ns), None, 0))
+        off = np.zeros(ns.value + 1, np.uint64)
+        check(lib().sf_batch_scene_offsets(self._h, None, ptr(off), len(off)))
+        return off.astype(np.int64)
+
+    def get_scene(self, name: str, s: int) -> np.ndarray:
+        off = self.offsets()
+        return self.get(name)[off[s]:off[s + 1]]

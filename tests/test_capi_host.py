@@ -31,7 +31,7 @@ def test_library_exports_every_header_symbol():
 
 def test_abi_version_and_error_text():
     L = sf.lib()
-    assert L.sf_abi_version() == 1
+    assert L.sf_abi_version() == 2
     p, b, o = make_params(Physics())
     bad = _abi.sf_params.from_buffer_copy(p)
     bad.sph_density = -1.0
