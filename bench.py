@@ -72,6 +72,40 @@ def cpu_baseline(scene, steps: int, dt: float) -> dict:
                       f"warm-up step; {sec:.1f} s"}
 
 
+def ensemble_bench(dt: float, steps_batch: int, steps_plain: int) -> dict:
+    """Batched ensemble: 256 water-block + sand-pile scenes (5k-60k particles
+    each, per-scene friction / capillary / drag, seed 3) stepped by one
+    batched handle (HIP-graph replay, tuned launch shapes) versus the plain
+    loop of 256 single-scene handles, each stepped in turn."""
+    from sph_seepage_flows_amd import BatchSystem, SFSystem, scenes
+
+    E = scenes.ensemble()
+    n = sum(e.n for e in E)
+    b = BatchSystem.from_scenes(E, device=0)
+    b.step(dt, 2)
+    tuned = b.autotune(reps=5)
+    b.step_graph(dt, 2)  # capture outside the timed region
+    b.synchronize()
+    t = time.perf_counter()
+    b.step_graph(dt, steps_batch)
+    tb = (time.perf_counter() - t) / steps_batch
+    b.close()
+    hs = [SFSystem.from_scene(sc, device=0) for sc in E]
+    for h in hs:
+        h.step(dt, 1)
+    t = time.perf_counter()
+    for h in hs:
+        h.step(dt, steps_plain)
+    tp = (time.perf_counter() - t) / steps_plain
+    for h in hs:
+        h.close()
+    return {"workload": f"256 scenes, {n} particles (seed 3)", "ms_per_step": tb * 1e3,
+            "particle_steps_per_s": n / tb, "plain_loop_ms_per_step": tp * 1e3,
+            "plain_loop_particle_steps_per_s": n / tp, "speedup_vs_plain_loop": tp / tb,
+            "steps": [steps_batch, steps_plain],
+            "launch_shapes": {k: [v[0], v[1]] for k, v in tuned.items()}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,7 +113,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="drop_in_1m", choices=["drop_in_1m", "small_block"])
+    ap.add_argument("--workload", default="drop_in_1m", choices=["drop_in_1m", "small_block", "dam_4m"])
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -92,7 +127,7 @@ def main() -> int:
     import sph_seepage_flows_amd as sf
     from sph_seepage_flows_amd import scenes
 
-    scene = scenes.drop_in_1m() if args.workload == "drop_in_1m" else scenes.small_block()
+    scene = {"drop_in_1m": scenes.drop_in_1m, "small_block": scenes.small_block, "dam_4m": scenes.dam}[args.workload]()
     dt = 1e-4
     print(f"[bench] scene {scene.name}: n={scene.n} nb={scene.nb}", file=sys.stderr, flush=True)
     g = sf.SFSystem.from_scene(scene, device=0)
@@ -144,7 +179,13 @@ def main() -> int:
                           "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "pass_ms": pass_ms,
         "cpu_baseline": None,
+        "batched": None,
     }
+    g.close()
+    if not args.no_ensemble:
+        print("[bench] batched ensemble...", file=sys.stderr, flush=True)
+        out["batched"] = ensemble_bench(dt, 50, 5)
+        print(f"[bench] batched: {json.dumps(out['batched'])}", file=sys.stderr, flush=True)
     if rank == 0 and not args.no_cpu_baseline:
         print(f"[bench] cpu baseline ({args.cpu_steps} oracle steps)...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_steps, dt)

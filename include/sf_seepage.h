@@ -218,6 +218,12 @@ int sf_size(sf_handle h, size_t* n, size_t* nb, size_t* num_cells);
 
 /* ---- data access -------------------------------------------------------- */
 int sf_download(sf_handle h, int field, void* dst, size_t dst_bytes);
+/* Overwrites a persistent per-particle field in the current slot order:
+ * SF_FIELD_VEL / SF_FIELD_COL / SF_FIELD_CDA0ASAT (float3), SF_FIELD_MAX_SATURATION
+ * (float), SF_FIELD_AMCAMCP (float2).  No counterpart in the reference, whose
+ * particle constructor zeroes maxSaturation (cuda_sf_particles.cuh:201-235);
+ * it gives scenes an initial saturation history. */
+int sf_set_field(sf_handle h, int field, const void* src, size_t src_bytes);
 /* Raw device pointer and element stride in bytes of a field's storage
  * (float3 fields are stored as float4, stride 16). Valid until the next step. */
 int sf_device_buffer(sf_handle h, int field, void** dptr, size_t* stride_bytes);

@@ -68,6 +68,7 @@ def lib() -> C.CDLL:
             "orc_size": (SZ, [VP]),
             "orc_get": (C.c_int, [VP, C.c_int, VP]),
             "orc_set_state": (None, [VP, VP, VP]),
+            "orc_set_maxsat": (None, [VP, VP]),
         }
         for k, (r, a) in sig.items():
             fn = getattr(L, k)
@@ -100,6 +101,9 @@ class OracleSystem:
         self._arr = [f(scene.pos), f(scene.vel), f(scene.col), np.ascontiguousarray(scene.label, np.uint64),
                      f(scene.mass), f(scene.radius), f(scene.cda0asat), f(scene.amcamcp)]
         lib().orc_upload_particles(self._h, scene.n, *[_p(a) for a in self._arr])
+        if getattr(scene, "max_sat", None) is not None:
+            self._ms = np.ascontiguousarray(scene.max_sat, np.float32)
+            lib().orc_set_maxsat(self._h, _p(self._ms))
         self.nb = 0
         if scene.bpos is not None:
             bp = np.ascontiguousarray(scene.bpos, np.float32)

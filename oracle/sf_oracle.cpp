@@ -964,6 +964,12 @@ int orc_get(void* hv, int field, void* dst) {
     Handle* h = (Handle*)hv;
     return h->prec == 32 ? get_T(S<float>(h), field, dst) : get_T(S<double>(h), field, dst);
 }
+/* initial maxSaturation history (current order) */
+void orc_set_maxsat(void* hv, const float* ms) {
+    Handle* h = (Handle*)hv;
+    if (h->prec == 32) { auto* s = S<float>(h); for (size_t i = 0; i < s->n; ++i) s->maxsat[i] = ms[i]; }
+    else { auto* s = S<double>(h); for (size_t i = 0; i < s->n; ++i) s->maxsat[i] = ms[i]; }
+}
 /* overwrite positions / velocities (current order) from double arrays */
 void orc_set_state(void* hv, const double* pos3, const double* vel3) {
     Handle* h = (Handle*)hv;

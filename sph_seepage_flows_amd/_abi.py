@@ -127,6 +127,7 @@ _SIG = {
     "sf_size": (C.c_int, [_VP, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "sf_download": (C.c_int, [_VP, C.c_int, _VP, C.c_size_t]),
     "sf_device_buffer": (C.c_int, [_VP, C.c_int, C.POINTER(_VP), C.POINTER(C.c_size_t)]),
+    "sf_set_field": (C.c_int, [_VP, C.c_int, _VP, C.c_size_t]),
     "sf_export_bgeo": (C.c_int, [_VP, C.c_char_p, C.c_char_p]),
     "sf_batch_create": (C.c_int, [C.POINTER(sf_params), C.POINTER(sf_boundary_params), C.POINTER(sf_solver_options),
                                   C.c_size_t, _VP, _VP, C.c_int, C.POINTER(_VP)]),

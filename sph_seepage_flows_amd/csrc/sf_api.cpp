@@ -793,6 +793,45 @@ int sf_download(sf_handle h, int field, void* dst, size_t dst_bytes) {
     }
 }
 
+int sf_set_field(sf_handle h, int field, const void* src, size_t src_bytes) {
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!h->uploaded) return set_error(SF_E_STATE, "no particles uploaded");
+    const size_t n = h->n;
+    if (n == 0) return SF_OK;
+    if (!src) return set_error(SF_E_INVALID, "null source");
+    const float* s = (const float*)src;
+    auto patch4 = [&](float4* dev, int comps, int first) -> int {  // overwrite comps floats from component `first`
+        if (src_bytes < n * comps * sizeof(float)) return set_error(SF_E_INVALID, "source buffer too small");
+        std::vector<float4> v;
+        int r;
+        if ((r = fetch4(h, dev, n, v))) return r;
+        for (size_t i = 0; i < n; ++i) {
+            float* e = &v[i].x;
+            for (int c = 0; c < comps; ++c) {
+                if (!std::isfinite(s[comps * i + c])) return set_error(SF_E_INVALID, "non-finite value");
+                e[first + c] = s[comps * i + c];
+            }
+        }
+        SF_HIP(hipMemcpyAsync(dev, v.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+        SF_HIP(hipStreamSynchronize(h->stream));
+        return SF_OK;
+    };
+    switch (field) {
+    case SF_FIELD_VEL: return patch4(h->vr[h->cur].p, 3, 0);
+    case SF_FIELD_COL: return patch4(h->cs[h->cur_cs].p, 3, 0);
+    case SF_FIELD_MAX_SATURATION: return patch4(h->cs[h->cur_cs].p, 1, 3);
+    case SF_FIELD_CDA0ASAT: return patch4(h->cda[h->cur_aux].p, 3, 0);
+    case SF_FIELD_AMCAMCP: {
+        if (src_bytes < n * 8) return set_error(SF_E_INVALID, "source buffer too small");
+        SF_HIP(hipMemcpyAsync(h->amc[h->cur_aux].p, src, n * 8, hipMemcpyHostToDevice, h->stream));
+        SF_HIP(hipStreamSynchronize(h->stream));
+        return SF_OK;
+    }
+    default: return set_error(SF_E_INVALID, "field cannot be set (vel, col, max_saturation, cda0asat, amcamcp)");
+    }
+}
+
 int sf_device_buffer(sf_handle h, int field, void** dptr, size_t* stride) {
     int rc;
     if ((rc = check_handle(h))) return rc;

@@ -75,6 +75,7 @@ class Scene:
     bpos: np.ndarray | None = None
     blabel: np.ndarray | None = None
     meta: dict = field(default_factory=dict)
+    max_sat: np.ndarray | None = None  # initial maxSaturation history (None: zero)
 
     @property
     def n(self) -> int:
@@ -219,4 +220,68 @@ def ensemble(seed: int = 3, n_scenes: int = 256, n_min: int = 5_000, n_max: int 
     return out
 
 
-SCENES = {"small_block": small_block, "drop_in_1m": drop_in_1m, "mixed": mixed}
+def dam(seed: int = 2, n_sand: int = 2_500_000, n_water: int = 1_500_000, r: float = 0.005,
+        base: float = 2.0, crest: float = 0.5, height: float = 0.6, depth: float = 0.55) -> Scene:
+    """Seepage embankment: a trapezoidal sand dam (base 2 m, crest 0.5 m,
+    height 0.6 m) with the water reservoir upstream at 0.55 m depth, both
+    extruded along z far enough to hold the counts (2.5M sand + 1.5M water at
+    r = 0.005: cross-sections 0.75 m^2 and about 0.45 m^2 give about 3.34 m).
+    Lattices at spacing 2r are filled slab by slab along z (z outermost, so the
+    truncation to the exact count trims the last slab) and within a slab in
+    (x, y) order.  Sand: friction 0.6, capillary sigma 0.02 (c0 = 0.7 sigma,
+    cmc = sigma), drag D = 5e3 (cd = g rho0 / D), U(-0.1r, 0.1r) jitter and an
+    initial maxSaturation U(0, 0.1) per grain from `seed`.  The reservoir
+    keeps one lattice spacing of clearance from the upstream face.  Box: x from
+    0 to 0.3 m past the downstream toe, y to 1 m."""
+    sp = 2 * r
+    slope = (base - crest) / 2 / height  # horizontal run per unit height of each face
+
+    def sand_slab(x0):
+        xs = x0 + r + sp * np.arange(int((base) / sp) + 1)
+        ys = r + sp * np.arange(int(height / sp) + 1)
+        X, Y = np.meshgrid(xs, ys, indexing="ij")
+        keep = (Y <= height) & (X >= x0 + slope * Y) & (X <= x0 + base - slope * Y)
+        return np.stack([X[keep], Y[keep]], 1)
+
+    def water_slab(x0):
+        xs = r + sp * np.arange(int((x0 + slope * depth) / sp) + 1)
+        ys = r + sp * np.arange(int(depth / sp) + 1)
+        X, Y = np.meshgrid(xs, ys, indexing="ij")
+        keep = (Y <= depth) & (X <= x0 + slope * Y - sp)
+        return np.stack([X[keep], Y[keep]], 1)
+
+    slab_s = sand_slab(0.0)
+    n_slabs = math.ceil(n_sand / len(slab_s))
+    x0 = 0.0
+    while len(water_slab(x0)) * n_slabs < n_water:
+        x0 += sp
+    slab_s = sand_slab(x0)
+    n_slabs = math.ceil(n_sand / len(slab_s))
+    slab_w = water_slab(x0)
+    n_slabs = max(n_slabs, math.ceil(n_water / len(slab_w)))
+
+    def extrude(slab, n):
+        k = np.arange(math.ceil(n / len(slab)))
+        z = r + sp * k
+        P = np.empty((len(k), len(slab), 3), np.float64)
+        P[:, :, :2] = slab[None]
+        P[:, :, 2] = z[:, None]
+        return P.reshape(-1, 3)[:n].astype(np.float32)
+
+    hi_x = x0 + base + 0.3
+    h = 4 * r
+    hi_z = math.ceil((n_slabs * sp + sp) / h) * h
+    ph = P.Physics(hi=(float(np.float32(hi_x)), 1.0, float(np.float32(hi_z))), r_water=r, r_sand=r, friction=0.6,
+                   sf_c0=0.7 * 0.02, sf_cmc=0.02, sf_cmc_p=0.01, sf_csat=0.0, sf_cd=9.81 * 1000.0 / 5e3)
+    sand = extrude(slab_s, n_sand)
+    sand = sand + (uniform(seed, 1, n_sand, 3) * np.float32(2.0) - np.float32(1.0)) * np.float32(0.1 * r)
+    water = extrude(slab_w, n_water)
+    sc = _assemble("dam_4m", ph, [(water, 0), (sand.astype(np.float32), 1)],
+                   meta={"seed": seed, "dam_x0": x0, "slabs": n_slabs})
+    ms = np.zeros(sc.n, np.float32)
+    ms[n_water:] = uniform(seed, 3, n_sand, 1)[:, 0] * np.float32(0.1)
+    sc.max_sat = ms
+    return sc
+
+
+SCENES = {"small_block": small_block, "drop_in_1m": drop_in_1m, "mixed": mixed, "dam": dam}

@@ -34,6 +34,8 @@ class SFSystem:
         p, b, o = scene.params()
         s = cls(p, b, o, max_num=max(scene.n, 1) if max_num is None else max_num, device=device)
         s.upload_particles(scene)
+        if getattr(scene, "max_sat", None) is not None:
+            s.set_field("max_saturation", scene.max_sat)
         if scene.bpos is not None:
             s.upload_boundary(scene.bpos, scene.blabel)
         return s
@@ -162,6 +164,12 @@ class SFSystem:
             raise KeyError(name)
         check(lib().sf_download(self._h, fid, ptr(out) if out.size else None, out.nbytes))
         return out
+
+    def set_field(self, name: str, values) -> None:
+        """Overwrites vel / col / max_saturation / cda0asat / amcamcp in the current slot order."""
+        fid, _ = FIELDS[name]
+        a = np.ascontiguousarray(values, np.float32)
+        check(lib().sf_set_field(self._h, fid, ptr(a), a.nbytes))
 
     def device_buffer(self, name: str) -> tuple[int, int]:
         fid, _ = FIELDS[name]

@@ -18,5 +18,6 @@ def oracle_mod():
     import oracle
 
     oracle.build()
-    oracle.threads(int(os.environ.get("SF_ORACLE_THREADS", "8")))
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    oracle.threads(int(os.environ.get("SF_ORACLE_THREADS", str(min(16, cores)))))
     return oracle
