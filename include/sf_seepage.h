@@ -27,7 +27,8 @@
  *   sf_write_bgeo        <- Partio::writeBGEO (uncompressed V5)  extlib/include/partio/io/BGEO.cpp:311-476
  *   sf_build_world_boundary <- CudaBoundaryEmitter::BuildWorldBoundary  cuda_boundary_emitter.cpp:12-80
  *   sf_grid_size         <- make_int3((highest - lowest) / kernel_radius)  main.cpp:104, cuda_neighbor_searcher.cu:26
- *   sf_batch_*, sf_step_graph, sf_*_launch_config, sf_autotune: no counterpart (see below)
+ *   sf_batch_*, sf_step_graph, sf_*_launch_config, sf_autotune, sf_set_field, sf_diff_*:
+ *                           no counterpart (see their declarations)
  *
  * Threading: a handle is single-threaded; all its GPU work is ordered on the
  * handle's own HIP stream (sf_set_stream can replace it).
@@ -269,6 +270,30 @@ int sf_get_launch_config(sf_handle h, int pass, int* nb, int* cap);
  * best_ms / default_ms (5 floats each, may be NULL) receive per pass the
  * chosen and the default (256, 16) shape's mean time. */
 int sf_autotune(sf_handle h, int reps, float* best_ms, float* default_ms);
+
+/* ---- differentiable window ----------------------------------------------
+ * No counterpart in the reference.  Gradients of a loss on the state after
+ * n_steps fixed-dt steps with respect to the initial positions and velocities
+ * and to four parameters: sf_cd (drag / permeability), sf_c0 and sf_cmc
+ * (capillary curve), dem_tan_friction_angle.  Hand-written reverse passes of
+ * every force pass and the integrator replay each step's recorded sort,
+ * neighbour ranges and branch decisions; the forward keeps a state checkpoint
+ * every checkpoint_every steps and the backward recomputes one segment at a
+ * time.  Single-scene SF-mode handles without label-1 collider particles.
+ * Every per-particle array here is in id order: the slot order of the state
+ * the window started from.  on_device: pointers are device memory on the
+ * handle's GPU (else host memory). */
+int sf_diff_forward(sf_handle h, float dt, int n_steps, int checkpoint_every);
+/* Back to the window's initial state; vel3 (may be NULL) replaces its velocities. */
+int sf_diff_reset(sf_handle h, const float* vel3, int on_device);
+/* SF_FIELD_POS / SF_FIELD_VEL after the window, SF_FIELD_SATURATION of its last step. */
+int sf_diff_get(sf_handle h, int field, float* dst, int on_device);
+/* Seeds (any may be NULL = 0): d loss / d final position, velocity, last-step
+ * saturation.  Results (any may be NULL): param_bar4 (host) = d loss / d
+ * (sf_cd, sf_c0, sf_cmc, dem_tan_friction_angle); d loss / d initial position
+ * and velocity.  The handle is left at the end of the window. */
+int sf_diff_backward(sf_handle h, const float* pos_bar3, const float* vel_bar3, const float* sat_bar,
+                     float* param_bar4, float* pos0_bar3, float* vel0_bar3, int on_device);
 
 #ifdef __cplusplus
 }

@@ -69,6 +69,9 @@ def lib() -> C.CDLL:
             "orc_get": (C.c_int, [VP, C.c_int, VP]),
             "orc_set_state": (None, [VP, VP, VP]),
             "orc_set_maxsat": (None, [VP, VP]),
+            "orc_get_ids": (None, [VP, VP]),
+            "orc_tangent_window": (C.c_int, [VP, VP, VP, SZ, VP, VP, VP, VP, VP, VP, SZ, VP, VP, F, C.c_int,
+                                             C.c_int, VP, VP, VP, VP, VP, VP, VP]),
         }
         for k, (r, a) in sig.items():
             fn = getattr(L, k)
@@ -147,6 +150,19 @@ class OracleSystem:
         vel = np.ascontiguousarray(vel, np.float64)
         lib().orc_set_state(self._h, _p(pos), _p(vel))
 
+    def ids(self) -> np.ndarray:
+        """Upload slot of each particle, in the current (sorted) order."""
+        out = np.zeros(self.n, np.uint32)
+        lib().orc_get_ids(self._h, _p(out))
+        return out
+
+    def get_by_id(self, name: str) -> np.ndarray:
+        """A per-particle field in upload order."""
+        v = self.get(name)
+        out = np.empty_like(v)
+        out[self.ids()] = v
+        return out
+
     def get(self, name: str) -> np.ndarray:
         fid, kind = FIELD_IDS[name]
         n = self.n
@@ -168,6 +184,36 @@ class OracleSystem:
         if rc:
             raise RuntimeError(lib().orc_last_error().decode())
         return out
+
+
+def tangent_window(scene, dt: float, steps: int, vdot=None, pdot=None, params=None, xdot=None):
+    """FP64 forward-mode tangent of `steps` fixed-dt steps from the scene's
+    initial state.  vdot: (ndir, n, 3) initial-velocity directions; pdot:
+    (ndir, 4) directions in (sf_cd, sf_c0, sf_cmc, dem_tan_friction_angle);
+    xdot: (ndir, n, 3) initial-position directions;
+    ndir <= 4.  Returns (pos (n,3), sat (n), posdot (ndir,n,3), satdot (ndir,n))
+    in upload order; sat is the last step's saturation."""
+    p, b, o = params if params is not None else scene.params()
+    n = scene.n
+    given = [a for a in (vdot, pdot, xdot) if a is not None]
+    ndir = int(given[0].shape[0]) if given else 0
+    f = lambda a, dt_=np.float32: None if a is None else np.ascontiguousarray(a, dt_)
+    pos, vel, lab = f(scene.pos), f(scene.vel), np.ascontiguousarray(scene.label, np.uint64)
+    mass, rad, ms = f(scene.mass), f(scene.radius), f(getattr(scene, "max_sat", None))
+    bp = f(scene.bpos) if scene.bpos is not None else None
+    bl = np.ascontiguousarray(scene.blabel, np.uint64) if scene.blabel is not None else None
+    vd = None if vdot is None else np.ascontiguousarray(vdot, np.float64)
+    pd = None if pdot is None else np.ascontiguousarray(pdot, np.float64)
+    xd = None if xdot is None else np.ascontiguousarray(xdot, np.float64)
+    out_p = np.zeros((n, 3)); out_s = np.zeros(n)
+    pdt = np.zeros((max(ndir, 1), n, 3)); sdt = np.zeros((max(ndir, 1), n))
+    rc = lib().orc_tangent_window(C.byref(p), C.byref(b), C.byref(o), n, _p(pos), _p(vel), _p(lab), _p(mass),
+                                  _p(rad), _p(ms), 0 if bp is None else len(bp), _p(bp), _p(bl), float(dt),
+                                  int(steps), ndir, _p(vd), _p(xd), _p(pd), _p(out_p), _p(out_s), _p(pdt),
+                                  _p(sdt))
+    if rc:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return out_p, out_s, pdt[:ndir], sdt[:ndir]
 
 
 # ---- scalar restatements (double) for the pinning tests ----

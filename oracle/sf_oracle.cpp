@@ -39,12 +39,65 @@
 #include <limits>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
 
 namespace orc {
+
+/* unqualified math calls: std:: for float / double, the Dual overloads below otherwise */
+using std::cos;
+using std::fabs;
+using std::pow;
+using std::sin;
+using std::sqrt;
+
+/* Forward-mode dual number: value + N tangent components (FP64 host tangent). */
+template <int N> struct Dual {
+    double v;
+    double d[N];
+    Dual(double x = 0.0) : v(x) { for (int q = 0; q < N; ++q) d[q] = 0.0; }
+    template <class I, class = typename std::enable_if<std::is_integral<I>::value>::type>
+    Dual(I x) : Dual((double)x) {}
+    explicit operator float() const { return (float)v; }
+    explicit operator double() const { return v; }
+    friend Dual operator+(const Dual& a, const Dual& b) { Dual r(a.v + b.v); for (int q = 0; q < N; ++q) r.d[q] = a.d[q] + b.d[q]; return r; }
+    friend Dual operator-(const Dual& a, const Dual& b) { Dual r(a.v - b.v); for (int q = 0; q < N; ++q) r.d[q] = a.d[q] - b.d[q]; return r; }
+    friend Dual operator-(const Dual& a) { Dual r(-a.v); for (int q = 0; q < N; ++q) r.d[q] = -a.d[q]; return r; }
+    friend Dual operator*(const Dual& a, const Dual& b) { Dual r(a.v * b.v); for (int q = 0; q < N; ++q) r.d[q] = a.d[q] * b.v + a.v * b.d[q]; return r; }
+    friend Dual operator/(const Dual& a, const Dual& b) {
+        Dual r(a.v / b.v);
+        for (int q = 0; q < N; ++q) r.d[q] = (a.d[q] - r.v * b.d[q]) / b.v;
+        return r;
+    }
+    Dual& operator+=(const Dual& b) { return *this = *this + b; }
+    Dual& operator-=(const Dual& b) { return *this = *this - b; }
+    Dual& operator*=(const Dual& b) { return *this = *this * b; }
+    friend bool operator<(const Dual& a, const Dual& b) { return a.v < b.v; }
+    friend bool operator>(const Dual& a, const Dual& b) { return a.v > b.v; }
+    friend bool operator<=(const Dual& a, const Dual& b) { return a.v <= b.v; }
+    friend bool operator>=(const Dual& a, const Dual& b) { return a.v >= b.v; }
+    friend bool operator==(const Dual& a, const Dual& b) { return a.v == b.v; }
+    friend bool operator!=(const Dual& a, const Dual& b) { return a.v != b.v; }
+    friend Dual sqrt(const Dual& a) {
+        Dual r(std::sqrt(a.v));
+        const double g = r.v > 0.0 ? 0.5 / r.v : 0.0;
+        for (int q = 0; q < N; ++q) r.d[q] = a.d[q] * g;
+        return r;
+    }
+    friend Dual pow(const Dual& a, const Dual& b) { /* a^b, b constant or a > 0 */
+        Dual r(std::pow(a.v, b.v));
+        const double ga = a.v != 0.0 ? b.v * std::pow(a.v, b.v - 1.0) : (b.v == 1.0 ? 1.0 : 0.0);
+        const double gb = a.v > 0.0 ? r.v * std::log(a.v) : 0.0;
+        for (int q = 0; q < N; ++q) r.d[q] = ga * a.d[q] + gb * b.d[q];
+        return r;
+    }
+    friend Dual fabs(const Dual& a) { return a.v < 0.0 ? -a : a; }
+    friend Dual cos(const Dual& a) { Dual r(std::cos(a.v)); for (int q = 0; q < N; ++q) r.d[q] = -std::sin(a.v) * a.d[q]; return r; }
+    friend Dual sin(const Dual& a) { Dual r(std::sin(a.v)); for (int q = 0; q < N; ++q) r.d[q] = std::cos(a.v) * a.d[q]; return r; }
+};
 
 constexpr float kEps = 1e-6f;                               /* KIRI_EPSILON kiri_pbs_pch.cuh:52 */
 constexpr double kPiD = 3.141592653589793238462643383279502884197; /* KIRI_PI :54 */
@@ -63,7 +116,7 @@ template <class T> inline V3<T> operator/(V3<T> a, T s) { return {a.x / s, a.y /
 template <class T> inline V3<T>& operator+=(V3<T>& a, V3<T> b) { a.x += b.x; a.y += b.y; a.z += b.z; return a; }
 template <class T> inline V3<T>& operator*=(V3<T>& a, T s) { a.x *= s; a.y *= s; a.z *= s; return a; }
 template <class T> inline T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-template <class T> inline T length(V3<T> a) { return std::sqrt(dot(a, a)); }
+template <class T> inline T length(V3<T> a) { return sqrt(dot(a, a)); }
 template <class T> inline T lengthSquared(V3<T> a) { return dot(a, a); }     /* cuda_math_utils.cuh:74 */
 template <class T> inline T sgn(T v) { return T((T(0) < v) - (v < T(0))); } /* cuda_math_utils.cuh:76-82 */
 
@@ -120,9 +173,9 @@ template <class T> struct SpikyGrad { /* :100-118 */
 };
 template <class T> struct Adhesion { /* AdhesionAkinci13 :189-203 */
     T coef, h;
-    explicit Adhesion(T r) : h(r) { coef = T(0.007) / std::pow(h, T(3.25)); }
+    explicit Adhesion(T r) : h(r) { coef = T(0.007) / pow(h, T(3.25)); }
     T operator()(T r) const {
-        if (T(2) * r > h && r <= h) return coef * std::pow(T(-4) * r * r / h + T(6) * r - T(2) * h, T(0.25));
+        if (T(2) * r > h && r <= h) return coef * pow(T(-4) * r * r / h + T(6) * r - T(2) * h, T(0.25));
         return T(0);
     }
 };
@@ -135,7 +188,7 @@ template <class T> struct Bezier { /* QuadraticBezierCoeff cuda_sf_utils.cuh:36-
         d = T(-4) * (T(1) + b - px1);
         e = T(2) * (T(1) + b - px1);
     }
-    T rx2t(T sr) const { return (b + std::sqrt(c + d * (px1 - sr))) / e; }
+    T rx2t(T sr) const { return (b + sqrt(c + d * (px1 - sr))) / e; }
     T operator()(T sr) const {
         if (sr < T(0)) return py0;
         if (sr >= T(1)) return py2;
@@ -160,9 +213,9 @@ V3<T> dem_force(V3<T> dij, V3<T> vij, T rij, T kn, T ks, T tanphi) { /* :20-51 *
         V3<T> vt = vij - de * n;
         V3<T> fn = (kn * pen) * n;
         V3<T> fs = (-ks) * vt;
-        T maxfs = lengthSquared(fn) * std::pow(tanphi, T(2));
+        T maxfs = lengthSquared(fn) * pow(tanphi, T(2));
         if (lengthSquared(fs) > maxfs) {
-            T ratio = std::sqrt(maxfs) / length(fs);
+            T ratio = sqrt(maxfs) / length(fs);
             fs *= ratio;
         }
         f = -fn - fs;
@@ -173,8 +226,8 @@ template <class T> struct CapConst { /* rupture / bridge constants, :63-65 */
     T theta, vb, srup;
     explicit CapConst(T R) {
         theta = T(30) / T(180) * T(kPiD);
-        vb = T(4) / T(3) * T(kPiD) * std::pow(R, T(3)) * T(0.01) * T(0.01);
-        srup = (T(1) + T(0.5) * theta) * (std::pow(vb, T(1) / T(3)) + T(0.1) * std::pow(vb, T(2) / T(3)));
+        vb = T(4) / T(3) * T(kPiD) * pow(R, T(3)) * T(0.01) * T(0.01);
+        srup = (T(1) + T(0.5) * theta) * (pow(vb, T(1) / T(3)) + T(0.1) * pow(vb, T(2) / T(3)));
     }
 };
 template <class T>
@@ -185,10 +238,10 @@ V3<T> capillary_force(V3<T> dij, T R, T sr, const Bezier<T>& G, const CapConst<T
     if (H < K.srup && H > T(0)) {
         V3<T> N = dij / dist;
         T cc = G(sr);
-        T d = -H + std::sqrt(H * H + K.vb / (T(kPiD) * R));
-        T phi = std::sqrt(T(2) * H / R * (T(-1) + std::sqrt(T(1) + K.vb / (T(kPiD) * R * H * H))));
-        T neck = T(-2) * T(kPiD) * cc * R * std::cos(K.theta) / (T(1) + H / (T(2) * d));
-        T surf = T(-2) * T(kPiD) * cc * R * phi * std::sin(K.theta);
+        T d = -H + sqrt(H * H + K.vb / (T(kPiD) * R));
+        T phi = sqrt(T(2) * H / R * (T(-1) + sqrt(T(1) + K.vb / (T(kPiD) * R * H * H))));
+        T neck = T(-2) * T(kPiD) * cc * R * cos(K.theta) / (T(1) + H / (T(2) * d));
+        T surf = T(-2) * T(kPiD) * cc * R * phi * sin(K.theta);
         f = -N * (neck + surf);
     }
     return f;
@@ -198,13 +251,13 @@ void world_boundary_force(V3<T>& f, V3<T> p, V3<T> v, T R, T Rb, T young, T pois
                           V3<T> lo, V3<T> hi) { /* :132-199 */
     T rij = Rb + R, kn = young * R, ks = kn * poisson;
     V3<T> mv = -v;
-    if (p.x > hi.x - rij) f += dem_force(mk<T>(1, 0, 0) * std::fabs(p.x - hi.x), mv, rij, kn, ks, tanphi);
-    if (p.x < lo.x + rij) f += dem_force(mk<T>(-1, 0, 0) * std::fabs(p.x - lo.x), mv, rij, kn, ks, tanphi);
-    if (p.y > hi.y - rij) f += dem_force(mk<T>(0, 1, 0) * std::fabs(p.y - hi.y), mv, rij, kn, ks, tanphi);
-    if (p.y < lo.y + rij) f += dem_force(mk<T>(0, -1, 0) * std::fabs(p.y - lo.y), mv, rij, kn, ks, tanphi);
-    if (p.z > hi.z - rij) f += dem_force(mk<T>(0, 0, 1) * std::fabs(p.z - hi.z), mv, rij, kn, ks, tanphi);
+    if (p.x > hi.x - rij) f += dem_force(mk<T>(1, 0, 0) * fabs(p.x - hi.x), mv, rij, kn, ks, tanphi);
+    if (p.x < lo.x + rij) f += dem_force(mk<T>(-1, 0, 0) * fabs(p.x - lo.x), mv, rij, kn, ks, tanphi);
+    if (p.y > hi.y - rij) f += dem_force(mk<T>(0, 1, 0) * fabs(p.y - hi.y), mv, rij, kn, ks, tanphi);
+    if (p.y < lo.y + rij) f += dem_force(mk<T>(0, -1, 0) * fabs(p.y - lo.y), mv, rij, kn, ks, tanphi);
+    if (p.z > hi.z - rij) f += dem_force(mk<T>(0, 0, 1) * fabs(p.z - hi.z), mv, rij, kn, ks, tanphi);
     if (p.z < lo.z + T(2) * rij) /* 2*rij trigger as written at :191 */
-        f += dem_force(mk<T>(0, 0, -1) * std::fabs(p.z - lo.z), mv, rij, kn, ks, tanphi);
+        f += dem_force(mk<T>(0, 0, -1) * fabs(p.z - lo.z), mv, rij, kn, ks, tanphi);
 }
 
 /* ---- the oracle system ----------------------------------------------------- */
@@ -225,6 +278,15 @@ template <class T> struct Sys : Base {
     std::vector<T> mass, radius, density, pressure, voidage, saturation, maxsat, amc0, amc1;
     std::vector<uint64_t> label;
     std::vector<uint32_t> keys, perm, sorted_keys, cellStart;
+    std::vector<uint32_t> ids; /* upload slot of each particle, follows every sort */
+    /* differentiated parameters (sf_cd, sf_c0, sf_cmc, dem_tan_friction_angle):
+     * read from P unless a tangent run seeded them */
+    bool th_set = false;
+    T th[4];
+    T p_cd() const { return th_set ? th[0] : T(P.sf_cd); }
+    T p_c0() const { return th_set ? th[1] : T(P.sf_c0); }
+    T p_cmc() const { return th_set ? th[2] : T(P.sf_cmc); }
+    T p_tan() const { return th_set ? th[3] : T(P.dem_tan_friction_angle); }
     std::vector<V3<T>> bpos;
     std::vector<uint64_t> blabel;
     std::vector<T> bvol;
@@ -245,6 +307,8 @@ template <class T> struct Sys : Base {
         for (auto* a : {&pos, &vel, &acc, &col, &avgflow, &avgdrag, &adh, &avgadh, &cda}) a->assign(k, mk<T>(0, 0, 0));
         for (auto* a : {&mass, &radius, &density, &pressure, &voidage, &saturation, &maxsat, &amc0, &amc1}) a->assign(k, T(0));
         label.assign(k, 0);
+        ids.resize(k);
+        std::iota(ids.begin(), ids.end(), 0u);
         keys.assign(k, 0);
         perm.assign(k, 0);
         sorted_keys.assign(k, 0);
@@ -283,6 +347,7 @@ template <class T> struct Sys : Base {
         perm = stable_order(keys);
         for (size_t i = 0; i < n; ++i) sorted_keys[i] = keys[perm[i]];
         permute(label, perm);
+        permute(ids, perm);
         permute(pos, perm);
         permute(vel, perm);
         permute(col, perm);
@@ -367,7 +432,7 @@ template <class T> struct Sys : Base {
         T neg = tait ? T(O.negative_scale) : T(1); /* ComputeSFPressure_CUDA passes 1 (cuda_sph_sf_solver.cu:63) */
         for (size_t i = 0; i < n; ++i) {
             if (label[i] == 0) {
-                T p = tait ? B_ * (std::pow(density[i] / rho0, T(7)) - T(1)) /* wcsph_gpu.cuh:34 */
+                T p = tait ? B_ * (pow(density[i] / rho0, T(7)) - T(1)) /* wcsph_gpu.cuh:34 */
                            : B_ * (density[i] - rho0);                       /* sph_sf_gpu.cuh:497 */
                 if (p < T(0)) p *= neg;
                 pressure[i] = p;
@@ -475,11 +540,11 @@ template <class T> struct Sys : Base {
         Poly6<T> W((T)B.kernel_radius);
         SpikyGrad<T> dW((T)B.kernel_radius);
         T R = P.dem_particle_radius, Rw = P.sph_particle_radius;
-        T young = P.dem_young, poisson = P.dem_poisson, tanphi = P.dem_tan_friction_angle;
+        T young = P.dem_young, poisson = P.dem_poisson, tanphi = p_tan();
         T kn = young * R, ks = kn * poisson, rij = T(2) * R;
-        Bezier<T> G(P.sf_c0, P.sf_cmc, P.sf_cmc_p, P.sf_csat); /* cuda_sph_sf_solver.cu:249,308 */
+        Bezier<T> G(p_c0(), p_cmc(), T(P.sf_cmc_p), T(P.sf_csat)); /* cuda_sph_sf_solver.cu:249,308 */
         CapConst<T> K(R);
-        T grav = std::fabs((T)P.gravity.y), rho0 = P.sph_density;
+        T grav = fabs((T)P.gravity.y), rho0 = P.sph_density;
         V3<T> lo = mk<T>(B.lowest_point.x, B.lowest_point.y, B.lowest_point.z);
         V3<T> hi = mk<T>(B.highest_point.x, B.highest_point.y, B.highest_point.z);
 #pragma omp parallel for schedule(static)
@@ -523,10 +588,10 @@ template <class T> struct Sys : Base {
             world_boundary_force(f, pos[i], vel[i], R, Rw, young, poisson, tanphi, lo, hi);
             T eps = voidage[i];
             if (eps < T(1) && eps > T(0)) {
-                T cd = multi ? cda[i].x : T(P.sf_cd);
-                T dc = cd * std::pow(eps, T(3)) / std::pow(T(1) - eps, T(2));
+                T cd = multi ? cda[i].x : p_cd();
+                T dc = cd * pow(eps, T(3)) / pow(T(1) - eps, T(2));
                 if (dc != T(0))
-                    df = (grav * rho0 * std::pow(eps, T(2)) / dc) * (avgflow[i] - vel[i]) * mass[i] / density[i];
+                    df = (grav * rho0 * pow(eps, T(2)) / dc) * (avgflow[i] - vel[i]) * mass[i] / density[i];
             }
             avgdrag[i] = vw != T(0) ? df / vw : mk<T>(0, 0, 0);
             acc[i] += T(2) * (f + df) / mass[i];
@@ -608,14 +673,14 @@ template <class T> struct Sys : Base {
     size_t cfl_sub_steps(float render_interval) {
         float amax = 0.f;
         for (size_t i = 0; i < n; ++i) {
-            float l = std::sqrt((float)acc[i].x * (float)acc[i].x + (float)acc[i].y * (float)acc[i].y +
+            float l = sqrt((float)acc[i].x * (float)acc[i].x + (float)acc[i].y * (float)acc[i].y +
                                 (float)acc[i].z * (float)acc[i].z);
             amax = std::max(amax, l);
         }
         float h = B.kernel_radius, m = P.sph_mass;
         float fmax_ = amax * m;
         float dts = 0.4f * h / O.speed_of_sound;
-        float dtf = 0.25f * std::sqrt(h * m / fmax_);
+        float dtf = 0.25f * sqrt(h * m / fmax_);
         float dt = std::min(O.time_step_limit_scale * std::min(dts, dtf), P.dt);
         size_t k = (size_t)(render_interval / dt);
         return k < 1 ? 1 : k; /* the reference divides by 0 here; the restatement keeps >= 1 */
@@ -964,6 +1029,12 @@ int orc_get(void* hv, int field, void* dst) {
     Handle* h = (Handle*)hv;
     return h->prec == 32 ? get_T(S<float>(h), field, dst) : get_T(S<double>(h), field, dst);
 }
+/* upload slot of each particle in the current order */
+void orc_get_ids(void* hv, uint32_t* out) {
+    Handle* h = (Handle*)hv;
+    const auto& v = h->prec == 32 ? S<float>(h)->ids : S<double>(h)->ids;
+    std::memcpy(out, v.data(), v.size() * 4);
+}
 /* initial maxSaturation history (current order) */
 void orc_set_maxsat(void* hv, const float* ms) {
     Handle* h = (Handle*)hv;
@@ -975,6 +1046,73 @@ void orc_set_state(void* hv, const double* pos3, const double* vel3) {
     Handle* h = (Handle*)hv;
     if (h->prec == 32) upload_state_T(S<float>(h), S<float>(h)->n, pos3, vel3);
     else upload_state_T(S<double>(h), S<double>(h)->n, pos3, vel3);
+}
+
+/* FP64 forward-mode tangent of a fixed-dt window (up to 4 directions at once).
+ * Direction q seeds d(initial velocity) = vdot3[q], d(initial position) =
+ * xdot3[q] (n x 3, upload order; either may be NULL) and d(sf_cd, sf_c0,
+ * sf_cmc, dem_tan_friction_angle) = pdot4[q].
+ * Outputs in upload order: final positions, the last step's saturation and
+ * their tangents (posdot3 / satdot: ndir blocks). */
+int orc_tangent_window(const sf_params* p, const sf_boundary_params* b, const sf_solver_options* o, size_t n,
+                       const float* pos3, const float* vel3, const uint64_t* label, const float* mass,
+                       const float* radius, const float* maxsat, size_t nb, const float* bpos3, const uint64_t* blabel,
+                       float dt, int steps, int ndir, const double* vdot3, const double* xdot3, const double* pdot4,
+                       double* pos_out3,
+                       double* sat_out, double* posdot3, double* satdot) {
+    constexpr int N = 4;
+    using D = Dual<N>;
+    if (ndir < 0 || ndir > N) return fail(-1, "ndir must be in [0, 4]");
+    if (p->sf_type == SF_TYPE_MULTI_SF) return fail(-1, "tangent runs need SF mode");
+    Sys<D> s;
+    s.P = *p;
+    s.B = *b;
+    s.O = o ? *o : sf_solver_options{0.f, 3.f, 100.f};
+    s.G = Grid{b->lowest_point.x, b->lowest_point.y, b->lowest_point.z, b->kernel_radius, b->grid_size.x,
+               b->grid_size.y, b->grid_size.z};
+    s.C = (size_t)b->grid_size.x * b->grid_size.y * b->grid_size.z;
+    s.multi = false;
+    s.cell_start_from_keys({}, s.C, s.cellStart);
+    s.cell_start_from_keys({}, s.C, s.bcellStart);
+    upload_T(&s, n, pos3, vel3, nullptr, label, mass, radius, nullptr, nullptr);
+    if (maxsat)
+        for (size_t i = 0; i < n; ++i) s.maxsat[i] = maxsat[i];
+    for (int q = 0; q < ndir; ++q)
+        for (size_t i = 0; i < n; ++i) {
+            if (vdot3) {
+                s.vel[i].x.d[q] = vdot3[(q * n + i) * 3];
+                s.vel[i].y.d[q] = vdot3[(q * n + i) * 3 + 1];
+                s.vel[i].z.d[q] = vdot3[(q * n + i) * 3 + 2];
+            }
+            if (xdot3) { /* the grid hash reads the value only */
+                s.pos[i].x.d[q] = xdot3[(q * n + i) * 3];
+                s.pos[i].y.d[q] = xdot3[(q * n + i) * 3 + 1];
+                s.pos[i].z.d[q] = xdot3[(q * n + i) * 3 + 2];
+            }
+        }
+    const float pv[4] = {p->sf_cd, p->sf_c0, p->sf_cmc, p->dem_tan_friction_angle};
+    s.th_set = true;
+    for (int c = 0; c < 4; ++c) {
+        s.th[c] = D(pv[c]);
+        for (int q = 0; q < ndir; ++q) s.th[c].d[q] = pdot4 ? pdot4[4 * q + c] : 0.0;
+    }
+    if (nb) s.upload_boundary(bpos3, blabel, nb);
+    for (int k = 0; k < steps; ++k) s.step(D((double)dt));
+    for (size_t k = 0; k < n; ++k) {
+        const size_t id = s.ids[k];
+        const V3<D>& x = s.pos[k];
+        pos_out3[3 * id] = x.x.v;
+        pos_out3[3 * id + 1] = x.y.v;
+        pos_out3[3 * id + 2] = x.z.v;
+        sat_out[id] = s.saturation[k].v;
+        for (int q = 0; q < ndir; ++q) {
+            posdot3[(q * n + id) * 3] = x.x.d[q];
+            posdot3[(q * n + id) * 3 + 1] = x.y.d[q];
+            posdot3[(q * n + id) * 3 + 2] = x.z.d[q];
+            satdot[q * n + id] = s.saturation[k].d[q];
+        }
+    }
+    return 0;
 }
 
 } /* extern "C" */

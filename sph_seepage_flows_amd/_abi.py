@@ -137,6 +137,10 @@ _SIG = {
     "sf_set_launch_config": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int]),
     "sf_get_launch_config": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "sf_autotune": (C.c_int, [_VP, C.c_int, _VP, _VP]),
+    "sf_diff_forward": (C.c_int, [_VP, C.c_float, C.c_int, C.c_int]),
+    "sf_diff_reset": (C.c_int, [_VP, _VP, C.c_int]),
+    "sf_diff_get": (C.c_int, [_VP, C.c_int, _VP, C.c_int]),
+    "sf_diff_backward": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, C.c_int]),
 }
 
 

@@ -28,6 +28,20 @@ struct DevBuf {
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
     ~DevBuf() { release(); }
     void release() {
         if (p) (void)hipFree(p);
@@ -111,6 +125,10 @@ struct Fields {
     float4* adh;
     float4* xf;
     float4* acc;
+    // differentiable window only (null otherwise): per-particle sums the
+    // reverse passes need: (P2 weight sum, P3 sand-voidage / water vs
+    // denominator, P4 vw, unused)
+    float4* den4;
     // batched handles only (null otherwise): per-slot scene id, per-scene constants
     const uint16_t* scene;
     const SceneConst* sct;

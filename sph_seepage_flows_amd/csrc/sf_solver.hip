@@ -198,6 +198,7 @@ __global__ __launch_bounds__(NB) void k_avg_flow(SFConst k, Fields f, int n) {
     });
     const v3 fl = vsum != 0.f ? av * (1.f / vsum) : mk3(0.f, 0.f, 0.f);
     f.flow[i] = f4(fl, 0.f);
+    if (f.den4) reinterpret_cast<float*>(f.den4)[4 * i] = vsum;
     const float eps_i = 1.f - sandv;
     f.es[i] = make_float2(eps_i, sand_i ? vsum : 0.f);
     if (!sand_i) {
@@ -240,6 +241,7 @@ __global__ __launch_bounds__(NB) void k_voidage_adhesion(SFConst k, Fields f, in
         });
         f.es[i].x = den != 0.f ? num / den : 0.f;
         f.adh[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (f.den4) reinterpret_cast<float*>(f.den4)[4 * i + 1] = den;
         return;
     }
     const float mi = pi.w;
@@ -265,6 +267,7 @@ __global__ __launch_bounds__(NB) void k_voidage_adhesion(SFConst k, Fields f, in
     });
     f.adh[i] = f4(F, 0.f);
     f.xf[i] = vs != 0.f ? f4(F * (1.f / vs), 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (f.den4) reinterpret_cast<float*>(f.den4)[4 * i + 1] = vs;
 }
 void launch_voidage_adhesion(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
     if (n <= 0) return;
@@ -398,6 +401,7 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
         }
     }
     f.xf[i] = vw != 0.f ? f4(df * (1.f / vw), 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (f.den4) reinterpret_cast<float*>(f.den4)[4 * i + 2] = vw;
     const v3 a = mk3(k.gvx, k.gvy, k.gvz) + (fsum + df) * (2.f / mi);
     f.acc[i] = f4(a, 0.f);
     // ComputeSFWetSandColor_CUDA (:952-977)

@@ -1,0 +1,115 @@
+// sf_system.h -- the handle behind sf_handle and the step drivers shared by
+// the C-ABI translation units (sf_api.cpp, sf_diff.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "sf_internal.h"
+
+namespace sf {
+struct DiffState;  // differentiable-window tape (sf_diff.cpp)
+void diff_release(DiffState* d);
+}  // namespace sf
+
+using namespace sf;
+
+struct sf_system {
+    sf_params P{};
+    sf_boundary_params B{};
+    sf_solver_options O{};
+    SFConst K{};
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    size_t cap = 0, n = 0, nb = 0;
+    uint32_t C = 0;
+    RadixPlan plan;
+    int blabel1 = 0;
+    // persistent state, ping-pong
+    DevBuf<float4> pm[2], vr[2], cs[2], cda[2];
+    DevBuf<float2> amc[2];
+    int cur = 0;      // pm / vr
+    int cur_cs = 0;   // cs
+    int cur_aux = 0;  // cda / amc
+    // searcher
+    DevBuf<uint32_t> keys, kA, kB, vA, vB, hist, cell, bcell, wpre, wpart;
+    DevBuf<uint2> ct;
+    size_t sort_cap = 0;
+    const uint32_t* sorted_keys = nullptr;
+    const uint32_t* perm = nullptr;
+    // transient fields
+    DevBuf<float2> rp, es, wt;
+    DevBuf<float4> flow, adh, xf, acc;
+    // boundary
+    DevBuf<float4> bpv;
+    // CFL
+    DevBuf<float> cfl_part;
+    DevBuf<CflOut> cfl_out;
+    size_t nsub = 1;
+    // batched scenes (nscenes == 0: one scene)
+    size_t nscenes = 0;
+    std::vector<size_t> soff;
+    std::vector<sf_scene_params> sp;
+    DevBuf<uint16_t> scene;
+    DevBuf<SceneConst> sct;
+    // launch shapes of the neighbour passes (SF_PASS_DENSITY ..)
+    PassCfg cfg[5];
+    // graph of two fixed-dt steps
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    float graph_dt = 0.f;
+    int graph_parity = -1;
+    bool graph_dirty = true;
+    // differentiable window (sf_diff_*), null until first used
+    DiffState* diff = nullptr;
+    // diagnostics
+    DevBuf<uint32_t> nbr_cnt, nbr_cand;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> ev_pool;  // sf_step_profiled
+    bool uploaded = false;
+    bool built = false;
+
+    Fields fields() {
+        Fields f;
+        f.pm = pm[cur].p;
+        f.vr = vr[cur].p;
+        f.cs = cs[cur_cs].p;
+        f.cda = cda[cur_aux].p;
+        f.ct = ct.p;
+        f.bpv = bpv.p;
+        f.bcell = bcell.p;
+        f.rp = rp.p;
+        f.es = es.p;
+        f.wt = wt.p;
+        f.flow = flow.p;
+        f.adh = adh.p;
+        f.xf = xf.p;
+        f.acc = acc.p;
+        f.den4 = nullptr;
+        f.scene = nscenes ? scene.p : nullptr;
+        f.sct = nscenes ? sct.p : nullptr;
+        return f;
+    }
+    uint32_t local_cells() const { return K.C; }
+    int parity() const { return cur | (cur_cs << 1) | (cur_aux << 2); }
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+        graph_dirty = true;
+    }
+};
+
+namespace sf {
+int ensure_sort_capacity(sf_system* h, size_t m);
+int build_neighbors(sf_system* h, hipEvent_t* ev = nullptr);
+int solve(sf_system* h, hipEvent_t* ev = nullptr);
+int solve_with(sf_system* h, const Fields& f, hipEvent_t* ev = nullptr);
+int advect(sf_system* h, float dt, const CflOut* dt_dev, float render_interval, hipEvent_t* ev = nullptr);
+int one_step(sf_system* h, float dt, hipEvent_t* ev = nullptr);
+int check_handle(sf_system* h);
+int set_device(sf_system* h);
+int fetch4(sf_system* h, const float4* src, size_t n, std::vector<float4>& out);
+}  // namespace sf
