@@ -106,6 +106,81 @@ def ensemble_bench(dt: float, steps_batch: int, steps_plain: int) -> dict:
             "launch_shapes": {k: [v[0], v[1]] for k, v in tuned.items()}}
 
 
+def diff_bench(dt: float, check: bool) -> dict:
+    """Differentiable seepage column: 48k sand + 16k water poured from above,
+    200-step window, checkpoints every 20 steps.  Loss = mean squared deviation
+    of the final sand positions and saturations from a target produced with
+    hidden parameters (drag D 4e3, capillary sigma 0.03, friction 0.55);
+    gradient at (5e3, 0.02, 0.5) and w.r.t. the initial velocities.  Times the
+    window's forward, its backward (segment recompute + reverse passes) and a
+    plain forward of the same steps.  With `check`, the gradient of a 2-step
+    window of the same scene is compared with the oracle's FP64 tangent (a
+    dot-product identity per direction)."""
+    import numpy as np
+
+    from sph_seepage_flows_amd import SFSystem, scenes
+    from sph_seepage_flows_amd.diff import DiffWindow, msd_loss, physical_grad, theta_of
+
+    sc = scenes.column()
+    T, K = 200, 20
+    sand = sc.label == 1
+    th_hidden, th0 = theta_of(4e3, 0.03, 0.55), theta_of(5e3, 0.02, 0.5)
+    win = DiffWindow(sc, T, dt, K, device=0)
+    tp, ts = win.forward(th_hidden)
+    pos, sat = win.forward(th0)
+    loss, pb, sb = msd_loss(pos, sat, tp, ts, sand)
+    win.backward(pb, sb)  # warm-up (tape allocation)
+    reps = 3
+    t = time.perf_counter()
+    for _ in range(reps):
+        win.forward(th0)
+    tf = (time.perf_counter() - t) / reps
+    t = time.perf_counter()
+    for _ in range(reps):
+        th, gx, gv = win.backward(pb, sb)
+    tb = (time.perf_counter() - t) / reps
+    g = SFSystem.from_scene(sc, device=0)
+    g.set_params(win.sys._p)
+    g.step(dt, 2)
+    g.synchronize()
+    t = time.perf_counter()
+    g.step(dt, T)
+    tplain = time.perf_counter() - t
+    out = {"workload": f"column: {int(sand.sum())} sand + {int((~sand).sum())} water, {T} steps, dt={dt}, "
+                       f"checkpoint every {K}", "forward_ms": tf * 1e3, "adjoint_ms": tb * 1e3,
+           "adjoint_over_forward": tb / tf, "plain_forward_ms": tplain * 1e3, "adjoint_over_plain_forward": tb / tplain,
+           "loss": loss, "grad": physical_grad(th, 5e3), "grad_vel0_norm": float(np.linalg.norm(gv)),
+           "grad_finite": bool(np.isfinite(th).all() and np.isfinite(gv).all())}
+    if check:
+        import oracle
+
+        oracle.build()
+        oracle.threads(min(16, len(os.sched_getaffinity(0))))
+        steps = 2
+        w2 = DiffWindow(sc, steps, dt, 1, device=0)
+        h2 = DiffWindow(sc, steps, dt, 1, device=0)
+        tp2, ts2 = h2.forward(th_hidden)
+        p2, s2 = w2.forward(th0)
+        _, pb2, sb2 = msd_loss(p2, s2, tp2, ts2, sand)
+        th2, _, gv2 = w2.backward(pb2, sb2)
+        p, b, o = sc.params()
+        for name, v in zip(("sf_cd", "sf_c0", "sf_cmc", "dem_tan_friction_angle"), th0):
+            setattr(p, name, float(v))
+        rng = np.random.default_rng(0)
+        u = rng.standard_normal((sc.n, 3))
+        vdot = np.zeros((4, sc.n, 3)); vdot[0] = u
+        pdot = np.zeros((4, 4)); pdot[1, 0] = 1.0; pdot[2, 1] = 1.0; pdot[3, 3] = 1.0
+        _, _, pd, sd = oracle.tangent_window(sc, dt, steps, vdot=vdot, pdot=pdot, params=(p, b, o))
+        _, _, pd2, sd2 = oracle.tangent_window(sc, dt, steps, pdot=np.array([[0, 0, 1.0, 0]]), params=(p, b, o))
+        J = lambda a, c: float((pb2 * a).sum() + (sb2 * c).sum())
+        pairs = {"vel0_direction": (float((gv2 * u).sum()), J(pd[0], sd[0])), "sf_cd": (float(th2[0]), J(pd[1], sd[1])),
+                 "sf_c0": (float(th2[1]), J(pd[2], sd[2])), "sf_cmc": (float(th2[2]), J(pd2[0], sd2[0])),
+                 "tan_friction": (float(th2[3]), J(pd[3], sd[3]))}
+        out["check_2_steps_vs_fp64_tangent"] = {
+            k: {"adjoint": a, "tangent": r, "rel": abs(a - r) / max(abs(r), 1e-30)} for k, (a, r) in pairs.items()}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +190,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="drop_in_1m", choices=["drop_in_1m", "small_block", "dam_4m"])
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-window measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -180,12 +256,17 @@ def main() -> int:
         "pass_ms": pass_ms,
         "cpu_baseline": None,
         "batched": None,
+        "differentiation": None,
     }
     g.close()
     if not args.no_ensemble:
         print("[bench] batched ensemble...", file=sys.stderr, flush=True)
         out["batched"] = ensemble_bench(dt, 50, 5)
         print(f"[bench] batched: {json.dumps(out['batched'])}", file=sys.stderr, flush=True)
+    if not args.no_diff:
+        print("[bench] differentiable window...", file=sys.stderr, flush=True)
+        out["differentiation"] = diff_bench(dt, check=not args.no_cpu_baseline)
+        print(f"[bench] differentiation: {json.dumps(out['differentiation'])}", file=sys.stderr, flush=True)
     if rank == 0 and not args.no_cpu_baseline:
         print(f"[bench] cpu baseline ({args.cpu_steps} oracle steps)...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_steps, dt)

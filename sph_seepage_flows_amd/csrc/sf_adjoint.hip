@@ -24,6 +24,11 @@ namespace sf {
 
 namespace {
 
+// Reverse passes run one wave per workgroup: small windows (tens of thousands
+// of particles) still spread over every CU, and the pair kernels are latency
+// bound (dependent loads of neighbour data), not LDS or occupancy bound.
+constexpr int kAB = 64;
+
 template <int K>
 SF_DEV void contract3(const G3<Dn<K>>& o, float3 ob, float* g) {
 #pragma unroll
@@ -59,9 +64,9 @@ SF_DEV void boundary_neighbours(const SFConst& k, const TapeStep& t, float4 p, F
 SF_DEV float sgn1(float v) { return (float)((0.f < v) - (v < 0.f)); }
 
 // ---- A0: advect (symplectic Euler, sand damping, water box clamp) ---------
-__global__ __launch_bounds__(kBlock) void k_adj_advect(SFConst k, TapeStep t, AdjFields a, const float* sat_seed,
+__global__ __launch_bounds__(kAB) void k_adj_advect(SFConst k, TapeStep t, AdjFields a, const float* sat_seed,
                                                        float dt, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = blockIdx.x * kAB + threadIdx.x;
     if (i >= n) return;
     const float4 p = t.pm[i], v = t.vr[i], ac = t.acc[i];
     const float4 gx = a.gx[i], gv = a.gv[i];
@@ -124,8 +129,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect(SFConst k, TapeStep t, Ad
 }
 
 // ---- A5: water momentum ----------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_adj_water(SFConst k, TapeStep t, AdjFields a, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kAB) void k_adj_water(SFConst k, TapeStep t, AdjFields a, int n) {
+    const int i = blockIdx.x * kAB + threadIdx.x;
     if (i >= n) return;
     const float4 pk = t.pm[i];
     const float4 vk = t.vr[i];
@@ -223,8 +228,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_water(SFConst k, TapeStep t, Adj
 }
 
 // ---- A4 (local): sand i's force and vw adjoints, drag, walls ------------------
-__global__ __launch_bounds__(kBlock) void k_adj_sand_local(SFConst k, TapeStep t, AdjFields a, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kAB) void k_adj_sand_local(SFConst k, TapeStep t, AdjFields a, int n) {
+    const int i = blockIdx.x * kAB + threadIdx.x;
     if (i >= n) return;
     const float4 p = t.pm[i];
     if (p.w >= 0.f) {
@@ -284,8 +289,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_sand_local(SFConst k, TapeStep t
 }
 
 // ---- A4 (pairs): sand-sand contacts / bridges, sand-water buoyancy + adhesion term
-__global__ __launch_bounds__(kBlock) void k_adj_sand_pairs(SFConst k, TapeStep t, AdjFields a, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kAB) void k_adj_sand_pairs(SFConst k, TapeStep t, AdjFields a, int n) {
+    const int i = blockIdx.x * kAB + threadIdx.x;
     if (i >= n) return;
     const float4 pk = t.pm[i];
     const float4 vk = t.vr[i];
@@ -411,8 +416,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_sand_pairs(SFConst k, TapeStep t
 }
 
 // ---- A3 (local) -------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_adj_voidage_local(SFConst k, TapeStep t, AdjFields a, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kAB) void k_adj_voidage_local(SFConst k, TapeStep t, AdjFields a, int n) {
+    const int i = blockIdx.x * kAB + threadIdx.x;
     if (i >= n) return;
     const float4 p = t.pm[i];
     const float den = t.den4[i].y;
@@ -439,8 +444,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_voidage_local(SFConst k, TapeSte
 }
 
 // ---- A3 (pairs) ---------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_adj_voidage_pairs(SFConst k, TapeStep t, AdjFields a, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kAB) void k_adj_voidage_pairs(SFConst k, TapeStep t, AdjFields a, int n) {
+    const int i = blockIdx.x * kAB + threadIdx.x;
     if (i >= n) return;
     const float4 pk = t.pm[i];
     const bool sand = pk.w < 0.f;
@@ -506,8 +511,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_voidage_pairs(SFConst k, TapeSte
 }
 
 // ---- A2 (local): rho~ factors of water, average-flow normalisation of sand ----
-__global__ __launch_bounds__(kBlock) void k_adj_flow_local(SFConst k, TapeStep t, AdjFields a, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kAB) void k_adj_flow_local(SFConst k, TapeStep t, AdjFields a, int n) {
+    const int i = blockIdx.x * kAB + threadIdx.x;
     if (i >= n) return;
     const float4 p = t.pm[i];
     if (p.w > 0.f) {
@@ -547,8 +552,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_flow_local(SFConst k, TapeStep t
 }
 
 // ---- A2 (pairs) ---------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_adj_flow_pairs(SFConst k, TapeStep t, AdjFields a, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kAB) void k_adj_flow_pairs(SFConst k, TapeStep t, AdjFields a, int n) {
+    const int i = blockIdx.x * kAB + threadIdx.x;
     if (i >= n) return;
     const float4 pk = t.pm[i];
     const bool sand = pk.w < 0.f;
@@ -590,8 +595,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_flow_pairs(SFConst k, TapeStep t
 }
 
 // ---- A1: density and EOS ------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_adj_density_local(SFConst k, TapeStep t, AdjFields a, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kAB) void k_adj_density_local(SFConst k, TapeStep t, AdjFields a, int n) {
+    const int i = blockIdx.x * kAB + threadIdx.x;
     if (i >= n) return;
     if (t.pm[i].w < 0.f) {
         a.grp[i] = make_float2(0.f, 0.f);
@@ -610,8 +615,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_density_local(SFConst k, TapeSte
     const float2 g = a.grp[i];
     a.grp[i] = make_float2(g.x + g.y * dp, 0.f);
 }
-__global__ __launch_bounds__(kBlock) void k_adj_density_pairs(SFConst k, TapeStep t, AdjFields a, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kAB) void k_adj_density_pairs(SFConst k, TapeStep t, AdjFields a, int n) {
+    const int i = blockIdx.x * kAB + threadIdx.x;
     if (i >= n) return;
     const float4 pk = t.pm[i];
     if (pk.w < 0.f) return;
@@ -636,35 +641,35 @@ __global__ __launch_bounds__(kBlock) void k_adj_density_pairs(SFConst k, TapeSte
     a.gx[i] = gx;
 }
 
-__global__ __launch_bounds__(kBlock) void k_adj_scatter(const uint32_t* __restrict__ perm, const float4* __restrict__ gx,
+__global__ __launch_bounds__(kAB) void k_adj_scatter(const uint32_t* __restrict__ perm, const float4* __restrict__ gx,
                                                         const float4* __restrict__ gv, float4* __restrict__ gxp,
                                                         float4* __restrict__ gvp, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = blockIdx.x * kAB + threadIdx.x;
     if (i >= n) return;
     const uint32_t o = perm[i];
     gxp[o] = gx[i];
     gvp[o] = gv[i];
 }
-__global__ __launch_bounds__(kBlock) void k_iota(uint32_t* out, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kAB) void k_iota(uint32_t* out, int n) {
+    const int i = blockIdx.x * kAB + threadIdx.x;
     if (i < n) out[i] = (uint32_t)i;
 }
-__global__ __launch_bounds__(kBlock) void k_gather_u32(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ in,
+__global__ __launch_bounds__(kAB) void k_gather_u32(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ in,
                                                        uint32_t* __restrict__ out, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = blockIdx.x * kAB + threadIdx.x;
     if (i < n) out[i] = in[perm[i]];
 }
-__global__ __launch_bounds__(kBlock) void k_scatter_by_id(const uint32_t* __restrict__ id, const float* __restrict__ src,
+__global__ __launch_bounds__(kAB) void k_scatter_by_id(const uint32_t* __restrict__ id, const float* __restrict__ src,
                                                           int stride, int first, int comps, float* __restrict__ dst,
                                                           int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = blockIdx.x * kAB + threadIdx.x;
     if (i >= n) return;
     const uint32_t o = id[i];
     for (int c = 0; c < comps; ++c) dst[(size_t)o * comps + c] = src[(size_t)i * stride + first + c];
 }
-__global__ __launch_bounds__(kBlock) void k_gather_by_id(const uint32_t* __restrict__ id, const float* __restrict__ src,
+__global__ __launch_bounds__(kAB) void k_gather_by_id(const uint32_t* __restrict__ id, const float* __restrict__ src,
                                                          int comps, float4* __restrict__ dst, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = blockIdx.x * kAB + threadIdx.x;
     if (i >= n) return;
     float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
     if (src) {
@@ -677,14 +682,14 @@ __global__ __launch_bounds__(kBlock) void k_gather_by_id(const uint32_t* __restr
     }
     dst[i] = r;
 }
-__global__ __launch_bounds__(kBlock) void k_gather_by_id1(const uint32_t* __restrict__ id, const float* __restrict__ src,
+__global__ __launch_bounds__(kAB) void k_gather_by_id1(const uint32_t* __restrict__ id, const float* __restrict__ src,
                                                           float* __restrict__ dst, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = blockIdx.x * kAB + threadIdx.x;
     if (i < n) dst[i] = src ? src[id[i]] : 0.f;
 }
-__global__ __launch_bounds__(kBlock) void k_set_vel_by_id(const uint32_t* __restrict__ id, const float* __restrict__ vel3,
+__global__ __launch_bounds__(kAB) void k_set_vel_by_id(const uint32_t* __restrict__ id, const float* __restrict__ vel3,
                                                           float4* __restrict__ vr, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = blockIdx.x * kAB + threadIdx.x;
     if (i >= n) return;
     const size_t o = (size_t)id[i] * 3;
     float4 v = vr[i];
@@ -694,7 +699,7 @@ __global__ __launch_bounds__(kBlock) void k_set_vel_by_id(const uint32_t* __rest
     vr[i] = v;
 }
 
-inline int blocks(int n) { return (n + kBlock - 1) / kBlock; }
+inline int blocks(int n) { return (n + kAB - 1) / kAB; }
 
 }  // namespace
 
@@ -702,39 +707,39 @@ void launch_adj_step(const SFConst& k, const TapeStep& t, const AdjFields& a, co
                      hipStream_t s) {
     if (n <= 0) return;
     const int g = blocks(n);
-    k_adj_advect<<<g, kBlock, 0, s>>>(k, t, a, sat_seed, dt, n);
-    k_adj_water<<<g, kBlock, 0, s>>>(k, t, a, n);
-    k_adj_sand_local<<<g, kBlock, 0, s>>>(k, t, a, n);
-    k_adj_sand_pairs<<<g, kBlock, 0, s>>>(k, t, a, n);
-    k_adj_voidage_local<<<g, kBlock, 0, s>>>(k, t, a, n);
-    k_adj_voidage_pairs<<<g, kBlock, 0, s>>>(k, t, a, n);
-    k_adj_flow_local<<<g, kBlock, 0, s>>>(k, t, a, n);
-    k_adj_flow_pairs<<<g, kBlock, 0, s>>>(k, t, a, n);
-    k_adj_density_local<<<g, kBlock, 0, s>>>(k, t, a, n);
-    k_adj_density_pairs<<<g, kBlock, 0, s>>>(k, t, a, n);
+    k_adj_advect<<<g, kAB, 0, s>>>(k, t, a, sat_seed, dt, n);
+    k_adj_water<<<g, kAB, 0, s>>>(k, t, a, n);
+    k_adj_sand_local<<<g, kAB, 0, s>>>(k, t, a, n);
+    k_adj_sand_pairs<<<g, kAB, 0, s>>>(k, t, a, n);
+    k_adj_voidage_local<<<g, kAB, 0, s>>>(k, t, a, n);
+    k_adj_voidage_pairs<<<g, kAB, 0, s>>>(k, t, a, n);
+    k_adj_flow_local<<<g, kAB, 0, s>>>(k, t, a, n);
+    k_adj_flow_pairs<<<g, kAB, 0, s>>>(k, t, a, n);
+    k_adj_density_local<<<g, kAB, 0, s>>>(k, t, a, n);
+    k_adj_density_pairs<<<g, kAB, 0, s>>>(k, t, a, n);
 }
 void launch_adj_scatter(const uint32_t* perm, const float4* gx, const float4* gv, float4* gx_prev, float4* gv_prev,
                         int n, hipStream_t s) {
-    if (n > 0) k_adj_scatter<<<blocks(n), kBlock, 0, s>>>(perm, gx, gv, gx_prev, gv_prev, n);
+    if (n > 0) k_adj_scatter<<<blocks(n), kAB, 0, s>>>(perm, gx, gv, gx_prev, gv_prev, n);
 }
 void launch_iota(uint32_t* out, int n, hipStream_t s) {
-    if (n > 0) k_iota<<<blocks(n), kBlock, 0, s>>>(out, n);
+    if (n > 0) k_iota<<<blocks(n), kAB, 0, s>>>(out, n);
 }
 void launch_gather_u32(const uint32_t* perm, const uint32_t* in, uint32_t* out, int n, hipStream_t s) {
-    if (n > 0) k_gather_u32<<<blocks(n), kBlock, 0, s>>>(perm, in, out, n);
+    if (n > 0) k_gather_u32<<<blocks(n), kAB, 0, s>>>(perm, in, out, n);
 }
 void launch_scatter_by_id(const uint32_t* id, const float* src, int src_stride, int first, int comps, float* dst, int n,
                           hipStream_t s) {
-    if (n > 0) k_scatter_by_id<<<blocks(n), kBlock, 0, s>>>(id, src, src_stride, first, comps, dst, n);
+    if (n > 0) k_scatter_by_id<<<blocks(n), kAB, 0, s>>>(id, src, src_stride, first, comps, dst, n);
 }
 void launch_gather_by_id(const uint32_t* id, const float* src, int comps, float4* dst, int n, hipStream_t s) {
-    if (n > 0) k_gather_by_id<<<blocks(n), kBlock, 0, s>>>(id, src, comps, dst, n);
+    if (n > 0) k_gather_by_id<<<blocks(n), kAB, 0, s>>>(id, src, comps, dst, n);
 }
 void launch_gather_by_id1(const uint32_t* id, const float* src, float* dst, int n, hipStream_t s) {
-    if (n > 0) k_gather_by_id1<<<blocks(n), kBlock, 0, s>>>(id, src, dst, n);
+    if (n > 0) k_gather_by_id1<<<blocks(n), kAB, 0, s>>>(id, src, dst, n);
 }
 void launch_set_vel_by_id(const uint32_t* id, const float* vel3, float4* vr, int n, hipStream_t s) {
-    if (n > 0) k_set_vel_by_id<<<blocks(n), kBlock, 0, s>>>(id, vel3, vr, n);
+    if (n > 0) k_set_vel_by_id<<<blocks(n), kAB, 0, s>>>(id, vel3, vr, n);
 }
 
 }  // namespace sf
