@@ -191,6 +191,8 @@ def main() -> int:
     ap.add_argument("--workload", default="drop_in_1m", choices=["drop_in_1m", "small_block", "dam_4m"])
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-window measurement")
+    ap.add_argument("--parts", default="step,ensemble,diff",
+                    help="comma list of step (the headline workload), ensemble, diff; a profiler can run one")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -203,71 +205,79 @@ def main() -> int:
     import sph_seepage_flows_amd as sf
     from sph_seepage_flows_amd import scenes
 
+    parts = set(args.parts.split(","))
     scene = {"drop_in_1m": scenes.drop_in_1m, "small_block": scenes.small_block, "dam_4m": scenes.dam}[args.workload]()
     dt = 1e-4
-    print(f"[bench] scene {scene.name}: n={scene.n} nb={scene.nb}", file=sys.stderr, flush=True)
-    g = sf.SFSystem.from_scene(scene, device=0)
-    n, nb, cells = g.sizes()
-    if args.warmup:
-        g.step(dt, args.warmup)
-    g.synchronize()
-    t0 = time.perf_counter()
-    total_ms, pass_ms = g.step_profiled(dt, args.steps, per_pass=True)
-    wall = time.perf_counter() - t0
-    ms_per_step = total_ms / args.steps
-    value = n * args.steps / (total_ms / 1e3)
-    print(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step (events), wall {wall * 1e3 / args.steps:.3f} "
-          f"ms/step; passes {json.dumps({k: round(v, 4) for k, v in pass_ms.items()})}", file=sys.stderr, flush=True)
-
-    n_s = int(np.count_nonzero(scene.label == 1))
-    n_w = n - n_s
-    pb = pass_bytes(n_w, n_s, nb, cells, radix_passes(cells), scene.physics.sf_type == 1)
-    dom = max(pass_ms, key=pass_ms.get)
-    ach = pb[dom] / (pass_ms[dom] * 1e-3) / 1e9
-    step_bytes = sum(pb.values())
-    traffic = None
-    prof = ROOT / "profiles" / f"pmc_{dom}_{scene.name}.json"
-    if prof.exists():
-        try:
-            traffic = json.loads(prof.read_text()).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": None,
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{scene.name}: {n_w} water + {n_s} sand, {nb} boundary particles, "
-                               f"{cells} grid cells, dt=1e-4, FP32", "n_particles": n, "parallelism": "single-gpu"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "alg_bytes_per_launch": pb[dom],
-                     "mean_launch_ms": pass_ms[dom]},
-        "step_roofline": {"alg_bytes_per_particle_step": step_bytes / n,
-                          "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
-                          "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
-        "pass_ms": pass_ms,
+        "config": {"workload": scene.name, "parallelism": "single-gpu"},
+        "roofline": None,
         "cpu_baseline": None,
         "batched": None,
         "differentiation": None,
     }
-    g.close()
-    if not args.no_ensemble:
+    if "step" in parts:
+        print(f"[bench] scene {scene.name}: n={scene.n} nb={scene.nb}", file=sys.stderr, flush=True)
+        g = sf.SFSystem.from_scene(scene, device=0)
+        n, nb, cells = g.sizes()
+        if args.warmup:
+            g.step(dt, args.warmup)
+        g.synchronize()
+        t0 = time.perf_counter()
+        total_ms, pass_ms = g.step_profiled(dt, args.steps, per_pass=True)
+        wall = time.perf_counter() - t0
+        g.close()
+        ms_per_step = total_ms / args.steps
+        print(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step (events), wall {wall * 1e3 / args.steps:.3f} "
+              f"ms/step; passes {json.dumps({k: round(v, 4) for k, v in pass_ms.items()})}", file=sys.stderr,
+              flush=True)
+        n_s = int(np.count_nonzero(scene.label == 1))
+        n_w = n - n_s
+        pb = pass_bytes(n_w, n_s, nb, cells, radix_passes(cells), scene.physics.sf_type == 1)
+        dom = max(pass_ms, key=pass_ms.get)
+        ach = pb[dom] / (pass_ms[dom] * 1e-3) / 1e9
+        step_bytes = sum(pb.values())
+        traffic = None
+        prof = ROOT / "profiles" / f"pmc_{dom}_{scene.name}.json"
+        if prof.exists():
+            try:
+                traffic = json.loads(prof.read_text()).get("hbm_bytes_per_launch")
+            except Exception:
+                traffic = None
+        out.update({
+            "value": n * args.steps / (total_ms / 1e3),
+            "ms_per_step": ms_per_step,
+            "config": {"workload": f"{scene.name}: {n_w} water + {n_s} sand, {nb} boundary particles, "
+                                   f"{cells} grid cells, dt=1e-4, FP32", "n_particles": n,
+                       "parallelism": "single-gpu"},
+            "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "alg_bytes_per_launch": pb[dom],
+                         "mean_launch_ms": pass_ms[dom]},
+            "step_roofline": {"alg_bytes_per_particle_step": step_bytes / n,
+                              "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
+                              "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+            "pass_ms": pass_ms,
+        })
+    if "ensemble" in parts and not args.no_ensemble:
         print("[bench] batched ensemble...", file=sys.stderr, flush=True)
         out["batched"] = ensemble_bench(dt, 50, 5)
         print(f"[bench] batched: {json.dumps(out['batched'])}", file=sys.stderr, flush=True)
-    if not args.no_diff:
+    if "diff" in parts and not args.no_diff:
         print("[bench] differentiable window...", file=sys.stderr, flush=True)
         out["differentiation"] = diff_bench(dt, check=not args.no_cpu_baseline)
         print(f"[bench] differentiation: {json.dumps(out['differentiation'])}", file=sys.stderr, flush=True)
-    if rank == 0 and not args.no_cpu_baseline:
+    if rank == 0 and "step" in parts and not args.no_cpu_baseline:
         print(f"[bench] cpu baseline ({args.cpu_steps} oracle steps)...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_steps, dt)
     if rank == 0:
