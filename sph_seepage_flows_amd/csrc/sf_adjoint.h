@@ -43,6 +43,8 @@ struct AdjFields {
 // be null) adds d loss / d saturation of this step.
 void launch_adj_step(const SFConst& k, const TapeStep& t, const AdjFields& a, const float* sat_seed, float dt, int n,
                      hipStream_t s);
+// Lanes per particle the reverse pair passes use for n particles (1, 2, 4 or 8).
+int adj_lanes(int n);
 // Undo the step's gather: prev[perm[k]] = cur[k] for position and velocity adjoints.
 void launch_adj_scatter(const uint32_t* perm, const float4* gx, const float4* gv, float4* gx_prev, float4* gv_prev,
                         int n, hipStream_t s);

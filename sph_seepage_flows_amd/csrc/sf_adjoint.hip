@@ -42,20 +42,47 @@ SF_DEV void add3(float4& a, float x, float y, float z) {
 }
 SF_DEV void add3(float4& a, v3 b) { add3(a, b.x, b.y, b.z); }
 
-template <class F>
-SF_DEV void neighbours(const SFConst& k, const TapeStep& t, float4 p, F&& fn) {
+// Lanes per particle: L consecutive lanes share one particle; lane `sub`
+// walks the neighbour columns q with q % L == sub, and the partial sums are
+// combined by a fixed butterfly (deterministic for a given L).  Small
+// windows get L > 1 so that their few thousand waves hide load latency.
+template <int L>
+SF_DEV float lsum(float x) {
+    if constexpr (L >= 2) x += __shfl_xor(x, 1);
+    if constexpr (L >= 4) x += __shfl_xor(x, 2);
+    if constexpr (L >= 8) x += __shfl_xor(x, 4);
+    return x;
+}
+template <int L>
+SF_DEV float4 lsum(float4 a) {
+    return make_float4(lsum<L>(a.x), lsum<L>(a.y), lsum<L>(a.z), lsum<L>(a.w));
+}
+SF_DEV float4 operator+(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+
+// Phases a neighbour loop needs: ranges without any are skipped through the
+// tape's cell table (cellStart, water count before it).
+constexpr int kNeedWater = 1, kNeedSand = 2, kNeedBoth = 3;
+
+template <int L, class F>
+SF_DEV void neighbours(const SFConst& k, const TapeStep& t, float4 p, int sub, int need, F&& fn) {
     int cx, cy, cz;
     cell_of(k, p.x, p.y, p.z, cx, cy, cz);
+    int q = 0;
     for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
-        const uint32_t e = t.ct[c1].x;
-        for (uint32_t j = t.ct[c0].x; j < e; ++j) fn(j);
+        if ((q++ % L) != sub) return;
+        const uint2 a = t.ct[c0], b = t.ct[c1];
+        const uint32_t nw = b.y - a.y, ns = (b.x - a.x) - nw;
+        if (!(((need & kNeedWater) && nw) || ((need & kNeedSand) && ns))) return;
+        for (uint32_t j = a.x; j < b.x; ++j) fn(j);
     });
 }
-template <class F>
-SF_DEV void boundary_neighbours(const SFConst& k, const TapeStep& t, float4 p, F&& fn) {
+template <int L, class F>
+SF_DEV void boundary_neighbours(const SFConst& k, const TapeStep& t, float4 p, int sub, F&& fn) {
     int cx, cy, cz;
     cell_of(k, p.x, p.y, p.z, cx, cy, cz);
+    int q = 0;
     for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+        if ((q++ % L) != sub) return;
         const uint32_t e = t.bcell[c1];
         for (uint32_t j = t.bcell[c0]; j < e; ++j) fn(j);
     });
@@ -129,81 +156,64 @@ __global__ __launch_bounds__(kAB) void k_adj_advect(SFConst k, TapeStep t, AdjFi
 }
 
 // ---- A5: water momentum ----------------------------------------------------
+template <int L>
 __global__ __launch_bounds__(kAB) void k_adj_water(SFConst k, TapeStep t, AdjFields a, int n) {
-    const int i = blockIdx.x * kAB + threadIdx.x;
+    const int tid = blockIdx.x * kAB + threadIdx.x;
+    const int i = tid / L, sub = tid % L;
     if (i >= n) return;
     const float4 pk = t.pm[i];
     const float4 vk = t.vr[i];
     const bool water = pk.w > 0.f;
     const float mk = fabsf(pk.w);
-    float4 gx = a.gx[i], gv = a.gv[i];
+    float4 gx = make_float4(0.f, 0.f, 0.f, 0.f), gv = gx, gdr = gx;
     float gp = 0.f, grt = 0.f, gpw = 0.f;
-    float4 gdr = make_float4(0.f, 0.f, 0.f, 0.f);
     const float3 ak = f3(a.gacc[i]);
     float2 wtk = water ? t.wt[i] : make_float2(0.f, 0.f);
     const float p_k = t.rp[i].y;
     const float4 xfk = t.xf[i];
     using D8 = Dn<8>;
-    using D6 = Dn<6>;
-    using D5 = Dn<5>;
-    neighbours(k, t, pk, [&](uint32_t j) {
+    neighbours<L>(k, t, pk, sub, water ? kNeedBoth : kNeedWater, [&](uint32_t j) {
         if ((int)j == i) return;
         const float4 pj = t.pm[j];
-        const float dx = pk.x - pj.x, dy = pk.y - pj.y, dz = pk.z - pj.z;
+        const float dx = pk.x - pj.x, dy = pk.y - pj.y, dz = pk.z - pj.z;  // x_k - x_j
         if (dx * dx + dy * dy + dz * dz >= k.h2) return;
-        const float4 vj = t.vr[j];
-        if (water) {  // i-side: k is the centre
-            if (pj.w > 0.f) {
-                const float2 wtj = t.wt[j];
-                G3<D8> d = {dvar<8>(dx, 0), dvar<8>(dy, 1), dvar<8>(dz, 2)};
-                G3<D8> dv = {dvar<8>(vk.x - vj.x, 3), dvar<8>(vk.y - vj.y, 4), dvar<8>(vk.z - vj.z, 5)};
-                G3<D8> o = t_water_water(k, d, dv, dvar<8>(wtk.y, 6), D8(wtj.y), dvar<8>(wtk.x, 7), D8(wtj.x), pj.w);
-                float g[8];
-                contract3(o, ak, g);
-                add3(gx, g[0], g[1], g[2]);
-                add3(gv, g[3], g[4], g[5]);
-                gpw += g[6];
-                grt += g[7];
-            } else {
-                const float4 drj = t.xf[j];
-                G3<D5> d = {dvar<5>(dx, 0), dvar<5>(dy, 1), dvar<5>(dz, 2)};
-                G3<D5> o = t_water_sand(k, d, dvar<5>(p_k, 3), dvar<5>(wtk.x, 4), lift<D5>(drj.x, drj.y, drj.z),
-                                        -pj.w, t.rp[j].x);
-                float g[5];
-                contract3(o, ak, g);
-                add3(gx, g[0], g[1], g[2]);
-                gp += g[3];
-                grt += g[4];
-            }
-        }
-        if (pj.w > 0.f) {  // j-side: water j is the centre, k its neighbour
-            const float3 aj = f3(a.gacc[j]);
+        if (!water && pj.w < 0.f) return;
+        if (water && pj.w > 0.f) {
+            // water-water: the pair term is antisymmetric up to the mass ratio,
+            // T(j <- k) = -(m_k / m_j) T(k <- j), so one Jacobian of T(k <- j)
+            // serves both sides, contracted with a_k - (m_k / m_j) a_j.
+            const float4 vj = t.vr[j];
             const float2 wtj = t.wt[j];
-            const float4 ppj = t.pm[j];
-            if (water) {
-                G3<D8> d = {dvar<8>(-dx, 0), dvar<8>(-dy, 1), dvar<8>(-dz, 2)};
-                G3<D8> dv = {dvar<8>(vj.x - vk.x, 3), dvar<8>(vj.y - vk.y, 4), dvar<8>(vj.z - vk.z, 5)};
-                G3<D8> o = t_water_water(k, d, dv, D8(wtj.y), dvar<8>(wtk.y, 6), D8(wtj.x), dvar<8>(wtk.x, 7), mk);
-                float g[8];
-                contract3(o, aj, g);
-                add3(gx, -g[0], -g[1], -g[2]);
-                add3(gv, -g[3], -g[4], -g[5]);
-                gpw += g[6];
-                grt += g[7];
-            } else {
-                G3<D6> d = {dvar<6>(-dx, 0), dvar<6>(-dy, 1), dvar<6>(-dz, 2)};
-                G3<D6> dr = {dvar<6>(xfk.x, 3), dvar<6>(xfk.y, 4), dvar<6>(xfk.z, 5)};
-                G3<D6> o = t_water_sand(k, d, D6(t.rp[j].y), D6(t.wt[j].x), dr, mk, t.rp[i].x);
-                float g[6];
-                contract3(o, aj, g);
-                add3(gx, -g[0], -g[1], -g[2]);
-                add3(gdr, g[3], g[4], g[5]);
-                (void)ppj;
-            }
+            const float3 aj = f3(a.gacc[j]);
+            const float c = mk / pj.w;
+            const float3 comb = make_float3(ak.x - c * aj.x, ak.y - c * aj.y, ak.z - c * aj.z);
+            G3<D8> d = {dvar<8>(dx, 0), dvar<8>(dy, 1), dvar<8>(dz, 2)};
+            G3<D8> dv = {dvar<8>(vk.x - vj.x, 3), dvar<8>(vk.y - vj.y, 4), dvar<8>(vk.z - vj.z, 5)};
+            G3<D8> o = t_water_water(k, d, dv, dvar<8>(wtk.y, 6), D8(wtj.y), dvar<8>(wtk.x, 7), D8(wtj.x), pj.w);
+            float g[8];
+            contract3(o, comb, g);
+            add3(gx, g[0], g[1], g[2]);
+            add3(gv, g[3], g[4], g[5]);
+            gpw += g[6];
+            grt += g[7];
+        } else if (water) {  // i-side: sand j's seepage terms on water k
+            const float4 drj = t.xf[j];
+            v3 gd = mk3(0.f, 0.f, 0.f), gdr_unused = gd;
+            vjp_water_sand(k, mk3(dx, dy, dz), p_k, wtk.x, xyz(drj), -pj.w, t.rp[j].x, mk3(ak.x, ak.y, ak.z), gd, gp,
+                           grt, gdr_unused);
+            add3(gx, gd);
+        } else {  // j-side: water j's seepage terms with sand k (k's avgDrag)
+            const float3 aj = f3(a.gacc[j]);
+            v3 gd = mk3(0.f, 0.f, 0.f), gdr3 = gd;
+            float gp_unused = 0.f, grt_unused = 0.f;
+            vjp_water_sand(k, mk3(-dx, -dy, -dz), t.rp[j].y, t.wt[j].x, xyz(xfk), mk, t.rp[i].x,
+                           mk3(aj.x, aj.y, aj.z), gd, gp_unused, grt_unused, gdr3);
+            add3(gx, -gd.x, -gd.y, -gd.z);
+            add3(gdr, gdr3);
         }
     });
     if (water) {
-        boundary_neighbours(k, t, pk, [&](uint32_t b) {
+        boundary_neighbours<L>(k, t, pk, sub, [&](uint32_t b) {
             const float4 bp = t.bpv[b];
             const float dx = pk.x - bp.x, dy = pk.y - bp.y, dz = pk.z - bp.z;
             if (dx * dx + dy * dy + dz * dz >= k.h2) return;
@@ -217,14 +227,23 @@ __global__ __launch_bounds__(kAB) void k_adj_water(SFConst k, TapeStep t, AdjFie
             add3(gv, g[3], g[4], g[5]);
             gp += g[6];
         });
+    }
+    gx = lsum<L>(gx);
+    gv = lsum<L>(gv);
+    gdr = lsum<L>(gdr);
+    gp = lsum<L>(gp);
+    grt = lsum<L>(grt);
+    gpw = lsum<L>(gpw);
+    if (sub != 0) return;
+    if (water) {
         a.gadh[i] = make_float4(-ak.x / mk, -ak.y / mk, -ak.z / mk, 0.f);
         a.gwt[i] = make_float2(grt, gpw);
         a.grp[i].y += gp;
     } else {
         a.gxf[i] = gdr;  // d loss / d avgDrag of sand k
     }
-    a.gx[i] = gx;
-    a.gv[i] = gv;
+    a.gx[i] = a.gx[i] + gx;
+    a.gv[i] = a.gv[i] + gv;
 }
 
 // ---- A4 (local): sand i's force and vw adjoints, drag, walls ------------------
@@ -289,122 +308,96 @@ __global__ __launch_bounds__(kAB) void k_adj_sand_local(SFConst k, TapeStep t, A
 }
 
 // ---- A4 (pairs): sand-sand contacts / bridges, sand-water buoyancy + adhesion term
+template <int L>
 __global__ __launch_bounds__(kAB) void k_adj_sand_pairs(SFConst k, TapeStep t, AdjFields a, int n) {
-    const int i = blockIdx.x * kAB + threadIdx.x;
+    const int tid = blockIdx.x * kAB + threadIdx.x;
+    const int i = tid / L, sub = tid % L;
     if (i >= n) return;
     const float4 pk = t.pm[i];
     const float4 vk = t.vr[i];
     const bool sand = pk.w < 0.f;
     const float mk = fabsf(pk.w);
     const float twoR = 2.f * k.R;
-    float4 gx = a.gx[i], gv = a.gv[i], gth = a.gth[i];
+    float4 gx = make_float4(0.f, 0.f, 0.f, 0.f), gv = gx, gth = gx, gadh = gx;
     float gsat = 0.f, grho = 0.f, geps = 0.f, gp = 0.f;
-    float4 gadh = make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 ok = a.o4[i];
     const float3 fk = make_float3(ok.x, ok.y, ok.z);
+    const v3 fk3 = mk3(ok.x, ok.y, ok.z);
     const float sat_k = t.es[i].y;
     const float rho_k = t.rp[i].x;
-    neighbours(k, t, pk, [&](uint32_t j) {
+    neighbours<L>(k, t, pk, sub, sand ? kNeedBoth : kNeedSand, [&](uint32_t j) {
         if ((int)j == i) return;
         const float4 pj = t.pm[j];
         const float dx = pj.x - pk.x, dy = pj.y - pk.y, dz = pj.z - pk.z;  // x_j - x_k
         const float r2 = dx * dx + dy * dy + dz * dz;
         if (sand && pj.w < 0.f) {
+            // sand-sand: DEM and bridge terms are antisymmetric, T(j <- k) = -T(k <- j),
+            // so one Jacobian of T(k <- j) serves both sides (contracted with f_j - f_k)
             const float dist = sqrtf(r2);
+            if (dist >= twoR + k.cap_srup) return;
             const float4 vj = t.vr[j];
             const float4 oj = a.o4[j];
-            const float3 fj = make_float3(oj.x, oj.y, oj.z);
+            const float3 dfb = make_float3(oj.x - fk.x, oj.y - fk.y, oj.z - fk.z);
             if (dist < twoR) {
                 using D7 = Dn<7>;
-                {  // i-side: f_k += dem(x_j - x_k, v_j - v_k)
-                    G3<D7> d = {dvar<7>(dx, 0), dvar<7>(dy, 1), dvar<7>(dz, 2)};
-                    G3<D7> vv = {dvar<7>(vj.x - vk.x, 3), dvar<7>(vj.y - vk.y, 4), dvar<7>(vj.z - vk.z, 5)};
-                    G3<D7> o = t_dem(k, d, vv, twoR, dvar<7>(k.tanphi, 6));
-                    float g[7];
-                    contract3(o, fk, g);
-                    add3(gx, -g[0], -g[1], -g[2]);
-                    add3(gv, -g[3], -g[4], -g[5]);
-                    gth.w += g[6];
-                }
-                {  // j-side: f_j += dem(x_k - x_j, v_k - v_j)
-                    G3<D7> d = {dvar<7>(-dx, 0), dvar<7>(-dy, 1), dvar<7>(-dz, 2)};
-                    G3<D7> vv = {dvar<7>(vk.x - vj.x, 3), dvar<7>(vk.y - vj.y, 4), dvar<7>(vk.z - vj.z, 5)};
-                    G3<D7> o = t_dem(k, d, vv, twoR, D7(k.tanphi));
-                    float g[7];
-                    contract3(o, fj, g);
-                    add3(gx, g[0], g[1], g[2]);
-                    add3(gv, g[3], g[4], g[5]);
-                }
+                G3<D7> d = {dvar<7>(dx, 0), dvar<7>(dy, 1), dvar<7>(dz, 2)};
+                G3<D7> vv = {dvar<7>(vj.x - vk.x, 3), dvar<7>(vj.y - vk.y, 4), dvar<7>(vj.z - vk.z, 5)};
+                G3<D7> o = t_dem(k, d, vv, twoR, dvar<7>(k.tanphi, 6));
+                float g[7];
+                contract3(o, dfb, g);
+                add3(gx, g[0], g[1], g[2]);
+                add3(gv, g[3], g[4], g[5]);
+                gth.w += fk.x * o.x.d[6] + fk.y * o.y.d[6] + fk.z * o.z.d[6];
             } else {
                 const float H = dist - twoR;
                 if (H > 0.f && H < k.cap_srup) {
                     using D6 = Dn<6>;
                     const float sr = (sat_k + t.es[j].y) * 0.5f;
-                    {  // i-side
-                        G3<D6> d = {dvar<6>(dx, 0), dvar<6>(dy, 1), dvar<6>(dz, 2)};
-                        G3<D6> o = t_capillary(k, d, dvar<6>(sr, 3), dvar<6>(k.gcap.py0, 4), dvar<6>(k.gcap.py1, 5));
-                        float g[6];
-                        contract3(o, fk, g);
-                        add3(gx, -g[0], -g[1], -g[2]);
-                        gsat += 0.5f * g[3];
-                        gth.y += g[4];
-                        gth.z += g[5];
-                    }
-                    {  // j-side
-                        G3<D6> d = {dvar<6>(-dx, 0), dvar<6>(-dy, 1), dvar<6>(-dz, 2)};
-                        G3<D6> o = t_capillary(k, d, dvar<6>(sr, 3), D6(k.gcap.py0), D6(k.gcap.py1));
-                        float g[6];
-                        contract3(o, fj, g);
-                        add3(gx, g[0], g[1], g[2]);
-                        gsat += 0.5f * g[3];
-                    }
+                    G3<D6> d = {dvar<6>(dx, 0), dvar<6>(dy, 1), dvar<6>(dz, 2)};
+                    G3<D6> o = t_capillary(k, d, dvar<6>(sr, 3), dvar<6>(k.gcap.py0, 4), dvar<6>(k.gcap.py1, 5));
+                    float g[6], h[6];
+                    contract3(o, dfb, g);
+                    contract3(o, fk, h);
+                    add3(gx, g[0], g[1], g[2]);
+                    gsat += -0.5f * g[3];
+                    gth.y += h[4];
+                    gth.z += h[5];
                 }
             }
             return;
         }
         if (r2 >= k.h2) return;
         if (sand && pj.w > 0.f) {  // i-side: water j acts on sand k
-            using D3 = Dn<3>;
             const float2 rpj = t.rp[j];
-            const float4 xfj = t.xf[j];
-            G3<D3> d = {dvar<3>(dx, 0), dvar<3>(dy, 1), dvar<3>(dz, 2)};
-            G3<D3> f;
-            D3 vw;
-            t_sand_water(k, d, D3(rpj.x), D3(t.es[j].x), D3(rpj.y), lift<D3>(xfj.x, xfj.y, xfj.z), mk, pj.w, rho_k,
-                         mk / fmaxf(kEps, rho_k), f, vw);
-            float g[3];
-            contract3(f, fk, g);
-#pragma unroll
-            for (int q = 0; q < 3; ++q) g[q] += ok.w * vw.d[q];
-            add3(gx, -g[0], -g[1], -g[2]);
+            v3 gd = mk3(0.f, 0.f, 0.f), ga_unused = gd;
+            float gr_unused = 0.f, ge_unused = 0.f, gp_unused = 0.f;
+            vjp_sand_water(k, mk3(dx, dy, dz), rpj.x, t.es[j].x, rpj.y, xyz(t.xf[j]), mk, pj.w, rho_k,
+                           mk / fmaxf(kEps, rho_k), fk3, ok.w, gd, gr_unused, ge_unused, gp_unused, ga_unused);
+            add3(gx, -gd.x, -gd.y, -gd.z);
         } else if (!sand && pj.w < 0.f) {  // j-side: water k acts on sand j
-            using D9 = Dn<9>;
             const float4 oj = a.o4[j];
             const float mi = -pj.w;
             const float rho_i = t.rp[j].x;
-            const float4 xfk = t.xf[i];
-            const float2 esk = t.es[i];
             const float2 rpk = t.rp[i];
-            G3<D9> d = {dvar<9>(-dx, 0), dvar<9>(-dy, 1), dvar<9>(-dz, 2)};  // x_k - x_j
-            G3<D9> ad = {dvar<9>(xfk.x, 6), dvar<9>(xfk.y, 7), dvar<9>(xfk.z, 8)};
-            G3<D9> f;
-            D9 vw;
-            t_sand_water(k, d, dvar<9>(rpk.x, 3), dvar<9>(esk.x, 4), dvar<9>(rpk.y, 5), ad, mi, mk, rho_i,
-                         mi / fmaxf(kEps, rho_i), f, vw);
-            float g[9];
-            contract3(f, make_float3(oj.x, oj.y, oj.z), g);
-#pragma unroll
-            for (int q = 0; q < 9; ++q) g[q] += oj.w * vw.d[q];
-            add3(gx, g[0], g[1], g[2]);
-            grho += g[3];
-            geps += g[4];
-            gp += g[5];
-            add3(gadh, g[6], g[7], g[8]);
+            v3 gd = mk3(0.f, 0.f, 0.f), ga = gd;
+            vjp_sand_water(k, mk3(-dx, -dy, -dz), rpk.x, t.es[i].x, rpk.y, xyz(t.xf[i]), mi, mk, rho_i,
+                           mi / fmaxf(kEps, rho_i), mk3(oj.x, oj.y, oj.z), oj.w, gd, grho, geps, gp, ga);
+            add3(gx, gd);
+            add3(gadh, ga);
         }
     });
-    a.gx[i] = gx;
-    a.gv[i] = gv;
-    a.gth[i] = gth;
+    gx = lsum<L>(gx);
+    gv = lsum<L>(gv);
+    gth = lsum<L>(gth);
+    gadh = lsum<L>(gadh);
+    gsat = lsum<L>(gsat);
+    grho = lsum<L>(grho);
+    geps = lsum<L>(geps);
+    gp = lsum<L>(gp);
+    if (sub != 0) return;
+    a.gx[i] = a.gx[i] + gx;
+    a.gv[i] = a.gv[i] + gv;
+    a.gth[i] = a.gth[i] + gth;
     if (sand) {
         a.ges[i].y += gsat;
     } else {
@@ -444,19 +437,22 @@ __global__ __launch_bounds__(kAB) void k_adj_voidage_local(SFConst k, TapeStep t
 }
 
 // ---- A3 (pairs) ---------------------------------------------------------------
+template <int L>
 __global__ __launch_bounds__(kAB) void k_adj_voidage_pairs(SFConst k, TapeStep t, AdjFields a, int n) {
-    const int i = blockIdx.x * kAB + threadIdx.x;
+    const int tid = blockIdx.x * kAB + threadIdx.x;
+    const int i = tid / L, sub = tid % L;
     if (i >= n) return;
     const float4 pk = t.pm[i];
     const bool sand = pk.w < 0.f;
     const float mk = fabsf(pk.w);
-    float4 gx = a.gx[i];
+    float4 gx = make_float4(0.f, 0.f, 0.f, 0.f);
     float gsat = 0.f, geps = 0.f, grho = 0.f;
     const float2 o2k = a.o2[i];
     const float4 o4k = a.o4[i];
     const float rho_k = t.rp[i].x;
     const float eps_k = t.es[i].x;
-    neighbours(k, t, pk, [&](uint32_t j) {
+    const float sat_k = t.es[i].y;
+    neighbours<L>(k, t, pk, sub, sand ? kNeedWater : kNeedSand, [&](uint32_t j) {
         const float4 pj = t.pm[j];
         if ((pj.w < 0.f) == sand) return;
         const float dx = pk.x - pj.x, dy = pk.y - pj.y, dz = pk.z - pj.z;  // x_k - x_j
@@ -472,7 +468,7 @@ __global__ __launch_bounds__(kAB) void k_adj_voidage_pairs(SFConst k, TapeStep t
             G3<D4> dp = {dvar<4>(-dx, 0), dvar<4>(-dy, 1), dvar<4>(-dz, 2)};
             G3<D4> F;
             D4 vs;
-            t_adhesion(k, dp, dvar<4>(t.es[i].y, 3), pj.w, mk, F, vs);
+            t_adhesion(k, dp, dvar<4>(sat_k, 3), pj.w, mk, F, vs);
             float g[4];
             contract3(F, make_float3(oj.x, oj.y, oj.z), g);
 #pragma unroll
@@ -501,7 +497,12 @@ __global__ __launch_bounds__(kAB) void k_adj_voidage_pairs(SFConst k, TapeStep t
             add3(gx, dpoly6(k, d, r2) * (c * Vk));
         }
     });
-    a.gx[i] = gx;
+    gx = lsum<L>(gx);
+    gsat = lsum<L>(gsat);
+    geps = lsum<L>(geps);
+    grho = lsum<L>(grho);
+    if (sub != 0) return;
+    a.gx[i] = a.gx[i] + gx;
     if (sand) {
         a.ges[i].y += gsat;
     } else {
@@ -552,20 +553,22 @@ __global__ __launch_bounds__(kAB) void k_adj_flow_local(SFConst k, TapeStep t, A
 }
 
 // ---- A2 (pairs) ---------------------------------------------------------------
+template <int L>
 __global__ __launch_bounds__(kAB) void k_adj_flow_pairs(SFConst k, TapeStep t, AdjFields a, int n) {
-    const int i = blockIdx.x * kAB + threadIdx.x;
+    const int tid = blockIdx.x * kAB + threadIdx.x;
+    const int i = tid / L, sub = tid % L;
     if (i >= n) return;
     const float4 pk = t.pm[i];
     const bool sand = pk.w < 0.f;
     const float mk = fabsf(pk.w);
-    float4 gx = a.gx[i], gv = a.gv[i];
+    float4 gx = make_float4(0.f, 0.f, 0.f, 0.f), gv = gx;
     float grho = 0.f;
     const float2 o2k = a.o2[i];
     const float4 o4k = a.o4[i];
     const float rho_k = t.rp[i].x;
     const float4 vk = t.vr[i];
     const float inv_rho_s = k.rho_s != 0.f ? 1.f / k.rho_s : 0.f;
-    neighbours(k, t, pk, [&](uint32_t j) {
+    neighbours<L>(k, t, pk, sub, sand ? kNeedWater : kNeedSand, [&](uint32_t j) {
         const float4 pj = t.pm[j];
         if ((pj.w < 0.f) == sand) return;
         const float dx = pk.x - pj.x, dy = pk.y - pj.y, dz = pk.z - pj.z;  // x_k - x_j
@@ -589,8 +592,12 @@ __global__ __launch_bounds__(kAB) void k_adj_flow_pairs(SFConst k, TapeStep t, A
             add3(gx, dw * (a.o2[j].x * (mk * inv_rho_s)));  // j-side: sandv_j
         }
     });
-    a.gx[i] = gx;
-    a.gv[i] = gv;
+    gx = lsum<L>(gx);
+    gv = lsum<L>(gv);
+    grho = lsum<L>(grho);
+    if (sub != 0) return;
+    a.gx[i] = a.gx[i] + gx;
+    a.gv[i] = a.gv[i] + gv;
     if (!sand) a.grp[i].x += grho;
 }
 
@@ -615,14 +622,16 @@ __global__ __launch_bounds__(kAB) void k_adj_density_local(SFConst k, TapeStep t
     const float2 g = a.grp[i];
     a.grp[i] = make_float2(g.x + g.y * dp, 0.f);
 }
+template <int L>
 __global__ __launch_bounds__(kAB) void k_adj_density_pairs(SFConst k, TapeStep t, AdjFields a, int n) {
-    const int i = blockIdx.x * kAB + threadIdx.x;
+    const int tid = blockIdx.x * kAB + threadIdx.x;
+    const int i = tid / L, sub = tid % L;
     if (i >= n) return;
     const float4 pk = t.pm[i];
     if (pk.w < 0.f) return;
     const float rb = a.grp[i].x;
-    float4 gx = a.gx[i];
-    neighbours(k, t, pk, [&](uint32_t j) {
+    float4 gx = make_float4(0.f, 0.f, 0.f, 0.f);
+    neighbours<L>(k, t, pk, sub, kNeedWater, [&](uint32_t j) {
         const float4 pj = t.pm[j];
         if (pj.w < 0.f || (int)j == i) return;
         const float dx = pk.x - pj.x, dy = pk.y - pj.y, dz = pk.z - pj.z;
@@ -631,14 +640,16 @@ __global__ __launch_bounds__(kAB) void k_adj_density_pairs(SFConst k, TapeStep t
         // i-side (k's density over water j) + j-side (j's density over k)
         add3(gx, dpoly6(k, mk3(dx, dy, dz), r2) * (rb * pj.w + a.grp[j].x * pk.w));
     });
-    boundary_neighbours(k, t, pk, [&](uint32_t b) {
+    boundary_neighbours<L>(k, t, pk, sub, [&](uint32_t b) {
         const float4 bp = t.bpv[b];
         const float dx = pk.x - bp.x, dy = pk.y - bp.y, dz = pk.z - bp.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
         if (r2 >= k.h2) return;
         add3(gx, dpoly6(k, mk3(dx, dy, dz), r2) * (rb * k.rho0 * fabsf(bp.w)));
     });
-    a.gx[i] = gx;
+    gx = lsum<L>(gx);
+    if (sub != 0) return;
+    a.gx[i] = a.gx[i] + gx;
 }
 
 __global__ __launch_bounds__(kAB) void k_adj_scatter(const uint32_t* __restrict__ perm, const float4* __restrict__ gx,
@@ -703,20 +714,39 @@ inline int blocks(int n) { return (n + kAB - 1) / kAB; }
 
 }  // namespace
 
+namespace {
+template <int L>
+void launch_pairs_step(const SFConst& k, const TapeStep& t, const AdjFields& a, const float* sat_seed, float dt, int n,
+                       hipStream_t s) {
+    const int g = blocks(n), gl = blocks(n * L);
+    k_adj_advect<<<g, kAB, 0, s>>>(k, t, a, sat_seed, dt, n);
+    k_adj_water<L><<<gl, kAB, 0, s>>>(k, t, a, n);
+    k_adj_sand_local<<<g, kAB, 0, s>>>(k, t, a, n);
+    k_adj_sand_pairs<L><<<gl, kAB, 0, s>>>(k, t, a, n);
+    k_adj_voidage_local<<<g, kAB, 0, s>>>(k, t, a, n);
+    k_adj_voidage_pairs<L><<<gl, kAB, 0, s>>>(k, t, a, n);
+    k_adj_flow_local<<<g, kAB, 0, s>>>(k, t, a, n);
+    k_adj_flow_pairs<L><<<gl, kAB, 0, s>>>(k, t, a, n);
+    k_adj_density_local<<<g, kAB, 0, s>>>(k, t, a, n);
+    k_adj_density_pairs<L><<<gl, kAB, 0, s>>>(k, t, a, n);
+}
+}  // namespace
+
+int adj_lanes(int n) {
+    // about 8 waves per SIMD (256 CUs x 4 SIMDs) or more
+    const long target = 8L * 1024 * 64;
+    return (long)n * 4 <= target ? 8 : (long)n * 2 <= target ? 4 : (long)n <= target ? 2 : 1;
+}
+
 void launch_adj_step(const SFConst& k, const TapeStep& t, const AdjFields& a, const float* sat_seed, float dt, int n,
                      hipStream_t s) {
     if (n <= 0) return;
-    const int g = blocks(n);
-    k_adj_advect<<<g, kAB, 0, s>>>(k, t, a, sat_seed, dt, n);
-    k_adj_water<<<g, kAB, 0, s>>>(k, t, a, n);
-    k_adj_sand_local<<<g, kAB, 0, s>>>(k, t, a, n);
-    k_adj_sand_pairs<<<g, kAB, 0, s>>>(k, t, a, n);
-    k_adj_voidage_local<<<g, kAB, 0, s>>>(k, t, a, n);
-    k_adj_voidage_pairs<<<g, kAB, 0, s>>>(k, t, a, n);
-    k_adj_flow_local<<<g, kAB, 0, s>>>(k, t, a, n);
-    k_adj_flow_pairs<<<g, kAB, 0, s>>>(k, t, a, n);
-    k_adj_density_local<<<g, kAB, 0, s>>>(k, t, a, n);
-    k_adj_density_pairs<<<g, kAB, 0, s>>>(k, t, a, n);
+    switch (adj_lanes(n)) {
+    case 8: launch_pairs_step<8>(k, t, a, sat_seed, dt, n, s); break;
+    case 4: launch_pairs_step<4>(k, t, a, sat_seed, dt, n, s); break;
+    case 2: launch_pairs_step<2>(k, t, a, sat_seed, dt, n, s); break;
+    default: launch_pairs_step<1>(k, t, a, sat_seed, dt, n, s); break;
+    }
 }
 void launch_adj_scatter(const uint32_t* perm, const float4* gx, const float4* gv, float4* gx_prev, float4* gv_prev,
                         int n, hipStream_t s) {

@@ -49,6 +49,7 @@ static SFConst make_const(const sf_params& P, const sf_boundary_params& B, const
         k.g_coef = -45.f / (PI * h6);
     }
     k.adh_coef = 0.007f / powf(h, 3.25f);  // AdhesionAkinci13 (:192-195)
+    k.ih = 1.f / h;
     k.rho0 = P.sph_density;
     k.stiff = P.sph_stiff;
     k.tait = P.solver_type == SF_WCSPH_SOLVER ? 1 : 0;
@@ -58,6 +59,7 @@ static SFConst make_const(const sf_params& P, const sf_boundary_params& B, const
     k.rw = P.sph_particle_radius;
     k.rho_s = P.dem_density;
     k.R = P.dem_particle_radius;
+    k.iR = 1.f / k.R;
     k.young = P.dem_young;
     k.poisson = P.dem_poisson;
     k.kn = P.dem_young * k.R;
@@ -87,6 +89,8 @@ static SFConst make_const(const sf_params& P, const sf_boundary_params& B, const
         g.c = g.b * g.b;
         g.d = -4.f * (1.f + g.b - px1);
         g.e = 2.f * (1.f + g.b - px1);
+        g.ipx1 = 1.f / px1;
+        g.ie = 1.f / g.e;
         return g;
     };
     k.gcap = bez(P.sf_c0, P.sf_cmc, P.sf_cmc_p, P.sf_csat);  // cuda_sph_sf_solver.cu:249,308

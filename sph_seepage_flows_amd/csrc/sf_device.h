@@ -56,6 +56,7 @@ SF_DEV float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 // cuda_sf_utils.cuh:36-57, cuda_dem_solver_common_gpu.cuh:63-65).
 struct Bez {
     float py0, py1, px1, py2, b, c, d, e;
+    float ipx1, ie;  // 1 / px1, 1 / e
 };
 struct SFConst {
     // grid (CudaBoundaryParams)
@@ -64,11 +65,11 @@ struct SFConst {
     int gx, gy, gz;
     uint32_t C;  // gx*gy*gz
     // SPH kernels
-    float w_coef, w0, g_coef, adh_coef;
+    float w_coef, w0, g_coef, adh_coef, ih;  // ih = 1 / h
     // water
     float rho0, stiff, neg_scale, nu, bnu, rw;
     // sand
-    float rho_s, R, kn, ks, tan2, tanphi, damping, young, poisson, cd;
+    float rho_s, R, iR, kn, ks, tan2, tanphi, damping, young, poisson, cd;
     // capillary bridge
     float cap_vb, cap_srup, cap_vb_piR, cap_cos, cap_sin, cap_pi;
     Bez gcap, gadh;
@@ -87,6 +88,7 @@ struct SFConst {
 struct SceneConst {
     float tan2, tanphi, cd, pad;
     Bez gcap;
+    float pad2[2];
 };
 
 // ---- grid hash (helper_thrust.cuh:56-63), FP32 and exactly as written -----
@@ -136,13 +138,14 @@ SF_DEV float poly6_r2(const SFConst& k, float r2) {
 SF_DEV float spiky_s(const SFConst& k, float r) {
     if (r >= k.h || r < kEps) return 0.f;
     float d = k.h - r;
-    return k.g_coef * d * d / r;
+    return k.g_coef * d * d * frcp(r);
 }
 // AdhesionAkinci13 (:189-203).  The base -4r^2/h + 6r - 2h vanishes at r = h
 // and can round to a tiny negative there (pow would give NaN); it is clamped
-// at 0, which changes nothing where the reference's value is finite.
+// at 0, which changes nothing where the reference's value is finite.  The
+// quarter power is two hardware square roots.
 SF_DEV float adhesion_A(const SFConst& k, float r) {
-    if (2.f * r > k.h && r <= k.h) return k.adh_coef * powf(fmaxf(-4.f * r * r / k.h + 6.f * r - 2.f * k.h, 0.f), 0.25f);
+    if (2.f * r > k.h && r <= k.h) return k.adh_coef * fsqrt(fsqrt(fmaxf(-4.f * r * r * k.ih + 6.f * r - 2.f * k.h, 0.f)));
     return 0.f;
 }
 // QuadraticBezierCoeff (cuda_sf_utils.cuh:36-82)
@@ -150,10 +153,10 @@ SF_DEV float bezier(const Bez& g, float sr) {
     if (sr < 0.f) return g.py0;
     if (sr >= 1.f) return g.py2;
     if (sr <= g.px1) {
-        float t = sr / g.px1, omt = 1.f - t;
+        float t = sr * g.ipx1, omt = 1.f - t;
         return omt * omt * g.py0 + 2.f * t * omt * g.py1 + t * t * g.py1;
     }
-    float t = (g.b + sqrtf(g.c + g.d * (g.px1 - sr))) / g.e;
+    float t = (g.b + fsqrt(g.c + g.d * (g.px1 - sr))) * g.ie;
     float omt = 1.f - t;
     return omt * omt * g.py1 + 2.f * t * omt * g.py1 + t * t * g.py2;
 }
@@ -162,29 +165,29 @@ SF_DEV float bezier(const Bez& g, float sr) {
 // overlap (dist < rij); dist = |dij| > 0.
 SF_DEV v3 dem_contact(v3 dij, float dist, v3 vij, float rij, float kn, float ks, float tan2) {
     float pen = rij - dist;
-    v3 n = dij * (1.f / dist);
+    v3 n = dij * frcp(dist);
     float de = dot(vij, n);
     v3 vt = vij - de * n;
     v3 fn = (kn * pen) * n;
     v3 fs = (-ks) * vt;
     float maxfs = dot(fn, fn) * tan2;
     float fs2 = dot(fs, fs);
-    if (fs2 > maxfs) fs = fs * (sqrtf(maxfs) / sqrtf(fs2));
+    if (fs2 > maxfs) fs = fs * (fsqrt(maxfs) * frsq(fs2));
     return -fn - fs;
 }
 SF_DEV v3 dem_force(v3 dij, v3 vij, float rij, float kn, float ks, float tan2) {
-    float dist = sqrtf(dot(dij, dij));
+    float dist = fsqrt(dot(dij, dij));
     if (rij - dist > 0.f) return dem_contact(dij, dist, vij, rij, kn, ks, tan2);
     return mk3(0.f, 0.f, 0.f);
 }
 // ComputeDemCapillaryForces (:53-90) inside the window 0 < H < s_rupture
 SF_DEV v3 capillary(const SFConst& k, v3 dij, float dist, float H, float sr) {
-    v3 N = dij * (1.f / dist);
+    v3 N = dij * frcp(dist);
     float cc = bezier(k.gcap, sr);
     float R = k.R;
-    float d = -H + sqrtf(H * H + k.cap_vb_piR);
-    float phi = sqrtf(2.f * H / R * (-1.f + sqrtf(1.f + k.cap_vb_piR / (H * H))));
-    float neck = -2.f * k.cap_pi * cc * R * k.cap_cos / (1.f + H / (2.f * d));
+    float d = -H + fsqrt(H * H + k.cap_vb_piR);
+    float phi = fsqrt(2.f * H * k.iR * (-1.f + fsqrt(1.f + k.cap_vb_piR * frcp(H * H))));
+    float neck = -2.f * k.cap_pi * cc * R * k.cap_cos * frcp(1.f + H * frcp(2.f * d));
     float surf = -2.f * k.cap_pi * cc * R * phi * k.cap_sin;
     return -N * (neck + surf);
 }

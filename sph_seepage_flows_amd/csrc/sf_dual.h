@@ -255,4 +255,73 @@ SF_DEV void t_adhesion(const SFConst& k, const G3<S>& dp, const S& sat_j, float 
     }
 }
 
+// ---- hand-derived vector-Jacobian products of the two hottest mixed-phase terms
+
+// d gs / d r for the Spiky scalar gs(r) = g (h - r)^2 / r (0 outside [eps, h))
+SF_DEV float spiky_ds(const SFConst& k, float r) {
+    if (r >= k.h || r < kEps) return 0.f;
+    const float q = k.h - r, ir = 1.f / r;
+    return k.g_coef * (-2.f * q * ir - q * q * ir * ir);
+}
+
+// t_sand_water (sand i <- water j, d = x_j - x_i) contracted with (fb, vwb):
+// adds the adjoints of d, rho_j, eps_j, p_j and adh_j.
+SF_DEV void vjp_sand_water(const SFConst& k, v3 d, float rho_j, float eps_j, float p_j, v3 adh_j, float m_i, float m_j,
+                           float rho_i, float vol_i, v3 fb, float vwb, v3& gd, float& grho, float& geps, float& gp,
+                           v3& gadh) {
+    const float r2 = dot(d, d);
+    if (r2 >= k.h2) return;
+    const float q2 = k.h2 - r2;
+    const float w = k.w_coef * q2 * q2 * q2;
+    const v3 dw = d * (-6.f * k.w_coef * q2 * q2);  // d w / d d
+    const float fd = dot(fb, d);
+    if (eps_j > 0.f && eps_j <= 1.f) {
+        const float ep = fmaxf(eps_j, kEps);
+        const float dj = rho_j / ep;
+        const float X = rho_i * dj, Xc = fmaxf(X, kEps);
+        const float B = m_i * m_j * p_j / Xc;
+        const float r = sqrtf(r2);
+        const float gs = spiky_s(k, r), gsd = spiky_ds(k, r);
+        gd += fb * (B * gs);
+        if (r > 0.f) gd += d * (fd * B * gsd / r);
+        const float Bb = fd * gs;
+        gp += Bb * m_i * m_j / Xc;
+        float djb = X > kEps ? -Bb * B / X * rho_i : 0.f;
+        const float vw = m_j / dj * w;
+        gd += dw * (vwb * m_j / dj);
+        djb += -vwb * vw / dj;
+        grho += djb / ep;
+        if (eps_j > kEps) geps += djb * (-rho_j / (ep * ep));
+    }
+    gadh += fb * (w * vol_i);
+    gd += dw * (dot(fb, adh_j) * vol_i);
+}
+
+// t_water_sand (water i <- sand j, d = x_i - x_j) contracted with ab: adds
+// the adjoints of d, p_i, rt_i and drag_j.
+SF_DEV void vjp_water_sand(const SFConst& k, v3 d, float p_i, float rt_i, v3 drag_j, float m_j, float rho_j, v3 ab,
+                           v3& gd, float& gp, float& grt, v3& gdrag) {
+    const float r2 = dot(d, d);
+    const float r = sqrtf(r2);
+    const float gs = spiky_s(k, r), gsd = spiky_ds(k, r);
+    const float Y = rt_i * rho_j, Yc = fmaxf(Y, kEps);
+    const float C = -m_j / Yc;
+    const float q = dot(ab, d);
+    const float ad = dot(ab, drag_j);
+    float w = 0.f;
+    v3 dw = mk3(0.f, 0.f, 0.f);
+    if (r2 < k.h2) {
+        const float q2 = k.h2 - r2;
+        w = k.w_coef * q2 * q2 * q2;
+        dw = d * (-6.f * k.w_coef * q2 * q2);
+    }
+    gd += ab * (C * p_i * gs);
+    if (r > 0.f) gd += d * (q * C * p_i * gsd / r);
+    gd += dw * (-ad / rt_i);
+    gp += q * C * gs;
+    if (Y > kEps) grt += q * p_i * gs * (m_j / (rt_i * rt_i * rho_j));
+    grt += ad * w / (rt_i * rt_i);
+    gdrag += ab * (-w / rt_i);
+}
+
 }  // namespace sf

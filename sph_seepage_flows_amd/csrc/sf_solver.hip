@@ -190,7 +190,7 @@ __global__ __launch_bounds__(NB) void k_avg_flow(SFConst k, Fields f, int n) {
             const float r2 = dx * dx + dy * dy + dz * dz;
             if (r2 >= k.h2 || (pj.w < 0.f) == sand_i) continue;
             const float w = k.w_coef * sq3(k.h2 - r2);
-            const float vw = (pj.w < 0.f ? -pj.w * inv_rho_s : pj.w / ldo(f.rp, j).x) * w;
+            const float vw = (pj.w < 0.f ? -pj.w * inv_rho_s : pj.w * frcp(ldo(f.rp, j).x)) * w;
             if (!sand_i) sandv += vw;
             av += xyz(ldo(f.vr, j)) * vw;
             vsum += vw;
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(NB) void k_voidage_adhesion(SFConst k, Fields f, in
                 const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                 const float r2 = dx * dx + dy * dy + dz * dz;
                 if (pj.w < 0.f || r2 >= k.h2) continue;
-                const float vw = (pj.w / ldo(f.rp, j).x) * (k.w_coef * sq3(k.h2 - r2));
+                const float vw = (pj.w * frcp(ldo(f.rp, j).x)) * (k.w_coef * sq3(k.h2 - r2));
                 num += ldo(f.es, j).x * vw;
                 den += vw;
             }
@@ -257,11 +257,11 @@ __global__ __launch_bounds__(NB) void k_voidage_adhesion(SFConst k, Fields f, in
             if (pj.w >= 0.f || r2 >= k.h2) continue;
             const float mj = -pj.w;
             vs += (mj * inv_rho_s) * (k.w_coef * sq3(k.h2 - r2));
-            const float l = sqrtf(r2);
+            const float l = fsqrt(r2);
             const float A = adhesion_A(k, l);
             if (A != 0.f) {
                 const float G = bezier(k.gadh, ldo(f.es, j).y);
-                F += dp * (G * mi * mj * A / fmaxf(kEps, l));
+                F += dp * (G * mi * mj * A * frcp(fmaxf(kEps, l)));
             }
         }
     });
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
     auto contact = [&](uint32_t j) {
         const float4 pj = ldo(f.pm, j);
         const v3 dji = mk3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-        const float dist = sqrtf(dot(dji, dji));
+        const float dist = fsqrt(dot(dji, dji));
         if (dist < twoR) {
             fsum += dem_contact(dji, dist, xyz(ldo(f.vr, j)) - vi, twoR, k.kn, k.ks, k.tan2);
         } else {
@@ -361,11 +361,11 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
         if (eps_j > 0.f && eps_j <= 1.f) {
             const float2 rpj = ldo(f.rp, j);
             const float mj = pj.w;
-            const float dj = rpj.x / fmaxf(kEps, eps_j);
-            vw += mj / dj * w;
-            const float gs = spiky_s(k, sqrtf(r2));
+            const float dj = rpj.x * frcp(fmaxf(kEps, eps_j));
+            vw += mj * frcp(dj) * w;
+            const float gs = spiky_s(k, fsqrt(r2));
             // gradW(x_i - x_j) = gs * (x_i - x_j) = -gs * dji
-            fsum += dji * (mi * mj * rpj.y / fmaxf(kEps, rho_i * dj) * gs);
+            fsum += dji * (mi * mj * rpj.y * frcp(fmaxf(kEps, rho_i * dj)) * gs);
         }
         fsum += xyz(ldo(f.xf, j)) * (w * vol_i);
     };
