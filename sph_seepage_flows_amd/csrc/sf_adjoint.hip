@@ -459,33 +459,20 @@ __global__ __launch_bounds__(kAB) void k_adj_voidage_pairs(SFConst k, TapeStep t
         const float r2 = dx * dx + dy * dy + dz * dz;
         if (r2 >= k.h2) return;
         const v3 d = mk3(dx, dy, dz);
-        using D4 = Dn<4>;
         if (sand) {  // j is water
             const float Vj = pj.w / t.rp[j].x;
             add3(gx, dpoly6(k, d, r2) * ((o2k.x * t.es[j].x + o2k.y) * Vj));  // i-side: sand voidage sums
             // j-side: F_j and vs_j of water j with sand k as neighbour; dp = x_j - x_k
             const float4 oj = a.o4[j];
-            G3<D4> dp = {dvar<4>(-dx, 0), dvar<4>(-dy, 1), dvar<4>(-dz, 2)};
-            G3<D4> F;
-            D4 vs;
-            t_adhesion(k, dp, dvar<4>(sat_k, 3), pj.w, mk, F, vs);
-            float g[4];
-            contract3(F, make_float3(oj.x, oj.y, oj.z), g);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) g[q] += oj.w * vs.d[q];
-            add3(gx, -g[0], -g[1], -g[2]);
-            gsat += g[3];
+            v3 gd = mk3(0.f, 0.f, 0.f);
+            vjp_adhesion(k, mk3(-dx, -dy, -dz), sat_k, pj.w, mk, mk3(oj.x, oj.y, oj.z), oj.w, gd, gsat);
+            add3(gx, -gd.x, -gd.y, -gd.z);
         } else {  // k water, j sand
             {  // i-side: F_k and vs_k
-                G3<D4> dp = {dvar<4>(dx, 0), dvar<4>(dy, 1), dvar<4>(dz, 2)};
-                G3<D4> F;
-                D4 vs;
-                t_adhesion(k, dp, D4(t.es[j].y), mk, -pj.w, F, vs);
-                float g[4];
-                contract3(F, make_float3(o4k.x, o4k.y, o4k.z), g);
-#pragma unroll
-                for (int q = 0; q < 3; ++q) g[q] += o4k.w * vs.d[q];
-                add3(gx, g[0], g[1], g[2]);
+                v3 gd = mk3(0.f, 0.f, 0.f);
+                float gsat_unused = 0.f;
+                vjp_adhesion(k, d, t.es[j].y, mk, -pj.w, mk3(o4k.x, o4k.y, o4k.z), o4k.w, gd, gsat_unused);
+                add3(gx, gd);
             }
             // j-side: sand j's voidage num += eps_k V_k W, den += V_k W
             const float2 oj = a.o2[j];

@@ -36,6 +36,7 @@ struct DiffState {
     int steps = 0, every = 0;
     float dt = 0.f;
     bool have_forward = false;
+    bool last_taped = false;  // the forward already taped the last segment
     DevBuf<uint32_t> id, id_tmp;
     std::vector<StateCopy> ck;  // state before steps 0, K, 2K, ...
     StateCopy fin;              // state after the window
@@ -194,11 +195,16 @@ int sf_diff_forward(sf_handle h, float dt, int n_steps, int checkpoint_every) {
     d->steps = n_steps;
     d->every = checkpoint_every;
     d->dt = dt;
+    // the last segment is taped right away, so the backward starts without recomputing it
+    if ((rc = alloc_tape(h, checkpoint_every))) return rc;
+    const int last0 = (nck - 1) * checkpoint_every;
+    d->last_taped = false;
     launch_iota(d->id.p, (int)h->n, h->stream);
     for (int q = 0; q < n_steps; ++q) {
         if (q % checkpoint_every == 0 && (rc = save_state(h, d->ck[q / checkpoint_every]))) return rc;
-        if ((rc = diff_step(h, dt, nullptr))) return rc;
+        if ((rc = diff_step(h, dt, q >= last0 ? &d->tape[q - last0] : nullptr))) return rc;
     }
+    d->last_taped = true;
     if ((rc = save_state(h, d->fin)) || (rc = copy(d->fin_rp.p, h->rp.p, h->n * sizeof(float2), h->stream)) ||
         (rc = copy(d->fin_es.p, h->es.p, h->n * sizeof(float2), h->stream)))
         return rc;
@@ -281,9 +287,11 @@ int sf_diff_backward(sf_handle h, const float* pos_bar3, const float* vel_bar3, 
     const int nck = (d->steps + d->every - 1) / d->every;
     for (int c = nck - 1; c >= 0; --c) {
         const int s0 = c * d->every, s1 = std::min(d->steps, s0 + d->every);
-        if ((rc = load_state(h, d->ck[c]))) return rc;
-        for (int q = s0; q < s1; ++q)
-            if ((rc = diff_step(h, d->dt, &d->tape[q - s0]))) return rc;
+        if (c < nck - 1 || !d->last_taped) {
+            if ((rc = load_state(h, d->ck[c]))) return rc;
+            for (int q = s0; q < s1; ++q)
+                if ((rc = diff_step(h, d->dt, &d->tape[q - s0]))) return rc;
+        }
         for (int q = s1 - 1; q >= s0; --q) {
             const StepTapeBufs& T = d->tape[q - s0];
             TapeStep t{T.pm.p, T.vr.p, T.rp.p, T.es.p, T.wt.p, T.flow.p, T.adh.p, T.xf.p, T.acc.p, T.den4.p,
@@ -323,6 +331,7 @@ int sf_diff_backward(sf_handle h, const float* pos_bar3, const float* vel_bar3, 
         }
         for (int q = 0; q < 4; ++q) param_bar4[q] = (float)acc[q];
     }
+    d->last_taped = false;  // the tape slots now hold the first segment
     // leave the handle at the end of the window, as after sf_diff_forward
     if ((rc = load_state(h, d->fin)) || (rc = copy(h->rp.p, d->fin_rp.p, (size_t)n * sizeof(float2), s)) ||
         (rc = copy(h->es.p, d->fin_es.p, (size_t)n * sizeof(float2), s)))

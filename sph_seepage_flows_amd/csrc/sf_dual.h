@@ -324,4 +324,28 @@ SF_DEV void vjp_water_sand(const SFConst& k, v3 d, float p_i, float rt_i, v3 dra
     gdrag += ab * (-w / rt_i);
 }
 
+// t_adhesion (water i <- sand j, dp = x_i - x_j) contracted with (Fb, vsb):
+// adds the adjoints of dp and sat_j.
+SF_DEV void vjp_adhesion(const SFConst& k, v3 dp, float sat_j, float m_i, float m_j, v3 Fb, float vsb, v3& gdp,
+                         float& gsat) {
+    const float l2 = dot(dp, dp);
+    if (l2 >= k.h2) return;
+    const float q2 = k.h2 - l2;
+    gdp += dp * (vsb * (m_j / k.rho_s) * (-6.f * k.w_coef * q2 * q2));
+    const float l = sqrtf(l2);
+    if (!(2.f * l > k.h && l <= k.h)) return;
+    const float base = -4.f * l * l / k.h + 6.f * l - 2.f * k.h;
+    if (!(base > 0.f)) return;
+    const float A = k.adh_coef * powf(base, 0.25f);
+    const float dA = k.adh_coef * 0.25f * powf(base, -0.75f) * (-8.f * l / k.h + 6.f);
+    const Dn<1> G = g_bezier(k.gadh, Dn<1>(k.gadh.py0), Dn<1>(k.gadh.py1), dvar<1>(sat_j, 0));
+    const float L = fmaxf(l, kEps), mm = m_i * m_j;
+    const float phi = G.v * mm * A / L;
+    const float dphi = G.v * mm * (dA / L - (l > kEps ? A / (L * L) : 0.f));  // d phi / d l
+    const float q = dot(Fb, dp);
+    gdp += Fb * phi;
+    if (l > 0.f) gdp += dp * (q * dphi / l);
+    gsat += q * G.d[0] * mm * A / L;
+}
+
 }  // namespace sf
