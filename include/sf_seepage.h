@@ -27,7 +27,8 @@
  *   sf_write_bgeo        <- Partio::writeBGEO (uncompressed V5)  extlib/include/partio/io/BGEO.cpp:311-476
  *   sf_build_world_boundary <- CudaBoundaryEmitter::BuildWorldBoundary  cuda_boundary_emitter.cpp:12-80
  *   sf_grid_size         <- make_int3((highest - lowest) / kernel_radius)  main.cpp:104, cuda_neighbor_searcher.cu:26
- *   sf_batch_*, sf_step_graph, sf_*_launch_config, sf_autotune, sf_set_field, sf_diff_*:
+ *   sf_batch_*, sf_step_graph, sf_*_launch_config, sf_autotune, sf_set_field, sf_diff_*,
+ *   sf_set_sort_strategy:
  *                           no counterpart (see their declarations)
  *
  * Threading: a handle is single-threaded; all its GPU work is ordered on the
@@ -260,6 +261,14 @@ int sf_batch_scene_offsets(sf_handle h, size_t* n_scenes, size_t* offsets, size_
  * HIP graph once (again after any change of dt, parameters, uploads or launch
  * shapes) and replays it; results are identical to sf_step_n. */
 int sf_step_graph(sf_handle h, float dt, int n_steps);
+/* Sort of the neighbour build.  SF_SORT_INCREMENTAL (default) rebuilds the
+ * stable sort from the previous step's keys and cellStart when the particles
+ * are still in that step's order (most keep their cell); the first build
+ * after an upload always uses the LSD radix sort.  Both give the stable
+ * sort's permutation element for element. */
+#define SF_SORT_RADIX 0
+#define SF_SORT_INCREMENTAL 1
+int sf_set_sort_strategy(sf_handle h, int strategy);
 /* Launch shape of a neighbour pass (SF_PASS_DENSITY .. SF_PASS_WATER_MOMENTUM):
  * workgroup size nb in {64, 128, 256}; LDS candidate-list depth cap in {8, 16}
  * (sand / water momentum only).  Results do not depend on the shape. */

@@ -27,6 +27,9 @@ SF_WCSPH_SOLVER = 1
 SF_TYPE_SF = 0
 SF_TYPE_MULTI_SF = 1
 
+SF_SORT_RADIX = 0
+SF_SORT_INCREMENTAL = 1
+
 FIELDS = {
     "pos": (0, "f3"), "vel": (1, "f3"), "acc": (2, "f3"), "col": (3, "f3"), "label": (4, "u64"),
     "mass": (5, "f1"), "radius": (6, "f1"), "density": (7, "f1"), "pressure": (8, "f1"),
@@ -137,6 +140,7 @@ _SIG = {
     "sf_set_launch_config": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int]),
     "sf_get_launch_config": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "sf_autotune": (C.c_int, [_VP, C.c_int, _VP, _VP]),
+    "sf_set_sort_strategy": (C.c_int, [_VP, C.c_int]),
     "sf_diff_forward": (C.c_int, [_VP, C.c_float, C.c_int, C.c_int]),
     "sf_diff_reset": (C.c_int, [_VP, _VP, C.c_int]),
     "sf_diff_get": (C.c_int, [_VP, C.c_int, _VP, C.c_int]),

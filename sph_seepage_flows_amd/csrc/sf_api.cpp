@@ -128,7 +128,21 @@ int ensure_sort_capacity(sf_system* h, size_t m) {
     size_t nblocks = (c + kRsTile - 1) / kRsTile;
     if ((rc = h->hist.alloc(256 * nblocks))) return rc;
     if ((rc = h->wpre.alloc(c + 1)) || (rc = h->wpart.alloc((size_t)wprefix_blocks((int)c + 1)))) return rc;
+    if ((rc = h->skeys[0].alloc(c)) || (rc = h->skeys[1].alloc(c)) || (rc = h->sperm.alloc(c)) ||
+        (rc = h->slot.alloc(c)) || (rc = h->in_list.alloc(c)) || (rc = h->stay_pre.alloc(c + 1)))
+        return rc;
     h->sort_cap = c;
+    return ensure_cell_capacity(h);
+}
+
+// arrays of the incremental sort sized by the cell count (C + 1) and the
+// prefix partials for max(n, C) + 1 slots
+int ensure_cell_capacity(sf_system* h) {
+    const size_t C1 = (size_t)h->C + 1;
+    int rc;
+    if (h->in_count.n < C1 && ((rc = h->in_count.alloc(C1)) || (rc = h->in_start.alloc(C1)))) return rc;
+    const size_t np = (size_t)wprefix_blocks((int)std::max<size_t>(h->sort_cap, h->C) + 1);
+    if (h->part2.n < np && (rc = h->part2.alloc(np))) return rc;
     return SF_OK;
 }
 
@@ -149,8 +163,18 @@ int build_neighbors(sf_system* h, hipEvent_t* ev) {
     launch_hash(h->K, h->pm[h->cur].p, h->nscenes ? h->scene.p : nullptr, h->keys.p, n, s);
     SortBuffers sb{h->kA.p, h->vA.p, h->kB.p, h->vB.p, h->hist.p};
     SF_MARK(ev, SF_PASS_SORT);
-    int rc = radix_sort(h->keys.p, n, h->plan, sb, &h->sorted_keys, &h->perm, s);
-    if (rc) return rc;
+    if (h->sort_inc && h->built && h->sorted_keys) {
+        // the particles are in the last sort's order: rebuild the stable sort
+        // from the previous keys and cellStart
+        uint32_t* out = h->sorted_keys == h->skeys[0].p ? h->skeys[1].p : h->skeys[0].p;
+        IncSortBuffers ib{h->in_count.p, h->in_start.p, h->slot.p, h->in_list.p, h->stay_pre.p, h->part2.p};
+        launch_incremental_sort(h->keys.p, h->sorted_keys, h->cell.p, n, h->C, ib, h->sperm.p, out, s);
+        h->sorted_keys = out;
+        h->perm = h->sperm.p;
+    } else {
+        int rc = radix_sort(h->keys.p, n, h->plan, sb, &h->sorted_keys, &h->perm, s);
+        if (rc) return rc;
+    }
     GatherArgs g;
     g.perm = h->perm;
     g.pm_in = h->pm[h->cur].p;
@@ -829,7 +853,7 @@ int sf_batch_create(const sf_params* p, const sf_boundary_params* b, const sf_so
         return code;
     };
     if ((rc = h->cell.alloc((size_t)h->C + 1)) || (rc = h->ct.alloc((size_t)h->C + 1)) ||
-        (rc = h->scene.alloc(sid.size())) || (rc = h->sct.alloc(n_scenes)))
+        (rc = h->scene.alloc(sid.size())) || (rc = h->sct.alloc(n_scenes)) || (rc = ensure_cell_capacity(h)))
         return bail(rc);
     {
         hipError_t e;
@@ -911,6 +935,16 @@ int sf_set_launch_config(sf_handle h, int pass, int nb, int cap) {
         std::find(std::begin(kCapChoices), std::end(kCapChoices), cap) == std::end(kCapChoices))
         return set_error(SF_E_INVALID, "nb must be 64, 128 or 256 and cap 8 or 16");
     h->cfg[pass - SF_PASS_DENSITY] = PassCfg{nb, cap};
+    h->drop_graph();
+    return SF_OK;
+}
+
+int sf_set_sort_strategy(sf_handle h, int strategy) {
+    int rc;
+    if ((rc = check_handle(h))) return rc;
+    if (strategy != SF_SORT_RADIX && strategy != SF_SORT_INCREMENTAL)
+        return set_error(SF_E_INVALID, "strategy must be SF_SORT_RADIX or SF_SORT_INCREMENTAL");
+    h->sort_inc = strategy == SF_SORT_INCREMENTAL;
     h->drop_graph();
     return SF_OK;
 }

@@ -105,6 +105,19 @@ void launch_boundary_volume(const SFConst& k, float4* bpv, const uint32_t* bcs, 
 void launch_nbr_count(const SFConst& k, const float4* pm, const uint16_t* scene, const uint32_t* cs, uint32_t* cnt,
                       uint32_t* cand, int n, hipStream_t s);
 void launch_hash_boundary(const SFConst& k, const float4* bp, uint32_t* keys, int nb, hipStream_t s);
+// Incremental stable sort (the particles are in the previous sort's order):
+// key / old_key per slot, old_start = the previous cellStart (C + 1).
+struct IncSortBuffers {
+    uint32_t* in_count;  // C + 1
+    uint32_t* in_start;  // C + 1
+    uint32_t* slot;      // n
+    uint32_t* in_list;   // n
+    uint32_t* stay_pre;  // n + 1
+    uint32_t* part;      // wprefix_blocks(max(n, C) + 1)
+};
+void launch_incremental_sort(const uint32_t* key, const uint32_t* old_key, const uint32_t* old_start, int n,
+                             uint32_t C, const IncSortBuffers& b, uint32_t* perm, uint32_t* sorted_keys,
+                             hipStream_t s);
 // wpre[0..n]: water count among sorted slots [0, i); part holds wprefix_blocks(n + 1) entries
 int wprefix_blocks(int n);
 void launch_wprefix(const float4* pm, int n, uint32_t* part, uint32_t* wpre, hipStream_t s);

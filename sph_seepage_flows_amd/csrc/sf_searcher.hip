@@ -315,7 +315,11 @@ constexpr int kWpItems = 8;
 constexpr int kWpTile = kBlock * kWpItems;
 int wprefix_blocks(int n) { return (n + kWpTile - 1) / kWpTile; }
 
-__global__ __launch_bounds__(kBlock) void k_wcount(const float4* __restrict__ pm, int n, uint32_t* __restrict__ part) {
+// Exclusive prefix of a per-slot value v(i), i in [0, n): block partial sums,
+// one-workgroup scan of the partials, then per-slot prefixes; slot n gets the
+// total.  Used for the water prefix and the incremental sort.
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_vsum(V v, int n, uint32_t* __restrict__ part) {
     __shared__ uint32_t ws[kBlock / 64];
     const int t = threadIdx.x;
     const int base = blockIdx.x * kWpTile;
@@ -323,7 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_wcount(const float4* __restrict__ pm
 #pragma unroll
     for (int q = 0; q < kWpItems; ++q) {
         const int i = base + q * kBlock + t;
-        c += (i < n && pm[i].w > 0.f) ? 1u : 0u;
+        c += i < n ? v(i) : 0u;
     }
     for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
     if ((t & 63) == 0) ws[t >> 6] = c;
@@ -336,8 +340,9 @@ __global__ __launch_bounds__(kBlock) void k_wcount(const float4* __restrict__ pm
 }
 // part is exclusive-scanned; each block writes its slots' prefixes (thread t
 // owns kWpItems consecutive slots); slot n gets the total.
-__global__ __launch_bounds__(kBlock) void k_wprefix(const float4* __restrict__ pm, int n,
-                                                    const uint32_t* __restrict__ part, uint32_t* __restrict__ wpre) {
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_vprefix(V v, int n, const uint32_t* __restrict__ part,
+                                                    uint32_t* __restrict__ out) {
     __shared__ uint32_t wsum[kBlock / 64];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int base = blockIdx.x * kWpTile + t * kWpItems;
@@ -346,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_wprefix(const float4* __restrict__ p
 #pragma unroll
     for (int q = 0; q < kWpItems; ++q) {
         const int i = base + q;
-        f[q] = (i < n && pm[i].w > 0.f) ? 1u : 0u;
+        f[q] = i < n ? v(i) : 0u;
         s += f[q];
     }
     uint32_t x = s;
@@ -362,17 +367,124 @@ __global__ __launch_bounds__(kBlock) void k_wprefix(const float4* __restrict__ p
 #pragma unroll
     for (int q = 0; q < kWpItems; ++q) {
         const int i = base + q;
-        if (i <= n) wpre[i] = run;  // slot n (one past the end) receives the total
+        if (i <= n) out[i] = run;  // slot n (one past the end) receives the total
         run += f[q];
     }
 }
-void launch_wprefix(const float4* pm, int n, uint32_t* part, uint32_t* wpre, hipStream_t s) {
-    // blocks cover slots [0, n]; the block holding slot n (one past the end)
-    // counts nothing and receives the grand total from the scan
+template <class V>
+void launch_vprefix(V v, int n, uint32_t* part, uint32_t* out, hipStream_t s) {
+    // blocks cover slots [0, n]; the block holding slot n counts nothing and
+    // receives the grand total from the scan
     const int nb = wprefix_blocks(n + 1);
-    k_wcount<<<nb, kBlock, 0, s>>>(pm, n, part);
+    k_vsum<<<nb, kBlock, 0, s>>>(v, n, part);
     k_rs_scan<<<1, 1024, 0, s>>>(part, nb);
-    k_wprefix<<<nb, kBlock, 0, s>>>(pm, n, part, wpre);
+    k_vprefix<<<nb, kBlock, 0, s>>>(v, n, part, out);
+}
+struct WaterVal {
+    const float4* pm;
+    SF_DEV uint32_t operator()(int i) const { return pm[i].w > 0.f ? 1u : 0u; }
+};
+struct StayVal {
+    const uint32_t* key;
+    const uint32_t* old;
+    SF_DEV uint32_t operator()(int i) const { return key[i] == old[i] ? 1u : 0u; }
+};
+struct ArrayVal {
+    const uint32_t* a;
+    SF_DEV uint32_t operator()(int i) const { return a[i]; }
+};
+
+void launch_wprefix(const float4* pm, int n, uint32_t* part, uint32_t* wpre, hipStream_t s) {
+    launch_vprefix(WaterVal{pm}, n, part, wpre, s);
+}
+
+// ---------------------------------------------------------------------------
+// Incremental stable sort.  The particles are in the previous step's sorted
+// order and most keep their cell, so the stable sort by the new keys is
+// rebuilt without radix passes: within each new cell the particles are
+// ordered by current index, so a particle's output slot is
+//   newStart(c) + #(stayers of c before it) + #(newcomers of c before it),
+// where stayers of c are the particles of c's old block whose key is still c
+// (a prefix over "key unchanged" flags counts them) and newcomers are the
+// few particles whose key changed to c (collected per cell and ordered by
+// index).  newStart(c) = stayers in cells < c + newcomers in cells < c.
+// The result is the stable sort's, element for element.
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ a, uint32_t v, int n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+__global__ __launch_bounds__(kBlock) void k_inc_flag(const uint32_t* __restrict__ key, const uint32_t* __restrict__ old,
+                                                     int n, uint32_t* __restrict__ in_count,
+                                                     uint32_t* __restrict__ slot) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = key[i];
+    if (c != old[i]) slot[i] = atomicAdd(in_count + c, 1u);
+}
+__global__ __launch_bounds__(kBlock) void k_inc_fill(const uint32_t* __restrict__ key, const uint32_t* __restrict__ old,
+                                                     int n, const uint32_t* __restrict__ in_start,
+                                                     const uint32_t* __restrict__ slot, uint32_t* __restrict__ in_list) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = key[i];
+    if (c != old[i]) in_list[in_start[c] + slot[i]] = (uint32_t)i;
+}
+// newcomers of each cell in increasing index (insertion sort; lists are short)
+__global__ __launch_bounds__(kBlock) void k_inc_order(const uint32_t* __restrict__ in_start, uint32_t C,
+                                                      uint32_t* __restrict__ in_list) {
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= C) return;
+    const uint32_t s = in_start[c], e = in_start[c + 1];
+    for (uint32_t a = s + 1; a < e; ++a) {
+        const uint32_t v = in_list[a];
+        uint32_t b = a;
+        while (b > s && in_list[b - 1] > v) {
+            in_list[b] = in_list[b - 1];
+            --b;
+        }
+        in_list[b] = v;
+    }
+}
+__global__ __launch_bounds__(kBlock) void k_inc_place(const uint32_t* __restrict__ key, const uint32_t* __restrict__ old,
+                                                      int n, const uint32_t* __restrict__ old_start,
+                                                      const uint32_t* __restrict__ stay_pre,
+                                                      const uint32_t* __restrict__ in_start,
+                                                      const uint32_t* __restrict__ in_list, uint32_t* __restrict__ perm,
+                                                      uint32_t* __restrict__ sorted_keys) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = key[i];
+    const uint32_t oa = old_start[c], ob = old_start[c + 1];
+    const uint32_t sa = stay_pre[oa];
+    const uint32_t s = in_start[c], e = in_start[c + 1];
+    uint32_t rank_in = 0;
+    for (uint32_t q = s; q < e; ++q) rank_in += in_list[q] < (uint32_t)i ? 1u : 0u;
+    uint32_t rank_stay;
+    if (c == old[i]) {
+        rank_stay = stay_pre[i] - sa;
+    } else {
+        const uint32_t u = (uint32_t)i;
+        rank_stay = u < oa ? 0u : (u >= ob ? stay_pre[ob] - sa : stay_pre[u] - sa);
+    }
+    const uint32_t pos = sa + s + rank_stay + rank_in;
+    perm[pos] = (uint32_t)i;
+    sorted_keys[pos] = c;
+}
+
+void launch_incremental_sort(const uint32_t* key, const uint32_t* old_key, const uint32_t* old_start, int n,
+                             uint32_t C, const IncSortBuffers& b, uint32_t* perm, uint32_t* sorted_keys,
+                             hipStream_t s) {
+    if (n <= 0) return;
+    const int gn = (n + kBlock - 1) / kBlock;
+    const int gc = (int)((C + kBlock) / kBlock);
+    k_fill_u32<<<gc, kBlock, 0, s>>>(b.in_count, 0u, (int)C);
+    k_inc_flag<<<gn, kBlock, 0, s>>>(key, old_key, n, b.in_count, b.slot);
+    launch_vprefix(StayVal{key, old_key}, n, b.part, b.stay_pre, s);
+    launch_vprefix(ArrayVal{b.in_count}, (int)C, b.part, b.in_start, s);
+    k_inc_fill<<<gn, kBlock, 0, s>>>(key, old_key, n, b.in_start, b.slot, b.in_list);
+    k_inc_order<<<(int)((C + kBlock - 1) / kBlock), kBlock, 0, s>>>(b.in_start, C, b.in_list);
+    k_inc_place<<<gn, kBlock, 0, s>>>(key, old_key, n, old_start, b.stay_pre, b.in_start, b.in_list, perm,
+                                       sorted_keys);
 }
 
 // in-range neighbour counts: r^2 = dx*dx + dy*dy + dz*dz in FP32, left to

@@ -35,6 +35,9 @@ struct sf_system {
     // searcher
     DevBuf<uint32_t> keys, kA, kB, vA, vB, hist, cell, bcell, wpre, wpart;
     DevBuf<uint2> ct;
+    // incremental sort (sf_set_sort_strategy): sorted-key ping-pong, permutation, work arrays
+    DevBuf<uint32_t> skeys[2], sperm, slot, in_list, stay_pre, in_count, in_start, part2;
+    bool sort_inc = true;
     size_t sort_cap = 0;
     const uint32_t* sorted_keys = nullptr;
     const uint32_t* perm = nullptr;
@@ -104,6 +107,7 @@ struct sf_system {
 
 namespace sf {
 int ensure_sort_capacity(sf_system* h, size_t m);
+int ensure_cell_capacity(sf_system* h);
 int build_neighbors(sf_system* h, hipEvent_t* ev = nullptr);
 int solve(sf_system* h, hipEvent_t* ev = nullptr);
 int solve_with(sf_system* h, const Fields& f, hipEvent_t* ev = nullptr);

@@ -93,6 +93,10 @@ class SFSystem:
     def step(self, dt: float, n: int = 1) -> None:
         check(lib().sf_step_n(self._h, float(dt), int(n)))
 
+    def set_sort_strategy(self, incremental: bool) -> None:
+        """Incremental stable sort (default) or the LSD radix sort every step."""
+        check(lib().sf_set_sort_strategy(self._h, _abi.SF_SORT_INCREMENTAL if incremental else _abi.SF_SORT_RADIX))
+
     def step_graph(self, dt: float, n: int) -> None:
         """n fixed-dt steps replayed from a captured HIP graph (same results as step)."""
         check(lib().sf_step_graph(self._h, float(dt), int(n)))
