@@ -140,12 +140,14 @@ SF_DEV float spiky_s(const SFConst& k, float r) {
     float d = k.h - r;
     return k.g_coef * d * d * frcp(r);
 }
-// AdhesionAkinci13 (:189-203).  The base -4r^2/h + 6r - 2h vanishes at r = h
-// and can round to a tiny negative there (pow would give NaN); it is clamped
-// at 0, which changes nothing where the reference's value is finite.  The
-// quarter power is two hardware square roots.
+// AdhesionAkinci13 (:189-203).  The base -4r^2/h + 6r - 2h equals
+// (2/h)(2r - h)(h - r); on h/2 < r <= h both differences are exact in FP32
+// (Sterbenz), so the factored form has no cancellation where the base goes
+// to zero (the expanded form loses most digits there, and its quarter power
+// amplifies that).  The quarter power is two hardware square roots.
+SF_DEV float adhesion_base(const SFConst& k, float r) { return 2.f * (2.f * r - k.h) * (k.h - r) * k.ih; }
 SF_DEV float adhesion_A(const SFConst& k, float r) {
-    if (2.f * r > k.h && r <= k.h) return k.adh_coef * fsqrt(fsqrt(fmaxf(-4.f * r * r * k.ih + 6.f * r - 2.f * k.h, 0.f)));
+    if (2.f * r > k.h && r <= k.h) return k.adh_coef * fsqrt(fsqrt(fmaxf(adhesion_base(k, r), 0.f)));
     return 0.f;
 }
 // QuadraticBezierCoeff (cuda_sf_utils.cuh:36-82)

@@ -246,7 +246,7 @@ SF_DEV void t_adhesion(const SFConst& k, const G3<S>& dp, const S& sat_j, float 
     S l = gsqrt(l2);
     F = zero3<S>();
     if (2.f * val(l) > k.h && val(l) <= k.h) {
-        S base = gmax(-4.f * l * l / k.h + 6.f * l - 2.f * k.h, 0.f);
+        S base = gmax(2.f * (2.f * l - k.h) * (k.h - l) * k.ih, 0.f);
         S A = k.adh_coef * gpow(base, 0.25f);
         if (val(A) != 0.f) {
             const S G = g_bezier(k.gadh, S(k.gadh.py0), S(k.gadh.py1), sat_j);
@@ -334,7 +334,7 @@ SF_DEV void vjp_adhesion(const SFConst& k, v3 dp, float sat_j, float m_i, float 
     gdp += dp * (vsb * (m_j / k.rho_s) * (-6.f * k.w_coef * q2 * q2));
     const float l = sqrtf(l2);
     if (!(2.f * l > k.h && l <= k.h)) return;
-    const float base = -4.f * l * l / k.h + 6.f * l - 2.f * k.h;
+    const float base = adhesion_base(k, l);
     if (!(base > 0.f)) return;
     const float A = k.adh_coef * powf(base, 0.25f);
     const float dA = k.adh_coef * 0.25f * powf(base, -0.75f) * (-8.f * l / k.h + 6.f);
