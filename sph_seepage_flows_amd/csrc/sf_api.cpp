@@ -127,7 +127,9 @@ int ensure_sort_capacity(sf_system* h, size_t m) {
         return rc;
     size_t nblocks = (c + kRsTile - 1) / kRsTile;
     if ((rc = h->hist.alloc(256 * nblocks))) return rc;
-    if ((rc = h->wpre.alloc(c + 1)) || (rc = h->wpart.alloc((size_t)wprefix_blocks((int)c + 1)))) return rc;
+    if ((rc = h->wpre.alloc(c + 1)) || (rc = h->wpart.alloc((size_t)wprefix_blocks((int)c + 1))) ||
+        (rc = h->plist.alloc(c)))
+        return rc;
     if ((rc = h->skeys[0].alloc(c)) || (rc = h->skeys[1].alloc(c)) || (rc = h->sperm.alloc(c)) ||
         (rc = h->slot.alloc(c)) || (rc = h->in_list.alloc(c)) || (rc = h->stay_pre.alloc(c + 1)))
         return rc;
@@ -191,6 +193,7 @@ int build_neighbors(sf_system* h, hipEvent_t* ev) {
     SF_MARK(ev, SF_PASS_GATHER);
     launch_gather(g, n, s);
     launch_wprefix(g.pm_out, n, h->wpart.p, h->wpre.p, s);
+    launch_phase_list(g.pm_out, h->wpre.p, n, h->plist.p, s);
     SF_MARK(ev, SF_PASS_CELL_START);
     launch_cell_start(h->sorted_keys, n, h->C, h->wpre.p, h->cell.p, h->ct.p, s);
     h->cur ^= 1;
@@ -383,7 +386,7 @@ int sf_create(const sf_params* p, const sf_boundary_params* b, const sf_solver_o
     }
     if ((rc = zero(h, h->rp)) || (rc = zero(h, h->es)) || (rc = zero(h, h->flow)) || (rc = zero(h, h->adh)) ||
         (rc = zero(h, h->xf)) || (rc = zero(h, h->acc)) || (rc = zero(h, h->cell)) || (rc = zero(h, h->bcell)) ||
-        (rc = zero(h, h->ct)) || (rc = zero(h, h->wt)) || (rc = zero(h, h->wpre)) ||
+        (rc = zero(h, h->ct)) || (rc = zero(h, h->wt)) || (rc = zero(h, h->wpre)) || (rc = zero(h, h->plist)) ||
         (rc = zero(h, h->keys)) || (rc = zero(h, h->bpv)))
         return bail(rc);
     {

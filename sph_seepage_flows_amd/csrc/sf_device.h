@@ -216,4 +216,23 @@ SF_DEV uint32_t xcd_block() {
     return x < r ? x * (q + 1u) + l : r * (q + 1u) + (x - r) * q + l;
 }
 
+// Block mapping of a pass over two phases of a slot list: na blocks of phase
+// 0, then nb blocks of phase 1.  Each XCD (hardware blocks b with b % 8 = x)
+// first runs its contiguous 1/8 share of the phase-0 blocks, then its share
+// of the phase-1 blocks; surplus hardware blocks get phase -1 and exit.  The
+// grid needs at least na + nb + 16 blocks (phase_grid).
+struct PhaseBlock {
+    int phase;
+    uint32_t blk;  // block within the phase
+};
+SF_DEV PhaseBlock phase_block(uint32_t na, uint32_t nb) {
+    const uint32_t b = blockIdx.x, x = b & 7u, l = b >> 3;
+    const uint32_t qa = na >> 3, ra = na & 7u, qb = nb >> 3, rb = nb & 7u;
+    const uint32_t ca = qa + (x < ra ? 1u : 0u), cbn = qb + (x < rb ? 1u : 0u);
+    if (l < ca) return {0, x * qa + (x < ra ? x : ra) + l};
+    if (l - ca < cbn) return {1, x * qb + (x < rb ? x : rb) + (l - ca)};
+    return {-1, 0u};
+}
+inline int phase_grid(int n, int nb) { return (n + nb - 1) / nb + 16; }
+
 }  // namespace sf

@@ -121,6 +121,8 @@ void launch_incremental_sort(const uint32_t* key, const uint32_t* old_key, const
 // wpre[0..n]: water count among sorted slots [0, i); part holds wprefix_blocks(n + 1) entries
 int wprefix_blocks(int n);
 void launch_wprefix(const float4* pm, int n, uint32_t* part, uint32_t* wpre, hipStream_t s);
+// plist[wpre[i]] = i for water slots, plist[wpre[n] + i - wpre[i]] = i for sand slots
+void launch_phase_list(const float4* pm, const uint32_t* wpre, int n, uint32_t* plist, hipStream_t s);
 
 // ---- solver launchers (sf_solver.hip) ----
 struct Fields {
@@ -145,6 +147,10 @@ struct Fields {
     // batched handles only (null otherwise): per-slot scene id, per-scene constants
     const uint16_t* scene;
     const SceneConst* sct;
+    // slots by phase: plist[0, nw) the water slots, plist[nw, n) the sand
+    // slots, each in slot order; *nwater = nw (device)
+    const uint32_t* plist;
+    const uint32_t* nwater;
 };
 // Launch shape of a neighbour pass: workgroup size and the per-lane LDS
 // candidate-list capacity (the latter only for the passes that use one).

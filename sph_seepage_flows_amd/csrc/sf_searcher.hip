@@ -398,6 +398,20 @@ void launch_wprefix(const float4* pm, int n, uint32_t* part, uint32_t* wpre, hip
     launch_vprefix(WaterVal{pm}, n, part, wpre, s);
 }
 
+// Slots grouped by phase (water first), each group in slot order, so a pass
+// over one phase can give every XCD an equal contiguous share of it.
+__global__ __launch_bounds__(kBlock) void k_phase_list(const float4* __restrict__ pm, const uint32_t* __restrict__ wpre,
+                                                       int n, uint32_t* __restrict__ plist) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t w = wpre[i];
+    plist[pm[i].w < 0.f ? wpre[n] + (uint32_t)i - w : w] = (uint32_t)i;
+}
+void launch_phase_list(const float4* pm, const uint32_t* wpre, int n, uint32_t* plist, hipStream_t s) {
+    if (n <= 0) return;
+    k_phase_list<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(pm, wpre, n, plist);
+}
+
 // ---------------------------------------------------------------------------
 // Incremental stable sort.  The particles are in the previous step's sorted
 // order and most keep their cell, so the stable sort by the new keys is

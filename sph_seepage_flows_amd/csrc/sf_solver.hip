@@ -99,6 +99,90 @@ SF_DEV void for_ranges(const SFConst& k, const Fields& f, uint32_t cb, int cx, i
 
 SF_DEV float sq3(float d) { return d * d * d; }
 
+// ---------------------------------------------------------------------------
+// Wave-staged neighbour columns.  The 64 lanes of a wave hold consecutive
+// slots of the cell-sorted order, so they sit in a few consecutive cells of
+// one grid column (x, y), rarely two or three, and their 27-cell
+// neighbourhoods overlap almost entirely: lane by lane, the eight or so lanes
+// of one cell would load exactly the same candidates.  Here the lanes that
+// share a column (a "segment") take, for each of the 9 neighbour columns, the
+// union of their z-ranges -- one contiguous slot range, from the first lane's
+// range start to the last lane's range end -- load it once with coalesced
+// loads into the wave's LDS stage, and each lane then walks its own window
+// of the union from LDS.  Each wave owns its stage; the lanes of one wave
+// order their LDS accesses with wave-scope fences only.
+constexpr int kStage = 128;  // candidates staged per chunk and wave
+
+SF_DEV void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <class T>
+SF_DEV T lane_value(T v, int lane) {
+    return __builtin_amdgcn_readlane(v, lane);
+}
+
+// For the lanes with `active` set: fn(j, staged_j) over every slot j of the
+// lane's 9 column ranges (cells z-1..z+1), column by column.  prep(pm[j]) is what gets staged (once per wave, not per
+// lane).  need(nw, ns), given the water / sand counts of a column's union
+// range, may skip the whole column.  Must be called by all lanes of the wave
+// (uniform control flow).
+template <class Prep, class Need, class Fn>
+SF_DEV void staged_walk(const SFConst& k, const Fields& f, float4* __restrict__ stage, bool active, uint32_t cb,
+                        int cx, int cy, int cz, Prep&& prep, Need&& need, Fn&& fn) {
+    const int lane = (int)(threadIdx.x & 63u);
+    const uint32_t col = cb + (uint32_t)((cx * k.gy + cy) * k.gz);
+    const int zlo = cz > 0 ? cz - 1 : 0;
+    const int zhi = cz + 1 < k.gz ? cz + 1 : k.gz - 1;
+    uint64_t todo = __ballot(active);
+    while (todo) {
+        const int lead = __builtin_ctzll(todo);
+        const uint32_t lcol = lane_value(col, lead);
+        const bool mine = active && col == lcol;
+        const uint64_t seg = __ballot(mine);
+        todo &= ~seg;
+        const int last = 63 - __builtin_clzll(seg);
+        const int sx = lane_value(cx, lead), sy = lane_value(cy, lead);
+        const uint32_t scb = lane_value(cb, lead);
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int x = sx + dx;
+            if (x < 0 || x >= k.gx) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int y = sy + dy;
+                if (y < 0 || y >= k.gy) continue;
+                const uint32_t base = scb + (uint32_t)((x * k.gy + y) * k.gz);
+                uint2 a = make_uint2(0u, 0u), b = make_uint2(0u, 0u);
+                if (mine) {
+                    a = ldo(f.ct, base + (uint32_t)zlo);
+                    b = ldo(f.ct, base + (uint32_t)zhi + 1u);
+                }
+                const uint32_t us = lane_value(a.x, lead), ue = lane_value(b.x, last);
+                if (us >= ue) continue;
+                const uint32_t nw = lane_value(b.y, last) - lane_value(a.y, lead);
+                if (!need(nw, (ue - us) - nw)) continue;
+                for (uint32_t u = us; u < ue; u += (uint32_t)kStage) {
+                    const uint32_t m = ue - u < (uint32_t)kStage ? ue - u : (uint32_t)kStage;
+                    for (uint32_t q = (uint32_t)lane; q < m; q += 64u) stage[q] = prep(ldo(f.pm, u + q));
+                    wave_sync();
+                    const uint32_t wlo = a.x > u ? a.x : u, whi = b.x < u + m ? b.x : u + m;
+                    uint32_t j = wlo;
+                    for (; j + 4u <= whi; j += 4u) {
+                        const float4 q0 = stage[j - u], q1 = stage[j + 1u - u],
+                                     q2 = stage[j + 2u - u], q3 = stage[j + 3u - u];
+                        fn(j, q0);
+                        fn(j + 1u, q1);
+                        fn(j + 2u, q2);
+                        fn(j + 3u, q3);
+                    }
+                    for (; j < whi; ++j) fn(j, stage[j - u]);
+                    wave_sync();
+                }
+            }
+        }
+    }
+}
+
 // Host-side launch dispatch over (batched?, workgroup size, list capacity);
 // the choices are kNbChoices x kCapChoices.
 template <int V>
@@ -124,25 +208,36 @@ void dispatch(const Fields& f, PassCfg c, L&& l) {
 // support and max(w, 0) drops sand (w = -mass).
 template <bool B, int NB>
 __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
-    const int i = xcd_block() * NB + threadIdx.x;
-    if (i >= n) return;
-    const float4 pi = ldo(f.pm, (uint32_t)i);
-    if (pi.w < 0.f) {
-        f.rp[i] = make_float2(k.rho_s, 0.f);
+    __shared__ float4 stage_lds[(NB / 64) * kStage];
+    float4* stage = stage_lds + (threadIdx.x >> 6) * kStage;
+    const uint32_t nw = *f.nwater, ns = (uint32_t)n - nw;
+    const PhaseBlock pb = phase_block((nw + NB - 1) / NB, (ns + NB - 1) / NB);
+    if (pb.phase < 0) return;
+    const uint32_t t = pb.blk * NB + threadIdx.x;
+    if (pb.phase == 1) {  // sand: rho_sand, p = 0
+        if (t < ns) f.rp[ldo(f.plist, nw + t)] = make_float2(k.rho_s, 0.f);
         return;
     }
-    const uint32_t cb = scene_base<B>(k, f, i);
+    const bool water = t < nw;
+    const int i = water ? (int)ldo(f.plist, t) : 0;
+    const float4 pi = water ? ldo(f.pm, (uint32_t)i) : make_float4(0.f, 0.f, 0.f, -1.f);
+    const uint32_t cb = water ? scene_base<B>(k, f, i) : 0u;
     int cx, cy, cz;
     cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
     float sf = 0.f, sb = 0.f;
-    for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t) {
-        if (nw == 0) return;
-        for_range4(f.pm, s, e, [&](uint32_t, const float4& pj) {
+    staged_walk(
+        k, f, stage, water, cb, cx, cy, cz,
+        [](float4 p) {
+            p.w = fmaxf(p.w, 0.f);
+            return p;
+        },
+        [](uint32_t nw, uint32_t) { return nw != 0u; },
+        [&](uint32_t, const float4& pj) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float d = fmaxf(k.h2 - (dx * dx + dy * dy + dz * dz), 0.f);
-            sf += fmaxf(pj.w, 0.f) * sq3(d);
+            sf += pj.w * sq3(d);
         });
-    });
+    if (!water) return;
     for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
         const uint32_t be = ldo(f.bcell, c1);
         for (uint32_t j = ldo(f.bcell, c0); j < be; ++j) {
@@ -158,7 +253,7 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
 void launch_density(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
     if (n <= 0) return;
     dispatch(f, c, [&](auto B, auto NB, auto) {
-        k_density<decltype(B)::value, decltype(NB)::value><<<(n + NB - 1) / NB, (int)NB, 0, s>>>(k, f, n);
+        k_density<decltype(B)::value, decltype(NB)::value><<<phase_grid(n, NB), (int)NB, 0, s>>>(k, f, n);
     });
 }
 
@@ -293,8 +388,11 @@ SF_DEV void near_span(float p, float lo, float h, int c, int g, float rc, int& a
 template <bool B, int NB, int CAP>
 __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n) {
     SF_NLIST(L);
-    const int i = xcd_block() * NB + threadIdx.x;
-    if (i >= n) return;
+    const uint32_t nw = *f.nwater, ns = (uint32_t)n - nw;
+    const PhaseBlock pb = phase_block(0u, (ns + NB - 1) / NB);
+    const uint32_t t = pb.blk * NB + threadIdx.x;
+    if (pb.phase < 0 || t >= ns) return;
+    const int i = (int)ldo(f.plist, nw + t);
     const float4 pi = ldo(f.pm, (uint32_t)i);
     if (pi.w >= 0.f) return;
     const uint32_t cb = scene_base<B>(k, f, i);
@@ -418,7 +516,7 @@ void launch_sand_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, h
     if (n <= 0) return;
     dispatch(f, c, [&](auto B, auto NB, auto CAP) {
         k_sand_momentum<decltype(B)::value, decltype(NB)::value, decltype(CAP)::value>
-            <<<(n + NB - 1) / NB, (int)NB, 0, s>>>(k, f, n);
+            <<<phase_grid(n, NB), (int)NB, 0, s>>>(k, f, n);
     });
 }
 
@@ -428,8 +526,11 @@ void launch_sand_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, h
 template <bool B, int NB, int CAP>
 __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int n) {
     SF_NLIST(L);
-    const int i = xcd_block() * NB + threadIdx.x;
-    if (i >= n) return;
+    const uint32_t nw = *f.nwater;
+    const PhaseBlock pb = phase_block((nw + NB - 1) / NB, 0u);
+    const uint32_t t = pb.blk * NB + threadIdx.x;
+    if (pb.phase < 0 || t >= nw) return;
+    const int i = (int)ldo(f.plist, t);
     const float4 pi = ldo(f.pm, (uint32_t)i);
     if (pi.w < 0.f) return;
     const uint32_t cb = scene_base<B>(k, f, i);
@@ -502,7 +603,7 @@ void launch_water_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, 
     if (n <= 0) return;
     dispatch(f, c, [&](auto B, auto NB, auto CAP) {
         k_water_momentum<decltype(B)::value, decltype(NB)::value, decltype(CAP)::value>
-            <<<(n + NB - 1) / NB, (int)NB, 0, s>>>(k, f, n);
+            <<<phase_grid(n, NB), (int)NB, 0, s>>>(k, f, n);
     });
 }
 

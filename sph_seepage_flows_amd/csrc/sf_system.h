@@ -33,7 +33,7 @@ struct sf_system {
     int cur_cs = 0;   // cs
     int cur_aux = 0;  // cda / amc
     // searcher
-    DevBuf<uint32_t> keys, kA, kB, vA, vB, hist, cell, bcell, wpre, wpart;
+    DevBuf<uint32_t> keys, kA, kB, vA, vB, hist, cell, bcell, wpre, wpart, plist;
     DevBuf<uint2> ct;
     // incremental sort (sf_set_sort_strategy): sorted-key ping-pong, permutation, work arrays
     DevBuf<uint32_t> skeys[2], sperm, slot, in_list, stay_pre, in_count, in_start, part2;
@@ -92,6 +92,8 @@ struct sf_system {
         f.den4 = nullptr;
         f.scene = nscenes ? scene.p : nullptr;
         f.sct = nscenes ? sct.p : nullptr;
+        f.plist = plist.p;
+        f.nwater = wpre.p + n;
         return f;
     }
     uint32_t local_cells() const { return K.C; }
