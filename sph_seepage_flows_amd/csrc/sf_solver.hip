@@ -36,9 +36,9 @@ SF_DEV float eos(const SFConst& k, float rho) {
 }
 
 // Per-thread candidate lists in LDS.  A neighbour loop first filters the
-// candidates of its cell ranges (cheap: one float4 load, r^2, compare) and
-// appends the survivors to the lane's list; the expensive pair bodies then run
-// over the compacted lists, so a wave no longer executes every body for every
+// candidates of its cell ranges (cheap: one load, r^2, compare) and appends
+// the survivors to the lane's list; the expensive pair bodies then run over
+// the compacted lists, so a wave no longer executes every body for every
 // candidate that some lane happens to keep.  The flush is decided wave-wide
 // before appending, so every list stays within CAP.  NB is the workgroup size.
 template <int NB, int CAP>
@@ -50,15 +50,30 @@ struct NList {
     __shared__ uint32_t name##_lds[CAP * NB];            \
     NList<NB, CAP> name{name##_lds + threadIdx.x, 0}
 
-template <int NB, int CAP, class Body>
-SF_DEV void nl_flush(NList<NB, CAP>& L, Body&& body) {
-    for (int k = 0; k < L.cnt; ++k) body(L.slot[k * NB]);
+// The pair body is split in two: fetch(j) issues every load a pair needs,
+// with no branch on loaded data, and apply(j, data) does the arithmetic.  The flush is software-pipelined: the next pair's loads are
+// issued before the current pair's arithmetic, so a flush no longer waits on
+// one pair's dependent load chain at a time.  apply has a single call site,
+// so every pair gets the same instruction sequence however the flushes fall.
+template <int NB, int CAP, class Fetch, class Apply>
+SF_DEV void nl_flush2(NList<NB, CAP>& L, Fetch&& fetch, Apply&& apply) {
+    if (L.cnt > 0) {
+        uint32_t j = L.slot[0];
+        auto cur = fetch(j);
+        for (int q = 0; q < L.cnt; ++q) {
+            const uint32_t jn = L.slot[(q + 1 < L.cnt ? q + 1 : q) * NB];
+            const auto nxt = fetch(jn);
+            apply(j, cur);
+            j = jn;
+            cur = nxt;
+        }
+    }
     L.cnt = 0;
 }
-template <int NB, int CAP, class Body>
-SF_DEV void nl_push(NList<NB, CAP>& L, bool keep, uint32_t j, Body&& body) {
-    if (__ballot(L.cnt == CAP)) nl_flush(L, body);
-    L.slot[L.cnt * NB] = j;  // cnt < CAP here; a dropped slot is overwritten later
+template <int NB, int CAP, class Fetch, class Apply>
+SF_DEV void nl_push2(NList<NB, CAP>& L, bool keep, uint32_t j, Fetch&& fetch, Apply&& apply) {
+    if (__ballot(L.cnt == CAP)) nl_flush2(L, fetch, apply);
+    L.slot[L.cnt * NB] = j;
     L.cnt += keep ? 1 : 0;
 }
 
@@ -419,15 +434,20 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
     float vw = 0.f;
 
     // sand-sand: DEM contact or liquid bridge
-    auto contact = [&](uint32_t j) {
-        const float4 pj = ldo(f.pm, j);
+    struct SandNbr {
+        float4 pj, vj;
+        float satj;
+    };
+    auto fetch_sand = [&](uint32_t j) { return SandNbr{ldo(f.pm, j), ldo(f.vr, j), ldo(f.es, j).y}; };
+    auto contact = [&](uint32_t, const SandNbr& q) {
+        const float4 pj = q.pj;
         const v3 dji = mk3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
         const float dist = fsqrt(dot(dji, dji));
         if (dist < twoR) {
-            fsum += dem_contact(dji, dist, xyz(ldo(f.vr, j)) - vi, twoR, k.kn, k.ks, k.tan2);
+            fsum += dem_contact(dji, dist, xyz(q.vj) - vi, twoR, k.kn, k.ks, k.tan2);
         } else {
             const float H = dist - twoR;
-            if (H > 0.f && H < k.cap_srup) fsum += capillary(k, dji, dist, H, (sat_i + ldo(f.es, j).y) * 0.5f);
+            if (H > 0.f && H < k.cap_srup) fsum += capillary(k, dji, dist, H, (sat_i + q.satj) * 0.5f);
         }
     };
     {
@@ -443,21 +463,27 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
                 for_range4(f.pm, s0, e0, [&](uint32_t j, const float4& pj) {
                     const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
                     const float r2 = dx * dx + dy * dy + dz * dz;
-                    nl_push(L, pj.w < 0.f && (int)j != i && r2 < cap_r2, j, contact);
+                    nl_push2(L, pj.w < 0.f && (int)j != i && r2 < cap_r2, j, fetch_sand, contact);
                 });
             }
-        nl_flush(L, contact);
+        nl_flush2(L, fetch_sand, contact);
     }
     // water neighbours: buoyancy (ComputeSFSandBuoyancyForces) + adhesion
     // (ComputeSFSandAdhesionTerm)
-    auto water = [&](uint32_t j) {
-        const float4 pj = ldo(f.pm, j);
+    struct WaterNbr {
+        float4 pj, xfj;
+        float2 rpj;
+        float eps_j;
+    };
+    auto fetch_water = [&](uint32_t j) { return WaterNbr{ldo(f.pm, j), ldo(f.xf, j), ldo(f.rp, j), ldo(f.es, j).x}; };
+    auto water = [&](uint32_t, const WaterNbr& q) {
+        const float4 pj = q.pj;
         const v3 dji = mk3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
         const float r2 = dot(dji, dji);
         const float w = k.w_coef * sq3(k.h2 - r2);
-        const float eps_j = ldo(f.es, j).x;
+        const float eps_j = q.eps_j;
         if (eps_j > 0.f && eps_j <= 1.f) {
-            const float2 rpj = ldo(f.rp, j);
+            const float2 rpj = q.rpj;
             const float mj = pj.w;
             const float dj = rpj.x * frcp(fmaxf(kEps, eps_j));
             vw += mj * frcp(dj) * w;
@@ -465,17 +491,17 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
             // gradW(x_i - x_j) = gs * (x_i - x_j) = -gs * dji
             fsum += dji * (mi * mj * rpj.y * frcp(fmaxf(kEps, rho_i * dj)) * gs);
         }
-        fsum += xyz(ldo(f.xf, j)) * (w * vol_i);
+        fsum += xyz(q.xfj) * (w * vol_i);
     };
     for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t) {
         if (nw == 0) return;
         for_range4(f.pm, s, e, [&](uint32_t j, const float4& pj) {
             const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            nl_push(L, pj.w > 0.f && r2 < k.h2, j, water);
+            nl_push2(L, pj.w > 0.f && r2 < k.h2, j, fetch_water, water);
         });
     });
-    nl_flush(L, water);
+    nl_flush2(L, fetch_water, water);
     if (k.blabel1) {  // ComputeSFUniRadiusDemBoundaryForces: boundary label 1 only
         for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
             const uint32_t be = ldo(f.bcell, c1);
@@ -545,8 +571,15 @@ __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int 
     int cx, cy, cz;
     cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
     v3 a = mk3(0.f, 0.f, 0.f);
-    auto pair = [&](uint32_t j) {
-        const float4 pj = ldo(f.pm, j);
+    // neighbour data of the common (water) case; every slot has these arrays
+    struct Nbr {
+        float4 pj;
+        float2 wtj;
+        float4 vj;
+    };
+    auto fetch = [&](uint32_t j) { return Nbr{ldo(f.pm, j), ldo(f.wt, j), ldo(f.vr, j)}; };
+    auto pair = [&](uint32_t j, const Nbr& q) {
+        const float4 pj = q.pj;
         const v3 d = mk3(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
         const float r2 = dot(d, d);
         const float rs = frsq(r2);
@@ -554,9 +587,9 @@ __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int 
         const float t = k.h - r;
         const float gs = r >= kEps ? k.g_coef * t * t * rs : 0.f;  // spiky gradient / r
         if (pj.w > 0.f) {  // water j: ComputeSFWaterPressureTerm + ArtificialViscosity
-            const float2 wtj = ldo(f.wt, j);
+            const float2 wtj = q.wtj;
             float s = -pj.w * (wti.y + wtj.y);
-            const v3 dv = vi - xyz(ldo(f.vr, j));
+            const v3 dv = vi - xyz(q.vj);
             const float dd = dot(dv, d);
             const float pij = -k.nu * frcp(rt_i + wtj.x) * (dd * frcp(r2 + kEps));
             s += dd < 0.f ? -pj.w * pij : 0.f;
@@ -573,10 +606,10 @@ __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int 
         for_range4(f.pm, s, e, [&](uint32_t j, const float4& pj) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            nl_push(L, r2 < k.h2 && (int)j != i, j, pair);
+            nl_push2(L, r2 < k.h2 && (int)j != i, j, fetch, pair);
         });
     });
-    nl_flush(L, pair);
+    nl_flush2(L, fetch, pair);
     for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
         const uint32_t be = ldo(f.bcell, c1);
         for (uint32_t j = ldo(f.bcell, c0); j < be; ++j) {
