@@ -127,7 +127,7 @@ int ensure_sort_capacity(sf_system* h, size_t m) {
         return rc;
     size_t nblocks = (c + kRsTile - 1) / kRsTile;
     if ((rc = h->hist.alloc(256 * nblocks))) return rc;
-    if ((rc = h->wpre.alloc(c + 1)) || (rc = h->wpart.alloc((size_t)wprefix_blocks((int)c + 1))) ||
+    if ((rc = h->wpre.alloc(c + 1)) || (rc = h->wpart.alloc((size_t)wprefix_blocks((int)c + 1) + 1)) ||
         (rc = h->plist.alloc(c)))
         return rc;
     if ((rc = h->skeys[0].alloc(c)) || (rc = h->skeys[1].alloc(c)) || (rc = h->sperm.alloc(c)) ||
@@ -142,8 +142,12 @@ int ensure_sort_capacity(sf_system* h, size_t m) {
 int ensure_cell_capacity(sf_system* h) {
     const size_t C1 = (size_t)h->C + 1;
     int rc;
-    if (h->in_count.n < C1 && ((rc = h->in_count.alloc(C1)) || (rc = h->in_start.alloc(C1)))) return rc;
-    const size_t np = (size_t)wprefix_blocks((int)std::max<size_t>(h->sort_cap, h->C) + 1);
+    if (h->in_count.n < C1) {
+        if ((rc = h->in_count.alloc(C1)) || (rc = h->in_start.alloc(C1))) return rc;
+        // zero before the first incremental sort; afterwards every cell-start pass clears it
+        SF_HIP(hipMemsetAsync(h->in_count.p, 0, C1 * sizeof(uint32_t), h->stream));
+    }
+    const size_t np = (size_t)vprefix2_partials((int)h->sort_cap, (int)h->C);
     if (h->part2.n < np && (rc = h->part2.alloc(np))) return rc;
     return SF_OK;
 }
@@ -191,11 +195,9 @@ int build_neighbors(sf_system* h, hipEvent_t* ev) {
     g.amc_out = h->amc[h->cur_aux ^ 1].p;
     g.multi = h->K.multi;
     SF_MARK(ev, SF_PASS_GATHER);
-    launch_gather(g, n, s);
-    launch_wprefix(g.pm_out, n, h->wpart.p, h->wpre.p, s);
-    launch_phase_list(g.pm_out, h->wpre.p, n, h->plist.p, s);
+    launch_gather_wprefix(g, n, h->wpart.p, h->wpre.p, h->plist.p, s);
     SF_MARK(ev, SF_PASS_CELL_START);
-    launch_cell_start(h->sorted_keys, n, h->C, h->wpre.p, h->cell.p, h->ct.p, s);
+    launch_cell_start(h->sorted_keys, n, h->C, h->wpre.p, h->cell.p, h->ct.p, h->in_count.p, s);
     h->cur ^= 1;
     h->cur_cs ^= 1;
     if (h->K.multi) h->cur_aux ^= 1;
@@ -506,7 +508,7 @@ int sf_upload_boundary(sf_handle h, size_t nb, const float* pos3, const uint64_t
     SortBuffers sb{h->kA.p, h->vA.p, h->kB.p, h->vB.p, h->hist.p};
     const uint32_t *sk = nullptr, *bperm = nullptr;
     if ((rc = radix_sort(h->keys.p, (int)nb, h->plan, sb, &sk, &bperm, h->stream))) return rc;
-    launch_cell_start(sk, (int)nb, h->local_cells(), nullptr, h->bcell.p, nullptr, h->stream);
+    launch_cell_start(sk, (int)nb, h->local_cells(), nullptr, h->bcell.p, nullptr, nullptr, h->stream);
     launch_boundary_gather(bperm, tmp.p, h->bpv.p, (int)nb, h->stream);
     launch_boundary_volume(h->K, h->bpv.p, h->bcell.p, (int)nb, h->stream);
     SF_HIP(hipGetLastError());

@@ -88,9 +88,10 @@ int radix_sort(const uint32_t* keys, int n, const RadixPlan& plan, const SortBuf
 // ---- searcher launchers (sf_searcher.hip) ----
 // scene (batched handles): per-slot scene id, keys offset by scene * C; null for one scene
 void launch_hash(const SFConst& k, const float4* pm, const uint16_t* scene, uint32_t* keys, int n, hipStream_t s);
-// cellStart[c] for c in [0, C] plus the cell table ct[c] = (cellStart[c], wpre[cellStart[c]])
+// cellStart[c] for c in [0, C] plus the cell table ct[c] = (cellStart[c], wpre[cellStart[c]]);
+// zero_c (may be null) is cleared over [0, C) (the incremental sort's newcomer counters)
 void launch_cell_start(const uint32_t* sorted_keys, int n, uint32_t C, const uint32_t* wpre, uint32_t* cell_start,
-                       uint2* ct, hipStream_t s);
+                       uint2* ct, uint32_t* zero_c, hipStream_t s);
 struct GatherArgs {
     const uint32_t* perm;
     const float4 *pm_in, *vr_in, *cs_in, *cda_in;
@@ -99,7 +100,11 @@ struct GatherArgs {
     float2* amc_out;
     int multi;
 };
-void launch_gather(const GatherArgs& a, int n, hipStream_t s);
+// Reorders the persistent state by perm, then wpre[0..n] = water count among
+// slots [0, i) and plist = water slots then sand slots, each in slot order.
+// wpart holds wprefix_blocks(n + 1) + 1 entries.
+void launch_gather_wprefix(const GatherArgs& a, int n, uint32_t* wpart, uint32_t* wpre, uint32_t* plist,
+                           hipStream_t s);
 void launch_boundary_gather(const uint32_t* perm, const float4* bp_in, float4* bp_out, int nb, hipStream_t s);
 void launch_boundary_volume(const SFConst& k, float4* bpv, const uint32_t* bcs, int nb, hipStream_t s);
 void launch_nbr_count(const SFConst& k, const float4* pm, const uint16_t* scene, const uint32_t* cs, uint32_t* cnt,
@@ -108,21 +113,19 @@ void launch_hash_boundary(const SFConst& k, const float4* bp, uint32_t* keys, in
 // Incremental stable sort (the particles are in the previous sort's order):
 // key / old_key per slot, old_start = the previous cellStart (C + 1).
 struct IncSortBuffers {
-    uint32_t* in_count;  // C + 1
+    uint32_t* in_count;  // C + 1, zero on entry (launch_cell_start clears it)
     uint32_t* in_start;  // C + 1
     uint32_t* slot;      // n
     uint32_t* in_list;   // n
     uint32_t* stay_pre;  // n + 1
-    uint32_t* part;      // wprefix_blocks(max(n, C) + 1)
+    uint32_t* part;      // vprefix2_partials(n, C)
 };
 void launch_incremental_sort(const uint32_t* key, const uint32_t* old_key, const uint32_t* old_start, int n,
                              uint32_t C, const IncSortBuffers& b, uint32_t* perm, uint32_t* sorted_keys,
                              hipStream_t s);
-// wpre[0..n]: water count among sorted slots [0, i); part holds wprefix_blocks(n + 1) entries
+// partial-sum counts of the prefix kernels
 int wprefix_blocks(int n);
-void launch_wprefix(const float4* pm, int n, uint32_t* part, uint32_t* wpre, hipStream_t s);
-// plist[wpre[i]] = i for water slots, plist[wpre[n] + i - wpre[i]] = i for sand slots
-void launch_phase_list(const float4* pm, const uint32_t* wpre, int n, uint32_t* plist, hipStream_t s);
+int vprefix2_partials(int n1, int n2);
 
 // ---- solver launchers (sf_solver.hip) ----
 struct Fields {

@@ -93,8 +93,9 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_upsweep(const uint32_t* __res
     if (t <= (int)mask) hist[(size_t)t * gridDim.x + blockIdx.x] = cnt[t];
 }
 
-// exclusive scan of hist[0, total) in place, one workgroup of 1024 threads
-__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ hist, int total) {
+// exclusive scan of hist[0, total) in place by one workgroup of 1024
+// threads; *total_out (if given) receives the sum
+SF_DEV void scan_range(uint32_t* __restrict__ hist, int total, uint32_t* __restrict__ total_out) {
     __shared__ uint32_t wsum[16];
     __shared__ uint32_t carry_s;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -140,6 +141,17 @@ __global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ hist, i
         if (t == 1023) carry_s = carry + wsum[15];
         __syncthreads();
     }
+    if (total_out && t == 0) *total_out = carry_s;
+    __syncthreads();
+}
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ hist, int total) {
+    scan_range(hist, total, nullptr);
+}
+// two independent scans, [0, n1) and [n1, n1 + n2), in one launch
+__global__ __launch_bounds__(1024) void k_rs_scan2(uint32_t* __restrict__ part, int n1, int n2,
+                                                  uint32_t* __restrict__ total_out) {
+    scan_range(part, n1, total_out);
+    scan_range(part + n1, n2, nullptr);
 }
 
 // stable scatter: each wave ranks a contiguous chunk of 64*kRsItems keys
@@ -225,9 +237,10 @@ int radix_sort(const uint32_t* keys, int n, const RadixPlan& plan, const SortBuf
 // (CountingInCell_CUDA + exclusive_scan, cuda_neighbor_searcher.cu:45-47)
 __global__ __launch_bounds__(kBlock) void k_cell_start(const uint32_t* __restrict__ sk, int n, uint32_t C,
                                                        const uint32_t* __restrict__ wpre, uint32_t* __restrict__ cs,
-                                                       uint2* __restrict__ ct) {
+                                                       uint2* __restrict__ ct, uint32_t* __restrict__ zero_c) {
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c > C) return;
+    if (zero_c && c < C) zero_c[c] = 0u;  // the next incremental sort's newcomer counters
     uint32_t lo = 0, hi = (uint32_t)n;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
@@ -238,30 +251,46 @@ __global__ __launch_bounds__(kBlock) void k_cell_start(const uint32_t* __restric
     if (ct) ct[c] = make_uint2(lo, wpre ? wpre[lo] : 0u);
 }
 void launch_cell_start(const uint32_t* sorted_keys, int n, uint32_t C, const uint32_t* wpre, uint32_t* cell_start,
-                       uint2* ct, hipStream_t s) {
+                       uint2* ct, uint32_t* zero_c, hipStream_t s) {
     const uint32_t total = C + 1;
-    k_cell_start<<<(total + kBlock - 1) / kBlock, kBlock, 0, s>>>(sorted_keys, n, C, wpre, cell_start, ct);
+    k_cell_start<<<(total + kBlock - 1) / kBlock, kBlock, 0, s>>>(sorted_keys, n, C, wpre, cell_start, ct, zero_c);
 }
 
 // ---------------------------------------------------------------------------
 // reorder of the persistent state (the zipped payload of sort_by_key,
 // cuda_neighbor_searcher.cu:90-111)
-__global__ __launch_bounds__(kBlock) void k_gather(GatherArgs a, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    uint32_t p = a.perm[i];
-    if (p >= (uint32_t)n) p = (uint32_t)i;  // a permutation of [0, n) never takes this branch
-    a.pm_out[i] = a.pm_in[p];
-    a.vr_out[i] = a.vr_in[p];
-    a.cs_out[i] = a.cs_in[p];
-    if (a.multi) {
-        a.cda_out[i] = a.cda_in[p];
-        a.amc_out[i] = a.amc_in[p];
+// Each block gathers one water-prefix tile (kWpTile slots) and stores the
+// tile's water count in wpart[block] for the prefix that follows.
+constexpr int kWpItems = 8;
+constexpr int kWpTile = kBlock * kWpItems;
+__global__ __launch_bounds__(kBlock) void k_gather(GatherArgs a, int n, uint32_t* __restrict__ wpart) {
+    __shared__ uint32_t ws[kBlock / 64];
+    const int t = threadIdx.x;
+    uint32_t c = 0;
+#pragma unroll
+    for (int q = 0; q < kWpItems; ++q) {
+        const int i = blockIdx.x * kWpTile + q * kBlock + t;
+        if (i >= n) break;
+        uint32_t p = a.perm[i];
+        if (p >= (uint32_t)n) p = (uint32_t)i;  // a permutation of [0, n) never takes this branch
+        const float4 pm = a.pm_in[p];
+        a.pm_out[i] = pm;
+        a.vr_out[i] = a.vr_in[p];
+        a.cs_out[i] = a.cs_in[p];
+        if (a.multi) {
+            a.cda_out[i] = a.cda_in[p];
+            a.amc_out[i] = a.amc_in[p];
+        }
+        c += pm.w > 0.f ? 1u : 0u;
     }
-}
-void launch_gather(const GatherArgs& a, int n, hipStream_t s) {
-    if (n <= 0) return;
-    k_gather<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a, n);
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+    if ((t & 63) == 0) ws[t >> 6] = c;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t sum = 0;
+        for (int q = 0; q < kBlock / 64; ++q) sum += ws[q];
+        wpart[blockIdx.x] = sum;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_boundary_gather(const uint32_t* __restrict__ perm,
@@ -311,18 +340,16 @@ void launch_boundary_volume(const SFConst& k, float4* bpv, const uint32_t* bcs, 
 // lets a neighbour loop see how many water / sand particles a contiguous
 // cell range holds (wpre[e] - wpre[s]) and skip ranges without the phase it
 // needs.  Two kernels around a tiny scan of per-block counts.
-constexpr int kWpItems = 8;
-constexpr int kWpTile = kBlock * kWpItems;
 int wprefix_blocks(int n) { return (n + kWpTile - 1) / kWpTile; }
 
 // Exclusive prefix of a per-slot value v(i), i in [0, n): block partial sums,
 // one-workgroup scan of the partials, then per-slot prefixes; slot n gets the
 // total.  Used for the water prefix and the incremental sort.
 template <class V>
-__global__ __launch_bounds__(kBlock) void k_vsum(V v, int n, uint32_t* __restrict__ part) {
+SF_DEV void tile_sum(V v, int n, int tile, uint32_t* __restrict__ part_out) {
     __shared__ uint32_t ws[kBlock / 64];
     const int t = threadIdx.x;
-    const int base = blockIdx.x * kWpTile;
+    const int base = tile * kWpTile;
     uint32_t c = 0;
 #pragma unroll
     for (int q = 0; q < kWpItems; ++q) {
@@ -333,19 +360,19 @@ __global__ __launch_bounds__(kBlock) void k_vsum(V v, int n, uint32_t* __restric
     if ((t & 63) == 0) ws[t >> 6] = c;
     __syncthreads();
     if (t == 0) {
-        uint32_t s = 0;
-        for (int q = 0; q < kBlock / 64; ++q) s += ws[q];
-        part[blockIdx.x] = s;
+        uint32_t sum = 0;
+        for (int q = 0; q < kBlock / 64; ++q) sum += ws[q];
+        *part_out = sum;
     }
 }
-// part is exclusive-scanned; each block writes its slots' prefixes (thread t
-// owns kWpItems consecutive slots); slot n gets the total.
-template <class V>
-__global__ __launch_bounds__(kBlock) void k_vprefix(V v, int n, const uint32_t* __restrict__ part,
-                                                    uint32_t* __restrict__ out) {
+// tile's slot prefixes from its scanned partial (thread t owns kWpItems
+// consecutive slots); slot n (one past the end) receives the total.
+// emit(i, prefix, value) is called for every slot i < n.
+template <class V, class Emit>
+SF_DEV void tile_prefix(V v, int n, int tile, uint32_t part, uint32_t* __restrict__ out, Emit&& emit) {
     __shared__ uint32_t wsum[kBlock / 64];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int base = blockIdx.x * kWpTile + t * kWpItems;
+    const int base = tile * kWpTile + t * kWpItems;
     uint32_t f[kWpItems];
     uint32_t s = 0;
 #pragma unroll
@@ -362,23 +389,46 @@ __global__ __launch_bounds__(kBlock) void k_vprefix(V v, int n, const uint32_t* 
     }
     if (lane == 63) wsum[w] = x;
     __syncthreads();
-    uint32_t run = part[blockIdx.x] + x - s;
+    uint32_t run = part + x - s;
     for (int q = 0; q < w; ++q) run += wsum[q];
 #pragma unroll
     for (int q = 0; q < kWpItems; ++q) {
         const int i = base + q;
-        if (i <= n) out[i] = run;  // slot n (one past the end) receives the total
+        if (i <= n) out[i] = run;
+        if (i < n) emit(i, run, f[q]);
         run += f[q];
     }
 }
-template <class V>
-void launch_vprefix(V v, int n, uint32_t* part, uint32_t* out, hipStream_t s) {
+struct NoEmit {
+    SF_DEV void operator()(int, uint32_t, uint32_t) const {}
+};
+
+// Exclusive prefixes of two per-slot values in three launches: v1 over
+// [0, n1) into out1[0, n1], v2 over [0, n2) into out2[0, n2].  Blocks
+// [0, nb1) work on the first, [nb1, nb1 + nb2) on the second.
+template <class V1, class V2>
+__global__ __launch_bounds__(kBlock) void k_vsum2(V1 v1, int n1, int nb1, V2 v2, int n2, uint32_t* __restrict__ part) {
+    const int b = blockIdx.x;
+    if (b < nb1) tile_sum(v1, n1, b, part + b);
+    else tile_sum(v2, n2, b - nb1, part + b);
+}
+template <class V1, class V2>
+__global__ __launch_bounds__(kBlock) void k_vprefix2(V1 v1, int n1, int nb1, uint32_t* __restrict__ out1, V2 v2,
+                                                     int n2, uint32_t* __restrict__ out2,
+                                                     const uint32_t* __restrict__ part) {
+    const int b = blockIdx.x;
+    if (b < nb1) tile_prefix(v1, n1, b, part[b], out1, NoEmit{});
+    else tile_prefix(v2, n2, b - nb1, part[b], out2, NoEmit{});
+}
+int vprefix2_partials(int n1, int n2) { return wprefix_blocks(n1 + 1) + wprefix_blocks(n2 + 1); }
+template <class V1, class V2>
+void launch_vprefix2(V1 v1, int n1, uint32_t* out1, V2 v2, int n2, uint32_t* out2, uint32_t* part, hipStream_t s) {
     // blocks cover slots [0, n]; the block holding slot n counts nothing and
-    // receives the grand total from the scan
-    const int nb = wprefix_blocks(n + 1);
-    k_vsum<<<nb, kBlock, 0, s>>>(v, n, part);
-    k_rs_scan<<<1, 1024, 0, s>>>(part, nb);
-    k_vprefix<<<nb, kBlock, 0, s>>>(v, n, part, out);
+    // receives the total from the scan
+    const int nb1 = wprefix_blocks(n1 + 1), nb2 = wprefix_blocks(n2 + 1);
+    k_vsum2<<<nb1 + nb2, kBlock, 0, s>>>(v1, n1, nb1, v2, n2, part);
+    k_rs_scan2<<<1, 1024, 0, s>>>(part, nb1, nb2, nullptr);
+    k_vprefix2<<<nb1 + nb2, kBlock, 0, s>>>(v1, n1, nb1, out1, v2, n2, out2, part);
 }
 struct WaterVal {
     const float4* pm;
@@ -394,22 +444,24 @@ struct ArrayVal {
     SF_DEV uint32_t operator()(int i) const { return a[i]; }
 };
 
-void launch_wprefix(const float4* pm, int n, uint32_t* part, uint32_t* wpre, hipStream_t s) {
-    launch_vprefix(WaterVal{pm}, n, part, wpre, s);
+// Water prefix of the gathered slots (k_gather left the tile counts in
+// wpart) and the slots grouped by phase (water first, each group in slot
+// order) so that a pass over one phase can give every XCD an equal
+// contiguous share of it.
+__global__ __launch_bounds__(kBlock) void k_wprefix_plist(const float4* __restrict__ pm, int n,
+                                                          const uint32_t* __restrict__ wpart, uint32_t* __restrict__ wpre,
+                                                          uint32_t* __restrict__ plist) {
+    const uint32_t nwater = wpart[gridDim.x];
+    tile_prefix(WaterVal{pm}, n, (int)blockIdx.x, wpart[blockIdx.x], wpre, [&](int i, uint32_t pre, uint32_t water) {
+        plist[water ? pre : nwater + (uint32_t)i - pre] = (uint32_t)i;
+    });
 }
-
-// Slots grouped by phase (water first), each group in slot order, so a pass
-// over one phase can give every XCD an equal contiguous share of it.
-__global__ __launch_bounds__(kBlock) void k_phase_list(const float4* __restrict__ pm, const uint32_t* __restrict__ wpre,
-                                                       int n, uint32_t* __restrict__ plist) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t w = wpre[i];
-    plist[pm[i].w < 0.f ? wpre[n] + (uint32_t)i - w : w] = (uint32_t)i;
-}
-void launch_phase_list(const float4* pm, const uint32_t* wpre, int n, uint32_t* plist, hipStream_t s) {
-    if (n <= 0) return;
-    k_phase_list<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(pm, wpre, n, plist);
+void launch_gather_wprefix(const GatherArgs& a, int n, uint32_t* wpart, uint32_t* wpre, uint32_t* plist,
+                           hipStream_t s) {
+    const int nb = wprefix_blocks(n + 1);
+    k_gather<<<nb, kBlock, 0, s>>>(a, n, wpart);
+    k_rs_scan2<<<1, 1024, 0, s>>>(wpart, nb, 0, wpart + nb);
+    k_wprefix_plist<<<nb, kBlock, 0, s>>>(a.pm_out, n, wpart, wpre, plist);
 }
 
 // ---------------------------------------------------------------------------
@@ -423,10 +475,6 @@ void launch_phase_list(const float4* pm, const uint32_t* wpre, int n, uint32_t* 
 // few particles whose key changed to c (collected per cell and ordered by
 // index).  newStart(c) = stayers in cells < c + newcomers in cells < c.
 // The result is the stable sort's, element for element.
-__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ a, uint32_t v, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) a[i] = v;
-}
 __global__ __launch_bounds__(kBlock) void k_inc_flag(const uint32_t* __restrict__ key, const uint32_t* __restrict__ old,
                                                      int n, uint32_t* __restrict__ in_count,
                                                      uint32_t* __restrict__ slot) {
@@ -490,11 +538,9 @@ void launch_incremental_sort(const uint32_t* key, const uint32_t* old_key, const
                              hipStream_t s) {
     if (n <= 0) return;
     const int gn = (n + kBlock - 1) / kBlock;
-    const int gc = (int)((C + kBlock) / kBlock);
-    k_fill_u32<<<gc, kBlock, 0, s>>>(b.in_count, 0u, (int)C);
+    // in_count is zero here: the cell-start kernel after every sort clears it
     k_inc_flag<<<gn, kBlock, 0, s>>>(key, old_key, n, b.in_count, b.slot);
-    launch_vprefix(StayVal{key, old_key}, n, b.part, b.stay_pre, s);
-    launch_vprefix(ArrayVal{b.in_count}, (int)C, b.part, b.in_start, s);
+    launch_vprefix2(StayVal{key, old_key}, n, b.stay_pre, ArrayVal{b.in_count}, (int)C, b.in_start, b.part, s);
     k_inc_fill<<<gn, kBlock, 0, s>>>(key, old_key, n, b.in_start, b.slot, b.in_list);
     k_inc_order<<<(int)((C + kBlock - 1) / kBlock), kBlock, 0, s>>>(b.in_start, C, b.in_list);
     k_inc_place<<<gn, kBlock, 0, s>>>(key, old_key, n, old_start, b.stay_pre, b.in_start, b.in_list, perm,
