@@ -143,10 +143,13 @@ SF_DEV T lane_value(T v, int lane) {
 // lane).  need(nw, ns), given the water / sand counts of a column's union
 // range, may skip the whole column.  Must be called by all lanes of the wave
 // (uniform control flow).
-template <class Prep, class Need, class Fn>
+// With L > 1 lanes per particle (adjacent lanes, same cell), lane sub of a
+// particle takes the window's slots j = first + sub, first + sub + L, ...
+template <int L = 1, class Prep, class Need, class Fn>
 SF_DEV void staged_walk(const SFConst& k, const Fields& f, float4* __restrict__ stage, bool active, uint32_t cb,
                         int cx, int cy, int cz, Prep&& prep, Need&& need, Fn&& fn) {
     const int lane = (int)(threadIdx.x & 63u);
+    const uint32_t sub = (uint32_t)lane & (uint32_t)(L - 1);
     const uint32_t col = cb + (uint32_t)((cx * k.gy + cy) * k.gz);
     const int zlo = cz > 0 ? cz - 1 : 0;
     const int zhi = cz + 1 < k.gz ? cz + 1 : k.gz - 1;
@@ -181,16 +184,17 @@ SF_DEV void staged_walk(const SFConst& k, const Fields& f, float4* __restrict__ 
                     for (uint32_t q = (uint32_t)lane; q < m; q += 64u) stage[q] = prep(ldo(f.pm, u + q));
                     wave_sync();
                     const uint32_t wlo = a.x > u ? a.x : u, whi = b.x < u + m ? b.x : u + m;
-                    uint32_t j = wlo;
-                    for (; j + 4u <= whi; j += 4u) {
-                        const float4 q0 = stage[j - u], q1 = stage[j + 1u - u],
-                                     q2 = stage[j + 2u - u], q3 = stage[j + 3u - u];
+                    constexpr uint32_t S = (uint32_t)L;
+                    uint32_t j = wlo + sub;
+                    for (; j + 3u * S < whi; j += 4u * S) {
+                        const float4 q0 = stage[j - u], q1 = stage[j + S - u], q2 = stage[j + 2u * S - u],
+                                     q3 = stage[j + 3u * S - u];
                         fn(j, q0);
-                        fn(j + 1u, q1);
-                        fn(j + 2u, q2);
-                        fn(j + 3u, q3);
+                        fn(j + S, q1);
+                        fn(j + 2u * S, q2);
+                        fn(j + 3u * S, q3);
                     }
-                    for (; j < whi; ++j) fn(j, stage[j - u]);
+                    for (; j < whi; j += S) fn(j, stage[j - u]);
                     wave_sync();
                 }
             }
@@ -221,18 +225,22 @@ void dispatch(const Fields& f, PassCfg c, L&& l) {
 // P1: water density (self counted twice, :466 + loop) and pressure; sand gets
 // rho_sand and p = 0.  Branch-free: max(h^2 - r^2, 0) vanishes outside the
 // support and max(w, 0) drops sand (w = -mass).
+constexpr int kDenL = 2;  // lanes per water particle in the density pass
 template <bool B, int NB>
 __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
     __shared__ float4 stage_lds[(NB / 64) * kStage];
     float4* stage = stage_lds + (threadIdx.x >> 6) * kStage;
+    constexpr uint32_t PB = NB / kDenL;  // water particles per block
     const uint32_t nw = *f.nwater, ns = (uint32_t)n - nw;
-    const PhaseBlock pb = phase_block((nw + NB - 1) / NB, (ns + NB - 1) / NB);
+    const PhaseBlock pb = phase_block((nw + PB - 1) / PB, (ns + NB - 1) / NB);
     if (pb.phase < 0) return;
-    const uint32_t t = pb.blk * NB + threadIdx.x;
     if (pb.phase == 1) {  // sand: rho_sand, p = 0
+        const uint32_t t = pb.blk * NB + threadIdx.x;
         if (t < ns) f.rp[ldo(f.plist, nw + t)] = make_float2(k.rho_s, 0.f);
         return;
     }
+    const uint32_t t = pb.blk * PB + threadIdx.x / kDenL;
+    const bool lead = (threadIdx.x % kDenL) == 0;
     const bool water = t < nw;
     const int i = water ? (int)ldo(f.plist, t) : 0;
     const float4 pi = water ? ldo(f.pm, (uint32_t)i) : make_float4(0.f, 0.f, 0.f, -1.f);
@@ -240,7 +248,7 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
     int cx, cy, cz;
     cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
     float sf = 0.f, sb = 0.f;
-    staged_walk(
+    staged_walk<kDenL>(
         k, f, stage, water, cb, cx, cy, cz,
         [](float4 p) {
             p.w = fmaxf(p.w, 0.f);
@@ -252,7 +260,9 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
             const float d = fmaxf(k.h2 - (dx * dx + dy * dy + dz * dz), 0.f);
             sf += pj.w * sq3(d);
         });
-    if (!water) return;
+#pragma unroll
+    for (int o = 1; o < kDenL; o <<= 1) sf += __shfl_xor(sf, o);
+    if (!water || !lead) return;
     for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
         const uint32_t be = ldo(f.bcell, c1);
         for (uint32_t j = ldo(f.bcell, c0); j < be; ++j) {
@@ -268,7 +278,7 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
 void launch_density(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
     if (n <= 0) return;
     dispatch(f, c, [&](auto B, auto NB, auto) {
-        k_density<decltype(B)::value, decltype(NB)::value><<<phase_grid(n, NB), (int)NB, 0, s>>>(k, f, n);
+        k_density<decltype(B)::value, decltype(NB)::value><<<phase_grid(n * kDenL, NB), (int)NB, 0, s>>>(k, f, n);
     });
 }
 
