@@ -99,6 +99,20 @@ SF_DEV void for_range4(const float4* __restrict__ pm, uint32_t s, uint32_t e, F&
     for (; j < e; ++j) fn(j, ldo(pm, j));
 }
 
+// Every S-th slot of [s, e), four loads in flight.
+template <uint32_t S, class F>
+SF_DEV void for_range4s(const float4* __restrict__ pm, uint32_t s, uint32_t e, F&& fn) {
+    uint32_t j = s;
+    for (; j + 3u * S < e; j += 4u * S) {
+        const float4 a = ldo(pm, j), b = ldo(pm, j + S), c = ldo(pm, j + 2u * S), d = ldo(pm, j + 3u * S);
+        fn(j, a);
+        fn(j + S, b);
+        fn(j + 2u * S, c);
+        fn(j + 3u * S, d);
+    }
+    for (; j < e; j += S) fn(j, ldo(pm, j));
+}
+
 // column ranges plus the number of water particles in each: the cell table
 // holds (cellStart[c], water prefix at cellStart[c]) so one 8-byte load per
 // range end gives both.
@@ -558,13 +572,18 @@ void launch_sand_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, h
 
 // ---------------------------------------------------------------------------
 // P5: water linear momentum (pressure + artificial viscosity + seepage +
-// boundary pressure/viscosity), acc = g + a - F_adh / m.
+// boundary pressure/viscosity), acc = g + a - F_adh / m.  kWmL adjacent lanes
+// share a particle: each filters every kWmL-th candidate of the ranges into
+// its own list, and the partial sums are combined with shuffles.
+constexpr int kWmL = 4;
 template <bool B, int NB, int CAP>
 __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int n) {
     SF_NLIST(L);
+    constexpr uint32_t PB = NB / kWmL;  // water particles per block
     const uint32_t nw = *f.nwater;
-    const PhaseBlock pb = phase_block((nw + NB - 1) / NB, 0u);
-    const uint32_t t = pb.blk * NB + threadIdx.x;
+    const PhaseBlock pb = phase_block((nw + PB - 1) / PB, 0u);
+    const uint32_t t = pb.blk * PB + threadIdx.x / kWmL;
+    const uint32_t sub = threadIdx.x % kWmL;  // this lane's share of the particle's candidates
     if (pb.phase < 0 || t >= nw) return;
     const int i = (int)ldo(f.plist, t);
     const float4 pi = ldo(f.pm, (uint32_t)i);
@@ -594,8 +613,8 @@ __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int 
         const float r2 = dot(d, d);
         const float rs = frsq(r2);
         const float r = r2 * rs;
-        const float t = k.h - r;
-        const float gs = r >= kEps ? k.g_coef * t * t * rs : 0.f;  // spiky gradient / r
+        const float hr = k.h - r;
+        const float gs = r >= kEps ? k.g_coef * hr * hr * rs : 0.f;  // spiky gradient / r
         if (pj.w > 0.f) {  // water j: ComputeSFWaterPressureTerm + ArtificialViscosity
             const float2 wtj = q.wtj;
             float s = -pj.w * (wti.y + wtj.y);
@@ -613,7 +632,7 @@ __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int 
         }
     };
     for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t, uint32_t) {
-        for_range4(f.pm, s, e, [&](uint32_t j, const float4& pj) {
+        for_range4s<kWmL>(f.pm, s + sub, e, [&](uint32_t j, const float4& pj) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             nl_push2(L, r2 < k.h2 && (int)j != i, j, fetch, pair);
@@ -622,15 +641,15 @@ __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int 
     nl_flush2(L, fetch, pair);
     for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
         const uint32_t be = ldo(f.bcell, c1);
-        for (uint32_t j = ldo(f.bcell, c0); j < be; ++j) {
+        for (uint32_t j = ldo(f.bcell, c0) + sub; j < be; j += kWmL) {
             const float4 b = ldo(f.bpv, j);
             const v3 d = mk3(pi.x - b.x, pi.y - b.y, pi.z - b.z);
             const float r2 = dot(d, d);
             if (r2 >= k.h2) continue;
             const float rs = frsq(r2);
             const float r = r2 * rs;
-            const float t = k.h - r;
-            const float gs = r >= kEps ? k.g_coef * t * t * rs : 0.f;
+            const float hr = k.h - r;
+            const float gs = r >= kEps ? k.g_coef * hr * hr * rs : 0.f;
             const float vb = fabsf(b.w);
             float s = vb * bpress;  // ComputeBoundaryPressure (densityi = rho0, :944)
             const float dd = dot(vi, d);
@@ -638,6 +657,13 @@ __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int 
             a += d * (s * gs);
         }
     });
+#pragma unroll
+    for (int o = 1; o < kWmL; o <<= 1) {
+        a.x += __shfl_xor(a.x, o);
+        a.y += __shfl_xor(a.y, o);
+        a.z += __shfl_xor(a.z, o);
+    }
+    if (sub != 0) return;
     const v3 ad = xyz(ldo(f.adh, (uint32_t)i));
     const v3 acc = mk3(k.gvx, k.gvy, k.gvz) + a - ad * (1.f / mi);
     f.acc[i] = f4(acc, 0.f);
@@ -646,7 +672,7 @@ void launch_water_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, 
     if (n <= 0) return;
     dispatch(f, c, [&](auto B, auto NB, auto CAP) {
         k_water_momentum<decltype(B)::value, decltype(NB)::value, decltype(CAP)::value>
-            <<<phase_grid(n, NB), (int)NB, 0, s>>>(k, f, n);
+            <<<phase_grid(n * kWmL, NB), (int)NB, 0, s>>>(k, f, n);
     });
 }
 
