@@ -424,12 +424,15 @@ SF_DEV void near_span(float p, float lo, float h, int c, int g, float rc, int& a
     b = (c < g - 1 && cl + h - p < rc) ? c + 1 : c;
 }
 
+constexpr int kSmL = 2;  // lanes per sand particle (as in the water momentum pass)
 template <bool B, int NB, int CAP>
 __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n) {
     SF_NLIST(L);
+    constexpr uint32_t PB = NB / kSmL;  // sand particles per block
     const uint32_t nw = *f.nwater, ns = (uint32_t)n - nw;
-    const PhaseBlock pb = phase_block(0u, (ns + NB - 1) / NB);
-    const uint32_t t = pb.blk * NB + threadIdx.x;
+    const PhaseBlock pb = phase_block(0u, (ns + PB - 1) / PB);
+    const uint32_t t = pb.blk * PB + threadIdx.x / kSmL;
+    const uint32_t sub = threadIdx.x % kSmL;  // this lane's share of the particle's candidates
     if (pb.phase < 0 || t >= ns) return;
     const int i = (int)ldo(f.plist, nw + t);
     const float4 pi = ldo(f.pm, (uint32_t)i);
@@ -484,7 +487,7 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
             for (int y = ya; y <= yb; ++y) {
                 const uint32_t base = cb + (uint32_t)((x * k.gy + y) * k.gz);
                 const uint32_t s0 = ldo(f.ct, base + (uint32_t)za).x, e0 = ldo(f.ct, base + (uint32_t)zb + 1u).x;
-                for_range4(f.pm, s0, e0, [&](uint32_t j, const float4& pj) {
+                for_range4s<kSmL>(f.pm, s0 + sub, e0, [&](uint32_t j, const float4& pj) {
                     const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
                     const float r2 = dx * dx + dy * dy + dz * dz;
                     nl_push2(L, pj.w < 0.f && (int)j != i && r2 < cap_r2, j, fetch_sand, contact);
@@ -519,7 +522,7 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
     };
     for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t) {
         if (nw == 0) return;
-        for_range4(f.pm, s, e, [&](uint32_t j, const float4& pj) {
+        for_range4s<kSmL>(f.pm, s + sub, e, [&](uint32_t j, const float4& pj) {
             const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             nl_push2(L, pj.w > 0.f && r2 < k.h2, j, fetch_water, water);
@@ -529,13 +532,21 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
     if (k.blabel1) {  // ComputeSFUniRadiusDemBoundaryForces: boundary label 1 only
         for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
             const uint32_t be = ldo(f.bcell, c1);
-            for (uint32_t j = ldo(f.bcell, c0); j < be; ++j) {
+            for (uint32_t j = ldo(f.bcell, c0) + sub; j < be; j += kSmL) {
                 const float4 b = ldo(f.bpv, j);
                 if (b.w >= 0.f) continue;
                 fsum += dem_force(xi - xyz(b), -vi, twoR, k.kn, k.ks, k.tan2);
             }
         });
     }
+#pragma unroll
+    for (int o = 1; o < kSmL; o <<= 1) {
+        fsum.x += __shfl_xor(fsum.x, o);
+        fsum.y += __shfl_xor(fsum.y, o);
+        fsum.z += __shfl_xor(fsum.z, o);
+        vw += __shfl_xor(vw, o);
+    }
+    if (sub != 0) return;
     fsum += world_walls(k, xi, vi);
     v3 df = mk3(0.f, 0.f, 0.f);
     const float eps_i = esi.x;
@@ -566,7 +577,7 @@ void launch_sand_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, h
     if (n <= 0) return;
     dispatch(f, c, [&](auto B, auto NB, auto CAP) {
         k_sand_momentum<decltype(B)::value, decltype(NB)::value, decltype(CAP)::value>
-            <<<phase_grid(n, NB), (int)NB, 0, s>>>(k, f, n);
+            <<<phase_grid(n * kSmL, NB), (int)NB, 0, s>>>(k, f, n);
     });
 }
 
