@@ -43,9 +43,10 @@ SF_DEV void add3(float4& a, float x, float y, float z) {
 SF_DEV void add3(float4& a, v3 b) { add3(a, b.x, b.y, b.z); }
 
 // Lanes per particle: L consecutive lanes share one particle; lane `sub`
-// walks the neighbour columns q with q % L == sub, and the partial sums are
-// combined by a fixed butterfly (deterministic for a given L).  Small
-// windows get L > 1 so that their few thousand waves hide load latency.
+// takes every L-th slot (from the sub-th) of each neighbour range, and the
+// partial sums are combined by a fixed butterfly (deterministic for a given
+// L).  Small windows get L > 1 so that their few thousand waves hide load
+// latency.
 template <int L>
 SF_DEV float lsum(float x) {
     if constexpr (L >= 2) x += __shfl_xor(x, 1);
@@ -67,24 +68,38 @@ template <int L, class F>
 SF_DEV void neighbours(const SFConst& k, const TapeStep& t, float4 p, int sub, int need, F&& fn) {
     int cx, cy, cz;
     cell_of(k, p.x, p.y, p.z, cx, cy, cz);
-    int q = 0;
     for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
-        if ((q++ % L) != sub) return;
         const uint2 a = t.ct[c0], b = t.ct[c1];
         const uint32_t nw = b.y - a.y, ns = (b.x - a.x) - nw;
         if (!(((need & kNeedWater) && nw) || ((need & kNeedSand) && ns))) return;
-        for (uint32_t j = a.x; j < b.x; ++j) fn(j);
+        for (uint32_t j = a.x + (uint32_t)sub; j < b.x; j += (uint32_t)L) fn(j);
     });
+}
+// The sand particles within reach of a sand-sand force (DEM overlap or liquid
+// bridge, r < 2R + s_rupture < h / 2): along each axis only the own cell and
+// the neighbour on the nearer side (the forward pass's near span).
+template <int L, class F>
+SF_DEV void near_neighbours(const SFConst& k, const TapeStep& t, float4 p, int sub, float rc, F&& fn) {
+    int cx, cy, cz;
+    cell_of(k, p.x, p.y, p.z, cx, cy, cz);
+    int xa, xb, ya, yb, za, zb;
+    near_span(p.x, k.lox, k.h, cx, k.gx, rc, xa, xb);
+    near_span(p.y, k.loy, k.h, cy, k.gy, rc, ya, yb);
+    near_span(p.z, k.loz, k.h, cz, k.gz, rc, za, zb);
+    for (int x = xa; x <= xb; ++x)
+        for (int y = ya; y <= yb; ++y) {
+            const uint32_t base = (uint32_t)((x * k.gy + y) * k.gz);
+            const uint32_t e = t.ct[base + (uint32_t)zb + 1u].x;
+            for (uint32_t j = t.ct[base + (uint32_t)za].x + (uint32_t)sub; j < e; j += (uint32_t)L) fn(j);
+        }
 }
 template <int L, class F>
 SF_DEV void boundary_neighbours(const SFConst& k, const TapeStep& t, float4 p, int sub, F&& fn) {
     int cx, cy, cz;
     cell_of(k, p.x, p.y, p.z, cx, cy, cz);
-    int q = 0;
     for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
-        if ((q++ % L) != sub) return;
         const uint32_t e = t.bcell[c1];
-        for (uint32_t j = t.bcell[c0]; j < e; ++j) fn(j);
+        for (uint32_t j = t.bcell[c0] + (uint32_t)sub; j < e; j += (uint32_t)L) fn(j);
     });
 }
 
@@ -325,7 +340,8 @@ __global__ __launch_bounds__(kAB) void k_adj_sand_pairs(SFConst k, TapeStep t, A
     const v3 fk3 = mk3(ok.x, ok.y, ok.z);
     const float sat_k = t.es[i].y;
     const float rho_k = t.rp[i].x;
-    neighbours<L>(k, t, pk, sub, sand ? kNeedBoth : kNeedSand, [&](uint32_t j) {
+    const float cap_r = twoR + k.cap_srup;
+    auto pair = [&](uint32_t j) {
         if ((int)j == i) return;
         const float4 pj = t.pm[j];
         const float dx = pj.x - pk.x, dy = pj.y - pk.y, dz = pj.z - pk.z;  // x_j - x_k
@@ -333,8 +349,9 @@ __global__ __launch_bounds__(kAB) void k_adj_sand_pairs(SFConst k, TapeStep t, A
         if (sand && pj.w < 0.f) {
             // sand-sand: DEM and bridge terms are antisymmetric, T(j <- k) = -T(k <- j),
             // so one Jacobian of T(k <- j) serves both sides (contracted with f_j - f_k)
+            if (r2 >= cap_r * cap_r) return;
             const float dist = sqrtf(r2);
-            if (dist >= twoR + k.cap_srup) return;
+            if (dist >= cap_r) return;
             const float4 vj = t.vr[j];
             const float4 oj = a.o4[j];
             const float3 dfb = make_float3(oj.x - fk.x, oj.y - fk.y, oj.z - fk.z);
@@ -385,7 +402,18 @@ __global__ __launch_bounds__(kAB) void k_adj_sand_pairs(SFConst k, TapeStep t, A
             add3(gx, gd);
             add3(gadh, ga);
         }
-    });
+    };
+    if (sand) {
+        // sand partners only within the near span; water partners in ranges holding water
+        near_neighbours<L>(k, t, pk, sub, cap_r + 1e-3f * k.h, [&](uint32_t j) {
+            if (t.pm[j].w < 0.f) pair(j);
+        });
+        neighbours<L>(k, t, pk, sub, kNeedWater, [&](uint32_t j) {
+            if (t.pm[j].w > 0.f) pair(j);
+        });
+    } else {
+        neighbours<L>(k, t, pk, sub, kNeedSand, pair);
+    }
     gx = lsum<L>(gx);
     gv = lsum<L>(gv);
     gth = lsum<L>(gth);

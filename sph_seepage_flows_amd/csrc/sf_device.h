@@ -128,6 +128,14 @@ SF_DEV void for_columns(const SFConst& k, int cx, int cy, int cz, F&& f) {
     }
 }
 
+// Cells along one axis that can hold a partner within rc < h / 2 of
+// coordinate p in cell c: the own cell and the neighbour on the nearer side.
+SF_DEV void near_span(float p, float lo, float h, int c, int g, float rc, int& a, int& b) {
+    const float cl = lo + (float)c * h;
+    a = (c > 0 && p - cl < rc) ? c - 1 : c;
+    b = (c < g - 1 && cl + h - p < rc) ? c + 1 : c;
+}
+
 // ---- SPH kernels on r^2 (Poly6 cuda_sph_kernel.cuh:79-98, Spiky :100-118) ----
 SF_DEV float poly6_r2(const SFConst& k, float r2) {
     if (r2 >= k.h2) return 0.f;

@@ -418,11 +418,6 @@ void launch_voidage_adhesion(const SFConst& k, const Fields& f, int n, PassCfg c
 // the bridge H < s_rupture), and r_c < h / 2, so along each axis a grain only
 // reaches its own cell and the one neighbour on its nearer side: the contact
 // search walks 2 x 2 x 2 cells instead of 27 (same non-zero terms).
-SF_DEV void near_span(float p, float lo, float h, int c, int g, float rc, int& a, int& b) {
-    const float cl = lo + (float)c * h;
-    a = (c > 0 && p - cl < rc) ? c - 1 : c;
-    b = (c < g - 1 && cl + h - p < rc) ? c + 1 : c;
-}
 
 constexpr int kSmL = 2;  // lanes per sand particle (as in the water momentum pass)
 template <bool B, int NB, int CAP>
