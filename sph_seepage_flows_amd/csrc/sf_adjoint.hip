@@ -75,23 +75,41 @@ SF_DEV void neighbours(const SFConst& k, const TapeStep& t, float4 p, int sub, i
         for (uint32_t j = a.x + (uint32_t)sub; j < b.x; j += (uint32_t)L) fn(j);
     });
 }
-// The sand particles within reach of a sand-sand force (DEM overlap or liquid
-// bridge, r < 2R + s_rupture < h / 2): along each axis only the own cell and
-// the neighbour on the nearer side (the forward pass's near span).
+// Partners of a sand particle: water within h and sand within reach of a
+// sand-sand force (DEM overlap or liquid bridge, r < rc = 2R + s_rupture <= h).
+// A column holding water is walked whole; a sand-only column only if it lies
+// in the near span (along each axis the own cell and the neighbour on the
+// nearer side, the forward pass's contact search), and only over its near
+// cells.
 template <int L, class F>
-SF_DEV void near_neighbours(const SFConst& k, const TapeStep& t, float4 p, int sub, float rc, F&& fn) {
+SF_DEV void sand_partners(const SFConst& k, const TapeStep& t, float4 p, int sub, float rc, F&& fn) {
     int cx, cy, cz;
     cell_of(k, p.x, p.y, p.z, cx, cy, cz);
     int xa, xb, ya, yb, za, zb;
     near_span(p.x, k.lox, k.h, cx, k.gx, rc, xa, xb);
     near_span(p.y, k.loy, k.h, cy, k.gy, rc, ya, yb);
     near_span(p.z, k.loz, k.h, cz, k.gz, rc, za, zb);
-    for (int x = xa; x <= xb; ++x)
-        for (int y = ya; y <= yb; ++y) {
+    const int zlo = cz > 0 ? cz - 1 : 0;
+    const int zhi = cz + 1 < k.gz ? cz + 1 : k.gz - 1;
+    for (int x = cx - 1; x <= cx + 1; ++x) {
+        if (x < 0 || x >= k.gx) continue;
+        for (int y = cy - 1; y <= cy + 1; ++y) {
+            if (y < 0 || y >= k.gy) continue;
             const uint32_t base = (uint32_t)((x * k.gy + y) * k.gz);
-            const uint32_t e = t.ct[base + (uint32_t)zb + 1u].x;
-            for (uint32_t j = t.ct[base + (uint32_t)za].x + (uint32_t)sub; j < e; j += (uint32_t)L) fn(j);
+            const uint2 a = t.ct[base + (uint32_t)zlo], b = t.ct[base + (uint32_t)zhi + 1u];
+            uint32_t s0, e0;
+            if (b.y != a.y) {  // water in the column
+                s0 = a.x;
+                e0 = b.x;
+            } else if (x >= xa && x <= xb && y >= ya && y <= yb) {
+                s0 = t.ct[base + (uint32_t)za].x;
+                e0 = t.ct[base + (uint32_t)zb + 1u].x;
+            } else {
+                continue;
+            }
+            for (uint32_t j = s0 + (uint32_t)sub; j < e0; j += (uint32_t)L) fn(j);
         }
+    }
 }
 template <int L, class F>
 SF_DEV void boundary_neighbours(const SFConst& k, const TapeStep& t, float4 p, int sub, F&& fn) {
@@ -404,13 +422,7 @@ __global__ __launch_bounds__(kAB) void k_adj_sand_pairs(SFConst k, TapeStep t, A
         }
     };
     if (sand) {
-        // sand partners only within the near span; water partners in ranges holding water
-        near_neighbours<L>(k, t, pk, sub, cap_r + 1e-3f * k.h, [&](uint32_t j) {
-            if (t.pm[j].w < 0.f) pair(j);
-        });
-        neighbours<L>(k, t, pk, sub, kNeedWater, [&](uint32_t j) {
-            if (t.pm[j].w > 0.f) pair(j);
-        });
+        sand_partners<L>(k, t, pk, sub, cap_r + 1e-3f * k.h, pair);
     } else {
         neighbours<L>(k, t, pk, sub, kNeedSand, pair);
     }
