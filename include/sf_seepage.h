@@ -28,7 +28,7 @@
  *   sf_build_world_boundary <- CudaBoundaryEmitter::BuildWorldBoundary  cuda_boundary_emitter.cpp:12-80
  *   sf_grid_size         <- make_int3((highest - lowest) / kernel_radius)  main.cpp:104, cuda_neighbor_searcher.cu:26
  *   sf_batch_*, sf_step_graph, sf_*_launch_config, sf_autotune, sf_set_field, sf_diff_*,
- *   sf_set_sort_strategy:
+ *   sf_set_sort_strategy, sf_set_interface_lanes:
  *                           no counterpart (see their declarations)
  *
  * Threading: a handle is single-threaded; all its GPU work is ordered on the
@@ -274,6 +274,13 @@ int sf_set_sort_strategy(sf_handle h, int strategy);
  * (sand / water momentum only).  Results do not depend on the shape. */
 int sf_set_launch_config(sf_handle h, int pass, int nb, int cap);
 int sf_get_launch_config(sf_handle h, int pass, int* nb, int* cap);
+/* Lanes per particle (1, 2 or 4; default 1) of the two interface passes
+ * (SF_PASS_AVG_FLOW, SF_PASS_VOIDAGE_ADHESION), whose neighbour sums run only
+ * where water and sand meet.  Several lanes per particle speed up small or
+ * well-mixed scenes; they regroup those sums, so results change at rounding
+ * level, and handles that must agree bit for bit use the same value.  The
+ * autotuner keeps it. */
+int sf_set_interface_lanes(sf_handle h, int lanes);
 /* Times every shape of every neighbour pass on the handle's current particles
  * (reps launches each, after one neighbour build) and keeps the fastest.
  * best_ms / default_ms (5 floats each, may be NULL) receive per pass the

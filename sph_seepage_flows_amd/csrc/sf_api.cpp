@@ -939,7 +939,19 @@ int sf_set_launch_config(sf_handle h, int pass, int nb, int cap) {
     if (std::find(std::begin(kNbChoices), std::end(kNbChoices), nb) == std::end(kNbChoices) ||
         std::find(std::begin(kCapChoices), std::end(kCapChoices), cap) == std::end(kCapChoices))
         return set_error(SF_E_INVALID, "nb must be 64, 128 or 256 and cap 8 or 16");
-    h->cfg[pass - SF_PASS_DENSITY] = PassCfg{nb, cap};
+    PassCfg& c = h->cfg[pass - SF_PASS_DENSITY];
+    c.nb = nb;
+    c.cap = cap;
+    h->drop_graph();
+    return SF_OK;
+}
+
+int sf_set_interface_lanes(sf_handle h, int lanes) {
+    int rc;
+    if ((rc = check_handle(h))) return rc;
+    if (lanes != 1 && lanes != 2 && lanes != 4) return set_error(SF_E_INVALID, "lanes must be 1, 2 or 4");
+    h->cfg[SF_PASS_AVG_FLOW - SF_PASS_DENSITY].lanes = lanes;
+    h->cfg[SF_PASS_VOIDAGE_ADHESION - SF_PASS_DENSITY].lanes = lanes;
     h->drop_graph();
     return SF_OK;
 }
@@ -978,11 +990,11 @@ int sf_autotune(sf_handle h, int reps, float* best_ms, float* default_ms) {
     for (int q = 0; q < 5; ++q) {
         const bool lists = q >= 3;
         float best = 0.f, dflt = 0.f;
-        PassCfg pick{256, 16};
+        PassCfg pick{256, 16, h->cfg[q].lanes};
         for (int nb : kNbChoices)
             for (int cap : kCapChoices) {
                 if (!lists && cap != 16) continue;
-                const PassCfg c{nb, cap};
+                const PassCfg c{nb, cap, h->cfg[q].lanes};
                 launch_pass(h, q, c);  // warm-up
                 SF_HIP(hipEventRecord(h->ev0, h->stream));
                 for (int r = 0; r < reps; ++r) launch_pass(h, q, c);

@@ -160,6 +160,7 @@ struct Fields {
 struct PassCfg {
     int nb = 256;
     int cap = 16;
+    int lanes = 1;  // lanes per particle (interface passes P2, P3)
 };
 constexpr int kNbChoices[3] = {64, 128, 256};
 constexpr int kCapChoices[2] = {8, 16};

@@ -216,6 +216,25 @@ SF_DEV void staged_walk(const SFConst& k, const Fields& f, float4* __restrict__ 
     }
 }
 
+// Lanes per particle of the interface passes (P2, P3), whose work sits at the
+// water-sand interface (PassCfg::lanes): several lanes per particle let the
+// few thousand waves of a small handle hide load latency.
+template <class L>
+void with_lanes(int lanes, L&& l) {
+    switch (lanes) {
+    case 4: l(std::integral_constant<int, 4>{}); break;
+    case 2: l(std::integral_constant<int, 2>{}); break;
+    default: l(std::integral_constant<int, 1>{}); break;
+    }
+}
+// Sums x over the LP lanes of a particle (adjacent lanes, fixed butterfly).
+template <int LP>
+SF_DEV float lane_sum(float x) {
+#pragma unroll
+    for (int o = 1; o < LP; o <<= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
 // Host-side launch dispatch over (batched?, workgroup size, list capacity);
 // the choices are kNbChoices x kCapChoices.
 template <int V>
@@ -304,9 +323,10 @@ void launch_density(const SFConst& k, const Fields& f, int n, PassCfg c, hipStre
 // its sand neighbours: ranges without the other phase are skipped.)
 // Water particles also store wt = (rho / max(eps, voidage), p / max(eps, that^2)),
 // the per-particle factors the water momentum pass needs from every neighbour.
-template <bool B, int NB>
+template <bool B, int NB, int LP>
 __global__ __launch_bounds__(NB) void k_avg_flow(SFConst k, Fields f, int n) {
-    const int i = xcd_block() * NB + threadIdx.x;
+    const int i = (int)(xcd_block() * (NB / LP) + threadIdx.x / LP);
+    const uint32_t sub = threadIdx.x % LP;
     if (i >= n) return;
     const float4 pi = ldo(f.pm, (uint32_t)i);
     const bool sand_i = pi.w < 0.f;
@@ -318,7 +338,7 @@ __global__ __launch_bounds__(NB) void k_avg_flow(SFConst k, Fields f, int n) {
     const float inv_rho_s = k.rho_s != 0.f ? 1.f / k.rho_s : 0.f;
     for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t ns) {
         if ((sand_i ? nw : ns) == 0) return;
-        for (uint32_t j = s; j < e; ++j) {
+        for (uint32_t j = s + sub; j < e; j += LP) {
             const float4 pj = ldo(f.pm, j);
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
@@ -330,6 +350,10 @@ __global__ __launch_bounds__(NB) void k_avg_flow(SFConst k, Fields f, int n) {
             vsum += vw;
         }
     });
+    sandv = lane_sum<LP>(sandv);
+    vsum = lane_sum<LP>(vsum);
+    av = mk3(lane_sum<LP>(av.x), lane_sum<LP>(av.y), lane_sum<LP>(av.z));
+    if (sub != 0) return;
     const v3 fl = vsum != 0.f ? av * (1.f / vsum) : mk3(0.f, 0.f, 0.f);
     f.flow[i] = f4(fl, 0.f);
     if (f.den4) reinterpret_cast<float*>(f.den4)[4 * i] = vsum;
@@ -344,15 +368,19 @@ __global__ __launch_bounds__(NB) void k_avg_flow(SFConst k, Fields f, int n) {
 void launch_avg_flow(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
     if (n <= 0) return;
     dispatch(f, c, [&](auto B, auto NB, auto) {
-        k_avg_flow<decltype(B)::value, decltype(NB)::value><<<(n + NB - 1) / NB, (int)NB, 0, s>>>(k, f, n);
+        with_lanes(c.lanes, [&](auto LP) {
+            k_avg_flow<decltype(B)::value, decltype(NB)::value, decltype(LP)::value>
+                <<<(n * LP + NB - 1) / NB, (int)NB, 0, s>>>(k, f, n);
+        });
     });
 }
 
 // ---------------------------------------------------------------------------
 // P3: sand i -> voidage from water neighbours; water i -> adhesion force.
-template <bool B, int NB>
+template <bool B, int NB, int LP>
 __global__ __launch_bounds__(NB) void k_voidage_adhesion(SFConst k, Fields f, int n) {
-    const int i = xcd_block() * NB + threadIdx.x;
+    const int i = (int)(xcd_block() * (NB / LP) + threadIdx.x / LP);
+    const uint32_t sub = threadIdx.x % LP;
     if (i >= n) return;
     const float4 pi = ldo(f.pm, (uint32_t)i);
     const bool sand_i = pi.w < 0.f;
@@ -363,7 +391,7 @@ __global__ __launch_bounds__(NB) void k_voidage_adhesion(SFConst k, Fields f, in
         float num = 0.f, den = 0.f;
         for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t) {
             if (nw == 0) return;
-            for (uint32_t j = s; j < e; ++j) {
+            for (uint32_t j = s + sub; j < e; j += LP) {
                 const float4 pj = ldo(f.pm, j);
                 const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
                 const float r2 = dx * dx + dy * dy + dz * dz;
@@ -373,6 +401,9 @@ __global__ __launch_bounds__(NB) void k_voidage_adhesion(SFConst k, Fields f, in
                 den += vw;
             }
         });
+        num = lane_sum<LP>(num);
+        den = lane_sum<LP>(den);
+        if (sub != 0) return;
         f.es[i].x = den != 0.f ? num / den : 0.f;
         f.adh[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (f.den4) reinterpret_cast<float*>(f.den4)[4 * i + 1] = den;
@@ -384,7 +415,7 @@ __global__ __launch_bounds__(NB) void k_voidage_adhesion(SFConst k, Fields f, in
     v3 F = mk3(0.f, 0.f, 0.f);
     for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t, uint32_t ns) {
         if (ns == 0) return;
-        for (uint32_t j = s; j < e; ++j) {
+        for (uint32_t j = s + sub; j < e; j += LP) {
             const float4 pj = ldo(f.pm, j);
             const v3 dp = mk3(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
             const float r2 = dot(dp, dp);
@@ -399,6 +430,9 @@ __global__ __launch_bounds__(NB) void k_voidage_adhesion(SFConst k, Fields f, in
             }
         }
     });
+    vs = lane_sum<LP>(vs);
+    F = mk3(lane_sum<LP>(F.x), lane_sum<LP>(F.y), lane_sum<LP>(F.z));
+    if (sub != 0) return;
     f.adh[i] = f4(F, 0.f);
     f.xf[i] = vs != 0.f ? f4(F * (1.f / vs), 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
     if (f.den4) reinterpret_cast<float*>(f.den4)[4 * i + 1] = vs;
@@ -406,7 +440,10 @@ __global__ __launch_bounds__(NB) void k_voidage_adhesion(SFConst k, Fields f, in
 void launch_voidage_adhesion(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
     if (n <= 0) return;
     dispatch(f, c, [&](auto B, auto NB, auto) {
-        k_voidage_adhesion<decltype(B)::value, decltype(NB)::value><<<(n + NB - 1) / NB, (int)NB, 0, s>>>(k, f, n);
+        with_lanes(c.lanes, [&](auto LP) {
+            k_voidage_adhesion<decltype(B)::value, decltype(NB)::value, decltype(LP)::value>
+                <<<(n * LP + NB - 1) / NB, (int)NB, 0, s>>>(k, f, n);
+        });
     });
 }
 

@@ -97,6 +97,12 @@ class SFSystem:
         """Incremental stable sort (default) or the LSD radix sort every step."""
         check(lib().sf_set_sort_strategy(self._h, _abi.SF_SORT_INCREMENTAL if incremental else _abi.SF_SORT_RADIX))
 
+    def set_interface_lanes(self, lanes: int) -> None:
+        """Lanes per particle (1, 2 or 4) of the avg-flow and voidage/adhesion
+        passes (sf_set_interface_lanes).  More lanes speed up small or mixed
+        scenes; results change at rounding level."""
+        check(lib().sf_set_interface_lanes(self._h, int(lanes)))
+
     def step_graph(self, dt: float, n: int) -> None:
         """n fixed-dt steps replayed from a captured HIP graph (same results as step)."""
         check(lib().sf_step_graph(self._h, float(dt), int(n)))

@@ -103,6 +103,41 @@ def test_launch_shapes_give_identical_results(nb, cap):
         np.testing.assert_array_equal(a.get(name), b.get(name), err_msg=name)
 
 
+def test_interface_lanes_change_only_rounding():
+    """1, 2 and 4 lanes per particle in the interface passes regroup their
+    sums: after several steps of the mixed scene the states agree to FP32
+    rounding (relative to each field's magnitude), not bit for bit."""
+    sc = scenes.mixed()
+    ref = SFSystem.from_scene(sc)
+    ref.step(DT, 6)
+    for lanes in (2, 4):
+        s = SFSystem.from_scene(sc)
+        s.set_interface_lanes(lanes)
+        s.step(DT, 6)
+        for name in ("pos", "vel", "voidage", "saturation", "avg_drag_force"):
+            a, b = ref.get(name), s.get(name)
+            scale = max(float(np.abs(a).max()), 1e-30)
+            err = float(np.abs(a - b).max()) / scale
+            print(f"[lanes {lanes}] {name} {err:.2e}", flush=True)
+            assert err < 2e-5, (lanes, name, err)
+    with pytest.raises(SFError):
+        ref.set_interface_lanes(3)
+
+
+def test_batch_matches_single_scene_handles_with_interface_lanes():
+    E = small_ensemble(3, seed=17)
+    b = BatchSystem.from_scenes(E)
+    b.set_interface_lanes(4)
+    b.step(DT, 8)
+    for q, sc in enumerate(E):
+        s = SFSystem.from_scene(sc)
+        s.set_interface_lanes(4)
+        s.step(DT, 8)
+        for name in FIELDS:
+            np.testing.assert_array_equal(b.get_scene(name, q), s.get(name), err_msg=f"scene {q} {name}")
+        s.close()
+
+
 def test_scene_params_change_only_their_scene():
     E = small_ensemble(3, seed=11)
     a = BatchSystem.from_scenes(E)

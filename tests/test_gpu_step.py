@@ -65,6 +65,19 @@ def test_single_step_mixed(sf_type, oracle_mod):
         assert np.abs(o.get(f)).max() > 0, f
 
 
+@pytest.mark.parametrize("scene", ["mixed", "small_block"])
+def test_single_step_interface_lanes(scene, oracle_mod):
+    """The avg-flow and voidage/adhesion passes with 4 lanes per particle
+    (sf_set_interface_lanes) meet the same single-step tolerances."""
+    sc = scenes.mixed() if scene == "mixed" else scenes.small_block()
+    g = SFSystem.from_scene(sc)
+    g.set_interface_lanes(4)
+    o = oracle_mod.OracleSystem(sc, 64)
+    g.step(1e-4)
+    o.step(1e-4)
+    _compare(g, o, f"{scene} lanes=4")
+
+
 def test_single_step_small_block(oracle_mod):
     sc = scenes.small_block()
     g = SFSystem.from_scene(sc)
