@@ -87,6 +87,67 @@ def test_single_step_small_block(oracle_mod):
     _compare(g, o, "small_block")
 
 
+def _lattice_scene(name, blocks, box=0.5):
+    """Scene of lattice blocks [(origin, spacing, counts, label)] with the
+    mixed scene's physics (r = 0.01, h = 0.04) in a cubic box."""
+    from sph_seepage_flows_amd import params as P
+    ph = P.Physics(hi=(box, box, box))
+    parts = []
+    for q, (origin, spacing, counts, label) in enumerate(blocks):
+        n = counts[0] * counts[1] * counts[2]
+        pos = scenes.lattice(origin, spacing, counts, n, jitter=0.1 * spacing, seed=21, stream=q)
+        parts.append((pos, label))
+    return scenes._assemble(name, ph, parts)
+
+
+def test_single_step_dense_clump(oracle_mod):
+    """Water and sand packed at h/4 (64 particles per cell): a wave's union of
+    three cells in a neighbour column holds ~190 candidates, more than the
+    density pass's 128-slot LDS stage, so the staged walk runs in chunks."""
+    sc = _lattice_scene("dense_clump", [((0.2, 0.2, 0.2), 0.01, (10, 10, 10), 0),
+                                        ((0.3, 0.2, 0.2), 0.01, (6, 10, 10), 1)])
+    g = SFSystem.from_scene(sc)
+    o = oracle_mod.OracleSystem(sc, 64)
+    g.step(1e-4)
+    o.step(1e-4)
+    counts = np.diff(g.get("cell_start").astype(np.int64))
+    assert counts.max() >= 43, counts.max()  # 3 cells of a column exceed 128
+    _compare(g, o, "dense_clump")
+
+
+@pytest.mark.parametrize("nw,ns", [(1, 0), (0, 1), (63, 2), (64, 0), (0, 65), (255, 3), (257, 256), (2, 511)])
+def test_single_step_phase_counts(nw, ns, oracle_mod):
+    """Phase-balanced launches at edge counts: one phase empty, counts around
+    the 64-lane wave and 256-thread block, one phase of a single particle."""
+    blocks = []
+    if nw:
+        blocks.append(((0.1, 0.1, 0.1), 0.02, (nw, 1, 1) if nw < 8 else (8, 8, (nw + 63) // 64), 0))
+    if ns:
+        blocks.append(((0.12, 0.11, 0.1), 0.02, (ns, 1, 1) if ns < 8 else (8, 8, (ns + 63) // 64), 1))
+    sc = _lattice_scene(f"phases_{nw}_{ns}", blocks)
+    n_water_built = int((sc.label == 0).sum())  # the blocks hold whole lattice layers; keep nw and ns
+    keep = np.concatenate([np.arange(nw), n_water_built + np.arange(ns)])
+    sc = scenes.Scene(sc.name, sc.physics, sc.pos[keep].copy(), sc.label[keep].copy(), sc.mass[keep].copy(),
+                      sc.radius[keep].copy(), sc.col[keep].copy(), bpos=sc.bpos, blabel=sc.blabel)
+    assert (int((sc.label == 0).sum()), int((sc.label == 1).sum())) == (nw, ns)
+    g = SFSystem.from_scene(sc)
+    o = oracle_mod.OracleSystem(sc, 64)
+    o32 = oracle_mod.OracleSystem(sc, 32)
+    g.step(1e-4)
+    o.step(1e-4)
+    o32.step(1e-4)
+    np.testing.assert_array_equal(g.get("sort_perm"), o.get("sort_perm"))
+    # The lattices sit at rest spacing, so p ~ (rho - rho0) amplifies the
+    # FP32 rounding of rho; each quantity is held to 1e-5 or twice the error
+    # of the same formulas in plain FP32 (the FP32 oracle), whichever is larger.
+    for q in ("density", "pressure", "vel", "pos", "acc"):
+        ref = o.get(q)
+        err = norm_err(g.get(q), ref, None)
+        floor = 2.0 * norm_err(o32.get(q), ref, None)
+        print(f"[phases {nw}+{ns}] {q} {err:.2e} (FP32 oracle {floor / 2:.2e})", flush=True)
+        assert err <= max(1e-5, floor), (nw, ns, q, err, floor)
+
+
 def test_update_system_cfl(oracle_mod):
     """UpdateSystem(renderInterval) with the WCSPH CFL sub-step count
     (ComputeSubTimeStepsByCFL, cuda_wcsph_sf_solver.cu:35-58)."""
