@@ -128,7 +128,7 @@ int ensure_sort_capacity(sf_system* h, size_t m) {
     size_t nblocks = (c + kRsTile - 1) / kRsTile;
     if ((rc = h->hist.alloc(256 * nblocks))) return rc;
     if ((rc = h->wpre.alloc(c + 1)) || (rc = h->wpart.alloc((size_t)wprefix_blocks((int)c + 1) + 1)) ||
-        (rc = h->plist.alloc(c)))
+        (rc = h->plist.alloc(c)) || (rc = h->wflag.alloc(c)))
         return rc;
     if ((rc = h->skeys[0].alloc(c)) || (rc = h->skeys[1].alloc(c)) || (rc = h->sperm.alloc(c)) ||
         (rc = h->slot.alloc(c)) || (rc = h->in_list.alloc(c)) || (rc = h->stay_pre.alloc(c + 1)))
@@ -193,6 +193,7 @@ int build_neighbors(sf_system* h, hipEvent_t* ev) {
     g.cs_out = h->cs[h->cur_cs ^ 1].p;
     g.cda_out = h->cda[h->cur_aux ^ 1].p;
     g.amc_out = h->amc[h->cur_aux ^ 1].p;
+    g.wflag = h->wflag.p;
     g.multi = h->K.multi;
     SF_MARK(ev, SF_PASS_GATHER);
     launch_gather_wprefix(g, n, h->wpart.p, h->wpre.p, h->plist.p, s);

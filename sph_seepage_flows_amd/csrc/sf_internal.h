@@ -98,6 +98,7 @@ struct GatherArgs {
     const float2* amc_in;
     float4 *pm_out, *vr_out, *cs_out, *cda_out;
     float2* amc_out;
+    uint8_t* wflag;  // out: 1 for water slots (read by the water prefix)
     int multi;
 };
 // Reorders the persistent state by perm, then wpre[0..n] = water count among

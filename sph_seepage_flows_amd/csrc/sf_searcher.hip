@@ -281,7 +281,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(GatherArgs a, int n, uint32_t
             a.cda_out[i] = a.cda_in[p];
             a.amc_out[i] = a.amc_in[p];
         }
-        c += pm.w > 0.f ? 1u : 0u;
+        const uint32_t water = pm.w > 0.f ? 1u : 0u;
+        a.wflag[i] = (uint8_t)water;
+        c += water;
     }
     for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
     if ((t & 63) == 0) ws[t >> 6] = c;
@@ -430,10 +432,6 @@ void launch_vprefix2(V1 v1, int n1, uint32_t* out1, V2 v2, int n2, uint32_t* out
     k_rs_scan2<<<1, 1024, 0, s>>>(part, nb1, nb2, nullptr);
     k_vprefix2<<<nb1 + nb2, kBlock, 0, s>>>(v1, n1, nb1, out1, v2, n2, out2, part);
 }
-struct WaterVal {
-    const float4* pm;
-    SF_DEV uint32_t operator()(int i) const { return pm[i].w > 0.f ? 1u : 0u; }
-};
 struct StayVal {
     const uint32_t* key;
     const uint32_t* old;
@@ -443,16 +441,20 @@ struct ArrayVal {
     const uint32_t* a;
     SF_DEV uint32_t operator()(int i) const { return a[i]; }
 };
+struct FlagVal {
+    const uint8_t* f;
+    SF_DEV uint32_t operator()(int i) const { return f[i]; }
+};
 
 // Water prefix of the gathered slots (k_gather left the tile counts in
 // wpart) and the slots grouped by phase (water first, each group in slot
 // order) so that a pass over one phase can give every XCD an equal
 // contiguous share of it.
-__global__ __launch_bounds__(kBlock) void k_wprefix_plist(const float4* __restrict__ pm, int n,
+__global__ __launch_bounds__(kBlock) void k_wprefix_plist(const uint8_t* __restrict__ wflag, int n,
                                                           const uint32_t* __restrict__ wpart, uint32_t* __restrict__ wpre,
                                                           uint32_t* __restrict__ plist) {
     const uint32_t nwater = wpart[gridDim.x];
-    tile_prefix(WaterVal{pm}, n, (int)blockIdx.x, wpart[blockIdx.x], wpre, [&](int i, uint32_t pre, uint32_t water) {
+    tile_prefix(FlagVal{wflag}, n, (int)blockIdx.x, wpart[blockIdx.x], wpre, [&](int i, uint32_t pre, uint32_t water) {
         plist[water ? pre : nwater + (uint32_t)i - pre] = (uint32_t)i;
     });
 }
@@ -461,7 +463,7 @@ void launch_gather_wprefix(const GatherArgs& a, int n, uint32_t* wpart, uint32_t
     const int nb = wprefix_blocks(n + 1);
     k_gather<<<nb, kBlock, 0, s>>>(a, n, wpart);
     k_rs_scan2<<<1, 1024, 0, s>>>(wpart, nb, 0, wpart + nb);
-    k_wprefix_plist<<<nb, kBlock, 0, s>>>(a.pm_out, n, wpart, wpre, plist);
+    k_wprefix_plist<<<nb, kBlock, 0, s>>>(a.wflag, n, wpart, wpre, plist);
 }
 
 // ---------------------------------------------------------------------------

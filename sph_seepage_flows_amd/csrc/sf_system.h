@@ -34,6 +34,7 @@ struct sf_system {
     int cur_aux = 0;  // cda / amc
     // searcher
     DevBuf<uint32_t> keys, kA, kB, vA, vB, hist, cell, bcell, wpre, wpart, plist;
+    DevBuf<uint8_t> wflag;
     DevBuf<uint2> ct;
     // incremental sort (sf_set_sort_strategy): sorted-key ping-pong, permutation, work arrays
     DevBuf<uint32_t> skeys[2], sperm, slot, in_list, stay_pre, in_count, in_start, part2;
