@@ -377,12 +377,14 @@ SF_DEV void tile_prefix(V v, int n, int tile, uint32_t part, uint32_t* __restric
     const int base = tile * kWpTile + t * kWpItems;
     uint32_t f[kWpItems];
     uint32_t s = 0;
+    if (base + kWpItems <= n) {
+        v.load8(base, f);  // whole run in range: vector loads
+    } else {
 #pragma unroll
-    for (int q = 0; q < kWpItems; ++q) {
-        const int i = base + q;
-        f[q] = i < n ? v(i) : 0u;
-        s += f[q];
+        for (int q = 0; q < kWpItems; ++q) f[q] = base + q < n ? v(base + q) : 0u;
     }
+#pragma unroll
+    for (int q = 0; q < kWpItems; ++q) s += f[q];
     uint32_t x = s;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -432,18 +434,45 @@ void launch_vprefix2(V1 v1, int n1, uint32_t* out1, V2 v2, int n2, uint32_t* out
     k_rs_scan2<<<1, 1024, 0, s>>>(part, nb1, nb2, nullptr);
     k_vprefix2<<<nb1 + nb2, kBlock, 0, s>>>(v1, n1, nb1, out1, v2, n2, out2, part);
 }
+// Per-slot values of the prefixes; load8(i, f) reads slots i .. i + 7 (i a
+// multiple of 8, all below n) with vector loads.
+SF_DEV void load8_u32(const uint32_t* a, int i, uint4& x, uint4& y) {
+    x = *reinterpret_cast<const uint4*>(a + i);
+    y = *reinterpret_cast<const uint4*>(a + i + 4);
+}
 struct StayVal {
     const uint32_t* key;
     const uint32_t* old;
     SF_DEV uint32_t operator()(int i) const { return key[i] == old[i] ? 1u : 0u; }
+    SF_DEV void load8(int i, uint32_t* f) const {
+        uint4 k0, k1, o0, o1;
+        load8_u32(key, i, k0, k1);
+        load8_u32(old, i, o0, o1);
+        f[0] = k0.x == o0.x; f[1] = k0.y == o0.y; f[2] = k0.z == o0.z; f[3] = k0.w == o0.w;
+        f[4] = k1.x == o1.x; f[5] = k1.y == o1.y; f[6] = k1.z == o1.z; f[7] = k1.w == o1.w;
+    }
 };
 struct ArrayVal {
     const uint32_t* a;
     SF_DEV uint32_t operator()(int i) const { return a[i]; }
+    SF_DEV void load8(int i, uint32_t* f) const {
+        uint4 x, y;
+        load8_u32(a, i, x, y);
+        f[0] = x.x; f[1] = x.y; f[2] = x.z; f[3] = x.w;
+        f[4] = y.x; f[5] = y.y; f[6] = y.z; f[7] = y.w;
+    }
 };
 struct FlagVal {
     const uint8_t* f;
     SF_DEV uint32_t operator()(int i) const { return f[i]; }
+    SF_DEV void load8(int i, uint32_t* o) const {
+        const uint2 w = *reinterpret_cast<const uint2*>(f + i);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            o[q] = (w.x >> (8 * q)) & 0xffu;
+            o[4 + q] = (w.y >> (8 * q)) & 0xffu;
+        }
+    }
 };
 
 // Water prefix of the gathered slots (k_gather left the tile counts in
