@@ -3,12 +3,14 @@
 
 Workload (default): the 1M-particle drop-in scene -- 600k WCSPH water + 400k
 DEM sand in a 2 x 1 x 1 m box, jitter seed 1, dt = 1e-4, capillary and
-seepage drag on.  One "step" = hash + radix sort + cellStart + reorder +
+seepage drag on.  One "step" = hash + stable sort + reorder + cellStart +
 the five solver passes + advect over all particles, inputs resident in HBM.
 
 Prints ONE JSON line.  `roofline` is for the dominant pass: its algorithmic
 HBM bytes per launch (byte model below) divided by its mean duration, timed
-with HIP events on the stream the passes run on, inside the timed region.
+with HIP events on the stream the passes run on, inside the timed region
+(events around that pass only; an untimed profiled run before it gives the
+per-pass breakdown `pass_ms_profiled` and picks the pass).
 `cpu_baseline` times the FP32 C++ oracle (a restatement of the reference's
 kernels; the reference itself has no CPU path) on the same scene on the host
 cores, for a bounded number of steps.
@@ -26,6 +28,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "coupled particle-steps/s @1M particles (%HBM roofline); adjoint/forward time"
+PROFILE_STEPS = 10  # untimed steps with an event between every two passes
 HBM_PEAK_GBS = 8000.0
 
 
@@ -239,20 +242,24 @@ def main() -> int:
         n, nb, cells = g.sizes()
         if args.warmup:
             g.step(dt, args.warmup)
+        # untimed: per-pass breakdown (an event between every two passes) and
+        # the dominant pass; the timed region then records events around that
+        # pass only, since each event between kernels costs the stream ~5 us
+        _, pass_ms = g.step_profiled(dt, PROFILE_STEPS, per_pass=True)
+        dom = max(pass_ms, key=pass_ms.get)
         g.synchronize()
         t0 = time.perf_counter()
-        total_ms, pass_ms = g.step_profiled(dt, args.steps, per_pass=True)
+        total_ms, dom_ms = g.step_timed(dt, args.steps, dom)
         wall = time.perf_counter() - t0
         g.close()
         ms_per_step = total_ms / args.steps
         print(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step (events), wall {wall * 1e3 / args.steps:.3f} "
-              f"ms/step; passes {json.dumps({k: round(v, 4) for k, v in pass_ms.items()})}", file=sys.stderr,
-              flush=True)
+              f"ms/step; {dom} {dom_ms:.4f} ms; passes of {PROFILE_STEPS} profiled steps "
+              f"{json.dumps({k: round(v, 4) for k, v in pass_ms.items()})}", file=sys.stderr, flush=True)
         n_s = int(np.count_nonzero(scene.label == 1))
         n_w = n - n_s
         pb = pass_bytes(n_w, n_s, nb, cells, radix_passes(cells), scene.physics.sf_type == 1)
-        dom = max(pass_ms, key=pass_ms.get)
-        ach = pb[dom] / (pass_ms[dom] * 1e-3) / 1e9
+        ach = pb[dom] / (dom_ms * 1e-3) / 1e9
         step_bytes = sum(pb.values())
         traffic = None
         prof = ROOT / "profiles" / f"pmc_{dom}_{scene.name}.json"
@@ -261,6 +268,18 @@ def main() -> int:
                 traffic = json.loads(prof.read_text()).get("hbm_bytes_per_launch")
             except Exception:
                 traffic = None
+        # The neighbour passes are bound by instruction issue, not HBM: VALU
+        # busy = (VALU wave-instructions per launch, from the committed PMC
+        # profile) x 4 cycles / (1024 SIMDs x 2.4 GHz x this run's mean launch time).
+        valu = None
+        vprof = ROOT / "profiles" / f"pmc_valu_{dom}_{scene.name}.json"
+        if vprof.exists():
+            try:
+                insts = float(json.loads(vprof.read_text())["valu_insts_per_launch"])
+                valu = {"valu_insts_per_launch": insts, "busy_frac": insts * 4 / (1024 * 2.4e9 * dom_ms * 1e-3),
+                        "source": str(vprof.relative_to(ROOT))}
+            except Exception:
+                valu = None
         out.update({
             "value": n * args.steps / (total_ms / 1e3),
             "ms_per_step": ms_per_step,
@@ -269,11 +288,11 @@ def main() -> int:
                        "parallelism": "single-gpu"},
             "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "alg_bytes_per_launch": pb[dom],
-                         "mean_launch_ms": pass_ms[dom]},
+                         "mean_launch_ms": dom_ms, "valu": valu},
             "step_roofline": {"alg_bytes_per_particle_step": step_bytes / n,
                               "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
                               "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
-            "pass_ms": pass_ms,
+            "pass_ms_profiled": pass_ms,
         })
     if "ensemble" in parts and not args.no_ensemble:
         print("[bench] batched ensemble...", file=sys.stderr, flush=True)

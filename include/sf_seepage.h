@@ -216,6 +216,13 @@ enum sf_pass {
  * the reference: it replaces the cudaEvent timing of UpdateSystem
  * (cuda_base_system.cu:31-55) with per-pass resolution. */
 int sf_step_profiled(sf_handle h, float dt, int n_steps, float* pass_ms, float* total_ms);
+/* The same with events around one pass only (pass in SF_PASS_HASH ..
+ * SF_PASS_ADVECT): every event recorded between two kernels costs the stream
+ * a few microseconds, so this times a whole run with the least disturbance
+ * while still timing that pass inside it.  *pass_ms (may be NULL) receives
+ * the pass's mean duration per step, *total_ms the whole region.  No
+ * counterpart in the reference. */
+int sf_step_timed(sf_handle h, float dt, int n_steps, int pass, float* pass_ms, float* total_ms);
 int sf_size(sf_handle h, size_t* n, size_t* nb, size_t* num_cells);
 
 /* ---- data access -------------------------------------------------------- */

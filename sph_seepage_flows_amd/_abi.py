@@ -142,6 +142,7 @@ _SIG = {
     "sf_autotune": (C.c_int, [_VP, C.c_int, _VP, _VP]),
     "sf_set_sort_strategy": (C.c_int, [_VP, C.c_int]),
     "sf_set_interface_lanes": (C.c_int, [_VP, C.c_int]),
+    "sf_step_timed": (C.c_int, [_VP, C.c_float, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "sf_diff_forward": (C.c_int, [_VP, C.c_float, C.c_int, C.c_int]),
     "sf_diff_reset": (C.c_int, [_VP, _VP, C.c_int]),
     "sf_diff_get": (C.c_int, [_VP, C.c_int, _VP, C.c_int]),

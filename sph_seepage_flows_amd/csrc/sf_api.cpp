@@ -160,7 +160,7 @@ int zero(sf_system* h, DevBuf<T>& b) {
 
 // ev (optional): events recorded before each pass of the step
 #define SF_MARK(ev, k) \
-    if (ev) SF_HIP(hipEventRecord((ev)[k], s))
+    if ((ev) && (ev)[k]) SF_HIP(hipEventRecord((ev)[k], s))
 
 int build_neighbors(sf_system* h, hipEvent_t* ev) {
     const int n = (int)h->n;
@@ -576,6 +576,42 @@ int sf_step_profiled(sf_handle h, float dt, int n_steps, float* pass_ms, float* 
             }
         }
         for (int k = 0; k < SF_PASS_COUNT; ++k) pass_ms[k] /= (float)n_steps;
+    }
+    return SF_OK;
+}
+
+int sf_step_timed(sf_handle h, float dt, int n_steps, int pass, float* pass_ms, float* total_ms) {
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!h->uploaded) return set_error(SF_E_STATE, "no particles uploaded");
+    if (!std::isfinite(dt) || !(dt > 0.f)) return set_error(SF_E_INVALID, "dt must be finite and > 0");
+    if (n_steps < 1 || n_steps > 100000) return set_error(SF_E_INVALID, "n_steps must be in [1, 100000]");
+    if (pass < 0 || pass >= SF_PASS_COUNT) return set_error(SF_E_INVALID, "pass must be one of the step's passes");
+    const size_t need = 2 * (size_t)n_steps;
+    while (h->ev_pool.size() < need) {
+        hipEvent_t e;
+        SF_HIP(hipEventCreate(&e));
+        h->ev_pool.push_back(e);
+    }
+    hipStream_t s = h->stream;
+    std::vector<hipEvent_t> marks(SF_PASS_COUNT + 1, nullptr);
+    SF_HIP(hipEventRecord(h->ev0, s));
+    for (int q = 0; q < n_steps; ++q) {
+        marks[pass] = h->ev_pool[2 * q];
+        marks[pass + 1] = h->ev_pool[2 * q + 1];
+        if ((rc = one_step(h, dt, marks.data()))) return rc;
+    }
+    SF_HIP(hipEventRecord(h->ev1, s));
+    SF_HIP(hipEventSynchronize(h->ev1));
+    if (total_ms) SF_HIP(hipEventElapsedTime(total_ms, h->ev0, h->ev1));
+    if (pass_ms) {
+        float sum = 0.f;
+        for (int q = 0; q < n_steps; ++q) {
+            float ms = 0.f;
+            SF_HIP(hipEventElapsedTime(&ms, h->ev_pool[2 * q], h->ev_pool[2 * q + 1]));
+            sum += ms;
+        }
+        *pass_ms = sum / (float)n_steps;
     }
     return SF_OK;
 }

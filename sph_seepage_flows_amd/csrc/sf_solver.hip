@@ -290,7 +290,8 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
         [](uint32_t nw, uint32_t) { return nw != 0u; },
         [&](uint32_t, const float4& pj) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float d = fmaxf(k.h2 - (dx * dx + dy * dy + dz * dz), 0.f);
+            // h^2 - r^2 as one fused chain (three FMAs, one rounding each)
+            const float d = fmaxf(fmaf(-dz, dz, fmaf(-dy, dy, fmaf(-dx, dx, k.h2))), 0.f);
             sf += pj.w * sq3(d);
         });
 #pragma unroll

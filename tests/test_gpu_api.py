@@ -84,3 +84,11 @@ def test_profiled_step_matches_plain_step():
     tot, passes = b.step_profiled(1e-4, 5)
     np.testing.assert_array_equal(a.get("pos"), b.get("pos"))
     assert tot > 0 and set(passes) == set(_abi.PASSES) and all(v >= 0 for v in passes.values())
+    c = SFSystem.from_scene(sc)
+    tot_c, ms = c.step_timed(1e-4, 5, "water_momentum")
+    np.testing.assert_array_equal(a.get("pos"), c.get("pos"))
+    assert 0 < ms < tot_c
+    for name in ("hash", "advect"):  # first and last pass
+        d = SFSystem.from_scene(sc)
+        assert d.step_timed(1e-4, 2, name)[1] > 0
+    assert _abi.lib().sf_step_timed(c._h, 1e-4, 1, len(_abi.PASSES), None, None) == _abi.SF_E_INVALID
