@@ -84,22 +84,8 @@ SF_DEV uint32_t scene_base(const SFConst& k, const Fields& f, int i) {
     else return 0u;
 }
 
-// Walks pm[s, e) with four independent loads in flight per lane (the loop
-// is otherwise bound by one dependent global load per candidate).
-template <class F>
-SF_DEV void for_range4(const float4* __restrict__ pm, uint32_t s, uint32_t e, F&& fn) {
-    uint32_t j = s;
-    for (; j + 4u <= e; j += 4u) {
-        const float4 a = ldo(pm, j), b = ldo(pm, j + 1u), c = ldo(pm, j + 2u), d = ldo(pm, j + 3u);
-        fn(j, a);
-        fn(j + 1u, b);
-        fn(j + 2u, c);
-        fn(j + 3u, d);
-    }
-    for (; j < e; ++j) fn(j, ldo(pm, j));
-}
-
-// Every S-th slot of [s, e), four loads in flight.
+// Every S-th slot of [s, e) with four independent loads in flight per lane
+// (the loop is otherwise bound by one dependent global load per candidate).
 template <uint32_t S, class F>
 SF_DEV void for_range4s(const float4* __restrict__ pm, uint32_t s, uint32_t e, F&& fn) {
     uint32_t j = s;
