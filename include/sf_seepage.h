@@ -27,8 +27,9 @@
  *   sf_write_bgeo        <- Partio::writeBGEO (uncompressed V5)  extlib/include/partio/io/BGEO.cpp:311-476
  *   sf_build_world_boundary <- CudaBoundaryEmitter::BuildWorldBoundary  cuda_boundary_emitter.cpp:12-80
  *   sf_grid_size         <- make_int3((highest - lowest) / kernel_radius)  main.cpp:104, cuda_neighbor_searcher.cu:26
+ *   sf_set_emitter / sf_get_emitter <- CudaEmitter + CudaSphEmitterParams (see their declarations)
  *   sf_batch_*, sf_step_graph, sf_*_launch_config, sf_autotune, sf_set_field, sf_diff_*,
- *   sf_set_sort_strategy, sf_set_interface_lanes:
+ *   sf_set_sort_strategy, sf_set_interface_lanes, sf_step_timed:
  *                           no counterpart (see their declarations)
  *
  * Threading: a handle is single-threaded; all its GPU work is ordered on the
@@ -238,6 +239,39 @@ int sf_set_field(sf_handle h, int field, const void* src, size_t src_bytes);
 int sf_device_buffer(sf_handle h, int field, void** dptr, size_t* stride_bytes);
 /* Writes folder/name.bgeo from the current state (creates folder). */
 int sf_export_bgeo(sf_handle h, const char* folder, const char* name);
+
+/* ---- inflow emitter ------------------------------------------------------
+ * sf_emitter_params mirrors CudaSphEmitterParams (cuda_sph_params.h:49-70);
+ * sf_set_emitter replaces the CudaEmitter ctor + Build*Emitter calls of the
+ * CudaSFSystem ctor (cuda_emitter.cuh:24-55, cuda_emitter.cpp:12-97,
+ * cuda_sf_system.cu:36-58): the sample points are built with the water
+ * particle radius exactly as the reference's float loops build them.  Each
+ * sf_update_system then emits after its step as CudaSFSystem::OnUpdateSolver
+ * does (cuda_sf_system.cu:86-115): WCSPH, when |v| * elapsed / (2 r_w) reaches
+ * the emission counter (elapsed += render_interval / sub-steps); SPH, every
+ * int(2 r_w / (|v| dt))-th call.  Emitted particles (AddSphParticles,
+ * cuda_sf_particles.cu:50-68) are water of mass sph_mass and radius
+ * sph_particle_radius with the emitter's velocity and colour, appended after
+ * the current particles; when they would not fit below max_num the emitter
+ * switches itself off.  sf_step / sf_step_n / sf_step_graph do not emit. */
+#define SF_EMITTER_SQUARE 0
+#define SF_EMITTER_CIRCLE 1
+#define SF_EMITTER_RECTANGLE 2
+typedef struct sf_emitter_params {
+    int enable;           /* CudaSphEmitterParams::enable: build the samples, emitter on */
+    int run;              /* CudaSphEmitterParams::run: emission switched on */
+    sf_float3 emit_pos;
+    sf_float3 emit_vel;   /* also the emitted particles' velocity; must not be zero */
+    sf_float3 emit_col;
+    int emit_type;        /* SF_EMITTER_SQUARE / CIRCLE / RECTANGLE */
+    float emit_radius;    /* square and circle */
+    float emit_width;     /* rectangle */
+    float emit_height;    /* rectangle */
+} sf_emitter_params;
+int sf_set_emitter(sf_handle h, const sf_emitter_params* e);
+/* enabled: the emitter's status (it turns itself off at capacity);
+ * num_points: samples per emission; emitted: particles emitted so far. */
+int sf_get_emitter(sf_handle h, int* enabled, size_t* num_points, size_t* emitted);
 
 /* ---- batched scenes ------------------------------------------------------
  * No counterpart in the reference (one CudaSFSystem per process, main.cpp:106-221):

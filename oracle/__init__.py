@@ -65,6 +65,8 @@ def lib() -> C.CDLL:
             "orc_step": (None, [VP, F, C.c_int]),
             "orc_update_system": (None, [VP, F]),
             "orc_num_sub_steps": (SZ, [VP]),
+            "orc_set_emitter": (None, [VP, C.c_int, C.c_int, VP, VP, VP, C.c_int, F, F, F, SZ]),
+            "orc_emitter_state": (None, [VP, VP, VP, VP]),
             "orc_size": (SZ, [VP]),
             "orc_get": (C.c_int, [VP, C.c_int, VP]),
             "orc_set_state": (None, [VP, VP, VP]),
@@ -144,6 +146,23 @@ class OracleSystem:
 
     def num_sub_steps(self) -> int:
         return int(lib().orc_num_sub_steps(self._h))
+
+    def size(self) -> int:
+        return int(lib().orc_size(self._h))
+
+    def set_emitter(self, cap: int, pos, vel, col, kind: int = 0, radius: float = 0.0, width: float = 0.0,
+                    height: float = 0.0, enable: bool = True, run: bool = True):
+        """Inflow emitter (kind 0 square, 1 circle, 2 rectangle); cap = the GPU
+        handle's max particle count (emission needs n + samples < cap)."""
+        f3 = [np.ascontiguousarray(a, np.float32) for a in (pos, vel, col)]
+        lib().orc_set_emitter(self._h, int(enable), int(run), *[_p(a) for a in f3], int(kind), float(radius),
+                              float(width), float(height), int(cap))
+        self._keep_em = f3
+
+    def emitter_state(self):
+        e, p, m = C.c_int(), C.c_size_t(), C.c_size_t()
+        lib().orc_emitter_state(self._h, C.byref(e), C.byref(p), C.byref(m))
+        return bool(e.value), p.value, m.value
 
     def set_state(self, pos, vel):
         pos = np.ascontiguousarray(pos, np.float64)

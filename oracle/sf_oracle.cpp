@@ -302,6 +302,60 @@ template <class T> struct Sys : Base {
         return -1; /* sentinel gx*gy*gz, helper_thrust.cuh:123-128 */
     }
 
+    /* Inflow emitter (host code of the reference, FP32 as written; this file
+     * is compiled without fused multiply-add): CudaEmitter samples and axes
+     * (cuda_emitter.cpp:12-97, cuda_emitter.cuh:33-55), the emission schedule
+     * of CudaSFSystem::OnUpdateSolver (cuda_sf_system.cu:86-115) and
+     * AddSphParticles (cuda_sf_particles.cu:50-68).  cap = max particle count. */
+    struct Emitter {
+        bool enable = false, run = false;
+        float pos[3] = {0, 0, 0}, vel[3] = {0, 0, 0}, col[3] = {0, 0, 0}, a1[3] = {0, 0, 0}, a2[3] = {0, 0, 0};
+        std::vector<float> s1, s2;
+        size_t counter = 0, emitted = 0, cap = 0;
+        float elapsed = 0.f;
+    } em;
+    void emit_water() {
+        const size_t k = em.s1.size();
+        if (n + k >= em.cap) { /* Size() + p.size() < MaxSize() failed: emitter off */
+            em.enable = false;
+            return;
+        }
+        for (size_t q = 0; q < k; ++q) {
+            float p[3];
+            for (int c = 0; c < 3; ++c) p[c] = em.pos[c] + em.s1[q] * em.a1[c] + em.s2[q] * em.a2[c];
+            pos.push_back(mk<T>(p[0], p[1], p[2]));
+            vel.push_back(mk<T>(em.vel[0], em.vel[1], em.vel[2]));
+            col.push_back(mk<T>(em.col[0], em.col[1], em.col[2]));
+            for (auto* a : {&acc, &avgflow, &avgdrag, &adh, &avgadh, &cda}) a->push_back(mk<T>(0, 0, 0));
+            mass.push_back(T(P.sph_mass));
+            radius.push_back(T(P.sph_particle_radius));
+            for (auto* a : {&density, &pressure, &voidage, &saturation, &maxsat, &amc0, &amc1}) a->push_back(T(0));
+            label.push_back(0);
+            ids.push_back((uint32_t)ids.size());
+            keys.push_back(0);
+            perm.push_back(0);
+            sorted_keys.push_back(0);
+        }
+        n += k;
+        em.emitted += k;
+    }
+    void emit_after_update(float render_interval) {
+        if (!em.enable || !em.run || em.s1.empty()) return;
+        const float speed = std::sqrt(em.vel[0] * em.vel[0] + em.vel[1] * em.vel[1] + em.vel[2] * em.vel[2]);
+        const float two_r = 2.f * P.sph_particle_radius;
+        if (P.solver_type == SF_WCSPH_SOLVER) {
+            em.elapsed += render_interval / static_cast<float>(nsub);
+            if ((speed * em.elapsed) / two_r >= static_cast<float>(em.counter)) {
+                emit_water();
+                em.counter++;
+            }
+        } else {
+            size_t step_num = (size_t)(two_r / (speed * P.dt));
+            if (step_num == 0) step_num = 1; /* the reference divides by zero here */
+            if (em.counter++ % step_num == 0) emit_water();
+        }
+    }
+
     void resize(size_t k) {
         n = k;
         for (auto* a : {&pos, &vel, &acc, &col, &avgflow, &avgdrag, &adh, &avgadh, &cda}) a->assign(k, mk<T>(0, 0, 0));
@@ -763,6 +817,7 @@ template <class T> struct Sys : Base {
             solve();
             advect((T)P.dt);
         }
+        emit_after_update(render_interval);
     }
 
     /* in-range neighbour count: r^2 < h^2 with r^2 = dx*dx + dy*dy + dz*dz in
@@ -964,6 +1019,65 @@ void orc_step(void* hv, float dt, int nsteps) {
 void orc_update_system(void* hv, float render_interval) {
     Handle* h = (Handle*)hv;
     if (h->prec == 32) S<float>(h)->update_system(render_interval); else S<double>(h)->update_system(render_interval);
+}
+extern "C++" {
+static void norm3(const float v[3], float out[3]) {
+    const float inv = 1.f / std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    for (int c = 0; c < 3; ++c) out[c] = v[c] * inv;
+}
+static void cross3(const float a[3], const float b[3], float out[3]) {
+    out[0] = a[1] * b[2] - a[2] * b[1];
+    out[1] = a[2] * b[0] - a[0] * b[2];
+    out[2] = a[0] * b[1] - a[1] * b[0];
+}
+template <class T>
+static void set_emitter_T(Sys<T>* s, int enable, int run, const float* pos3, const float* vel3, const float* col3,
+                          int type, float radius, float width, float height, size_t cap) {
+    auto& e = s->em;
+    e = typename Sys<T>::Emitter();
+    e.enable = enable != 0;
+    e.run = run != 0;
+    e.cap = cap;
+    for (int c = 0; c < 3; ++c) {
+        e.pos[c] = pos3[c];
+        e.vel[c] = vel3[c];
+        e.col[c] = col3[c];
+    }
+    float axis[3], t[3];
+    norm3(e.vel, axis);
+    const float ex[3] = {1.f, 0.f, 0.f}, ey[3] = {0.f, 1.f, 0.f};
+    if (std::fabs(axis[0]) == 1.f && std::fabs(axis[1]) == 0.f && std::fabs(axis[2]) == 0.f) cross3(axis, ey, t);
+    else cross3(axis, ex, t);
+    norm3(t, e.a1);
+    cross3(axis, e.a1, t);
+    norm3(t, e.a2);
+    if (!e.enable) return;
+    const float offset = s->P.sph_particle_radius * 2.f;
+    const float a = type == 2 ? width : radius, b = type == 2 ? height : radius;
+    for (float i = -a; i < a; i += offset)
+        for (float j = -b; j < b; j += offset) {
+            if (type == 1 && !(std::sqrt(i * i + j * j) <= a)) continue; /* circle: length(p) <= radius */
+            e.s1.push_back(i);
+            e.s2.push_back(j);
+        }
+}
+}
+/* type 0 square, 1 circle, 2 rectangle; cap = the handle's max particle count */
+void orc_set_emitter(void* hv, int enable, int run, const float* pos3, const float* vel3, const float* col3, int type,
+                     float radius, float width, float height, size_t cap) {
+    Handle* h = (Handle*)hv;
+    if (h->prec == 32) set_emitter_T(S<float>(h), enable, run, pos3, vel3, col3, type, radius, width, height, cap);
+    else set_emitter_T(S<double>(h), enable, run, pos3, vel3, col3, type, radius, width, height, cap);
+}
+void orc_emitter_state(void* hv, int* enabled, size_t* points, size_t* emitted) {
+    Handle* h = (Handle*)hv;
+    if (h->prec == 32) {
+        auto& e = S<float>(h)->em;
+        *enabled = e.enable; *points = e.s1.size(); *emitted = e.emitted;
+    } else {
+        auto& e = S<double>(h)->em;
+        *enabled = e.enable; *points = e.s1.size(); *emitted = e.emitted;
+    }
 }
 size_t orc_num_sub_steps(void* hv) {
     Handle* h = (Handle*)hv;

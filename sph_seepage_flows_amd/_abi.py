@@ -96,6 +96,16 @@ class sf_solver_options(C.Structure):
                 ("speed_of_sound", C.c_float)]
 
 
+SF_EMITTER_SQUARE, SF_EMITTER_CIRCLE, SF_EMITTER_RECTANGLE = 0, 1, 2
+
+
+class sf_emitter_params(C.Structure):
+    """CudaSphEmitterParams (cuda_sph_params.h:56-70) as sf_set_emitter takes it."""
+    _fields_ = [("enable", C.c_int), ("run", C.c_int), ("emit_pos", sf_float3), ("emit_vel", sf_float3),
+                ("emit_col", sf_float3), ("emit_type", C.c_int), ("emit_radius", C.c_float),
+                ("emit_width", C.c_float), ("emit_height", C.c_float)]
+
+
 class sf_scene_params(C.Structure):
     """Per-scene overrides of a batched handle (names as in sf_params)."""
     _fields_ = [(n, C.c_float) for n in ("dem_tan_friction_angle", "sf_c0", "sf_cmc", "sf_cmc_p", "sf_csat",
@@ -142,6 +152,8 @@ _SIG = {
     "sf_autotune": (C.c_int, [_VP, C.c_int, _VP, _VP]),
     "sf_set_sort_strategy": (C.c_int, [_VP, C.c_int]),
     "sf_set_interface_lanes": (C.c_int, [_VP, C.c_int]),
+    "sf_set_emitter": (C.c_int, [_VP, C.POINTER(sf_emitter_params)]),
+    "sf_get_emitter": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "sf_step_timed": (C.c_int, [_VP, C.c_float, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "sf_diff_forward": (C.c_int, [_VP, C.c_float, C.c_int, C.c_int]),
     "sf_diff_reset": (C.c_int, [_VP, _VP, C.c_int]),

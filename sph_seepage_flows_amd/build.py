@@ -19,7 +19,8 @@ BUILD = PKG / "_build"
 LIB = LIBDIR / "libsf_seepage.so"
 INCLUDE = PKG.parent / "include"
 
-SOURCES = ["sf_searcher.hip", "sf_solver.hip", "sf_adjoint.hip", "sf_api.cpp", "sf_diff.cpp", "sf_host.cpp"]
+SOURCES = ["sf_searcher.hip", "sf_solver.hip", "sf_adjoint.hip", "sf_api.cpp", "sf_diff.cpp", "sf_emitter.cpp",
+           "sf_host.cpp"]
 ARCH = os.environ.get("SF_OFFLOAD_ARCH", "gfx950")
 
 

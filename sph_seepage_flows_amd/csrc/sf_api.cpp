@@ -644,6 +644,8 @@ int sf_update_system(sf_handle h, float render_interval, float* ms_out) {
         SF_HIP(hipEventRecord(h->ev1, h->stream));
         SF_HIP(hipEventSynchronize(h->ev1));
     }
+    // CudaSFSystem::OnUpdateSolver emits after the solver step (cuda_sf_system.cu:86-115)
+    if ((rc = emit_after_update(h, render_interval))) return rc;
     if (ms_out) {
         float ms = 0.f;
         SF_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));

@@ -14,6 +14,15 @@ void diff_release(DiffState* d);
 
 using namespace sf;
 
+// Inflow emitter (host side, sf_emitter.cpp).
+struct EmitterState {
+    bool enable = false, run = false;
+    sf_float3 pos{}, vel{}, col{}, axis1{}, axis2{};
+    std::vector<float> samples;  // (s1, s2) pairs in the emitter plane
+    size_t counter = 0, emitted = 0;
+    float elapsed = 0.f;
+};
+
 struct sf_system {
     sf_params P{};
     sf_boundary_params B{};
@@ -67,6 +76,7 @@ struct sf_system {
     bool graph_dirty = true;
     // differentiable window (sf_diff_*), null until first used
     DiffState* diff = nullptr;
+    EmitterState emitter;
     // diagnostics
     DevBuf<uint32_t> nbr_cnt, nbr_cand;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -117,6 +127,9 @@ int solve_with(sf_system* h, const Fields& f, hipEvent_t* ev = nullptr);
 int advect(sf_system* h, float dt, const CflOut* dt_dev, float render_interval, hipEvent_t* ev = nullptr);
 int one_step(sf_system* h, float dt, hipEvent_t* ev = nullptr);
 int check_handle(sf_system* h);
+// Inflow emitter after a sf_update_system step (sf_emitter.cpp);
+// render_interval as passed to that call.
+int emit_after_update(sf_system* h, float render_interval);
 int set_device(sf_system* h);
 int fetch4(sf_system* h, const float4* src, size_t n, std::vector<float4>& out);
 }  // namespace sf

@@ -97,6 +97,19 @@ class SFSystem:
         """Incremental stable sort (default) or the LSD radix sort every step."""
         check(lib().sf_set_sort_strategy(self._h, _abi.SF_SORT_INCREMENTAL if incremental else _abi.SF_SORT_RADIX))
 
+    def set_emitter(self, pos, vel, col, kind: int = _abi.SF_EMITTER_SQUARE, radius: float = 0.0,
+                    width: float = 0.0, height: float = 0.0, enable: bool = True, run: bool = True) -> None:
+        """Inflow emitter (sf_set_emitter): emits water in UpdateSystem."""
+        e = _abi.sf_emitter_params(int(enable), int(run), _abi.sf_float3.of(pos), _abi.sf_float3.of(vel),
+                                   _abi.sf_float3.of(col), int(kind), float(radius), float(width), float(height))
+        check(lib().sf_set_emitter(self._h, C.byref(e)))
+
+    def emitter_state(self) -> tuple[bool, int, int]:
+        """(enabled, samples per emission, particles emitted so far)."""
+        e, p, m = C.c_int(), C.c_size_t(), C.c_size_t()
+        check(lib().sf_get_emitter(self._h, C.byref(e), C.byref(p), C.byref(m)))
+        return bool(e.value), p.value, m.value
+
     def set_interface_lanes(self, lanes: int) -> None:
         """Lanes per particle (1, 2 or 4) of the avg-flow and voidage/adhesion
         passes (sf_set_interface_lanes).  More lanes speed up small or mixed
