@@ -253,7 +253,10 @@ int sf_export_bgeo(sf_handle h, const char* folder, const char* name);
  * cuda_sf_particles.cu:50-68) are water of mass sph_mass and radius
  * sph_particle_radius with the emitter's velocity and colour, appended after
  * the current particles; when they would not fit below max_num the emitter
- * switches itself off.  sf_step / sf_step_n / sf_step_graph do not emit. */
+ * switches itself off.  sf_step / sf_step_n / sf_step_graph do not emit.
+ * The emission counter and elapsed time are the system's (zero at sf_create)
+ * and survive later sf_set_emitter calls, as the reference's CudaSFSystem
+ * members survive changes to its emitter and to CUDA_SPH_EMITTER_PARAMS.run. */
 #define SF_EMITTER_SQUARE 0
 #define SF_EMITTER_CIRCLE 1
 #define SF_EMITTER_RECTANGLE 2

@@ -1034,7 +1034,12 @@ template <class T>
 static void set_emitter_T(Sys<T>* s, int enable, int run, const float* pos3, const float* vel3, const float* col3,
                           int type, float radius, float width, float height, size_t cap) {
     auto& e = s->em;
+    const size_t counter = e.counter, emitted = e.emitted;
+    const float elapsed = e.elapsed;
     e = typename Sys<T>::Emitter();
+    e.counter = counter; /* system state (CudaSFSystem members), kept across reconfiguration */
+    e.emitted = emitted;
+    e.elapsed = elapsed;
     e.enable = enable != 0;
     e.run = run != 0;
     e.cap = cap;

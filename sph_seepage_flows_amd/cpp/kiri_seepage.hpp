@@ -21,8 +21,9 @@
 //
 // Differences a caller sees: Get*Ptr() return device pointers to this
 // library's float4 storage (stride 16 bytes, see sf_device_buffer); the bgeo
-// export takes the particle object instead of four device pointers; the
-// inflow emitter is not part of the seepage step and must stay disabled.
+// export takes the particle object instead of four device pointers.
+//   CudaEmitter / CUDA_SPH_EMITTER_PARAMS             -> sf_set_emitter
+//     (emitter/cuda_emitter.cpp:12-97, system/cuda_sf_system.cu:36-58,86-115)
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -271,9 +272,84 @@ class CudaWCSphSFSolver {
 };
 typedef SharedPtr<CudaWCSphSFSolver> CudaSphSFSolverPtr;
 
-// The inflow emitter is outside the seepage step; only a disabled one is accepted.
-struct CudaEmitter {
+// Inflow emitter: CudaSphEmitterParams / CUDA_SPH_EMITTER_PARAMS
+// (data/cuda_sph_params.h:49-84) and CudaEmitter (emitter/cuda_emitter.cuh:19-85).
+// The emitter records its configuration; CudaSFSystem hands it to the library
+// (sf_set_emitter) before every UpdateSystem, which samples and emits exactly
+// as the reference does, and reads its status back (it switches itself off
+// at capacity).
+enum CudaSphEmitterType { SQUARE = SF_EMITTER_SQUARE, CIRCLE = SF_EMITTER_CIRCLE, RECTANGLE = SF_EMITTER_RECTANGLE };
+struct CudaSphEmitterParams {
     bool enable = false;
+    bool run = false;
+    float3 emit_pos{0.f, 0.f, 0.f};
+    float3 emit_vel{0.f, 0.f, 0.f};
+    float3 emit_col{0.f, 0.f, 0.f};
+    CudaSphEmitterType emit_type = SQUARE;
+    float emit_radius = 0.f;
+    float emit_width = 0.f;
+    float emit_height = 0.f;
+};
+inline CudaSphEmitterParams CUDA_SPH_EMITTER_PARAMS{};
+
+class CudaEmitter {
+  public:
+    CudaEmitter() : CudaEmitter(float3{0.f, 0.f, 0.f}, float3{1.f, 0.f, 0.f}, false) {}
+    CudaEmitter(float3 emitPosition, float3 emitVelocity, bool enable)
+        : mPos(emitPosition), mVel(emitVelocity), bEnable(enable) {}
+    CudaEmitter(const CudaEmitter&) = delete;
+    CudaEmitter& operator=(const CudaEmitter&) = delete;
+
+    void SetEmitterStatus(bool enable) { bEnable = enable; }
+    void UpdateEmitterVelocity(float3 emitVelocity) { mVel = emitVelocity; }
+    // the library samples with the water particle radius, as the CudaSFSystem
+    // ctor calls these (cuda_sf_system.cu:36-58)
+    void BuildSquareEmitter(float, float emitterRadius) { build(SF_EMITTER_SQUARE, emitterRadius, 0.f, 0.f); }
+    void BuildCircleEmitter(float, float emitterRadius) { build(SF_EMITTER_CIRCLE, emitterRadius, 0.f, 0.f); }
+    void BuildRectangleEmitter(float, float emitterWidth, float emitterHeight) {
+        build(SF_EMITTER_RECTANGLE, 0.f, emitterWidth, emitterHeight);
+    }
+    bool GetEmitterStatus() const { return bEnable; }
+    size_t GetNumOfEmitterPoints() const { return mPoints; }
+    float3 GetEmitterPosition() const { return mPos; }
+    float3 GetEmitterVelocity() const { return mVel; }
+
+    // CudaSFSystem side: push the configuration, pull the status
+    void Push(sf_handle h) const {
+        sf_emitter_params e{};
+        e.enable = (bEnable && bBuild) ? 1 : 0;
+        e.run = CUDA_SPH_EMITTER_PARAMS.run ? 1 : 0;
+        e.emit_pos = sf_float3{mPos.x, mPos.y, mPos.z};
+        e.emit_vel = sf_float3{mVel.x, mVel.y, mVel.z};
+        const float3 c = CUDA_SPH_EMITTER_PARAMS.emit_col;
+        e.emit_col = sf_float3{c.x, c.y, c.z};
+        e.emit_type = mType;
+        e.emit_radius = mRadius;
+        e.emit_width = mWidth;
+        e.emit_height = mHeight;
+        if (sf_set_emitter(h, &e) != SF_OK) throw std::runtime_error(std::string("sf_set_emitter: ") + sf_last_error());
+    }
+    void Pull(sf_handle h) {
+        int enabled = 0;
+        size_t points = 0;
+        sf_get_emitter(h, &enabled, &points, nullptr);
+        if (bEnable && bBuild && !enabled) bEnable = false;  // switched off at capacity
+        mPoints = points;
+    }
+
+  private:
+    void build(int type, float radius, float width, float height) {
+        mType = type;
+        mRadius = radius;
+        mWidth = width;
+        mHeight = height;
+        bBuild = true;
+    }
+    float3 mPos, mVel;
+    bool bEnable, bBuild = false;
+    int mType = SF_EMITTER_SQUARE;
+    float mRadius = 0.f, mWidth = 0.f, mHeight = 0.f;
+    size_t mPoints = 0;
 };
 typedef SharedPtr<CudaEmitter> CudaEmitterPtr;
 
@@ -284,8 +360,7 @@ class CudaSFSystem {
                  CudaEmitterPtr& emitter, bool adaptiveSubTimeStep = false)
         : mParticles(std::move(particles)), mBoundaries(std::move(boundaryParticles)), mSolver(std::move(solver)),
           mSearcher(std::move(searcher)), mBoundarySearcher(std::move(boundarySearcher)),
-          bAdaptiveSubTimeStep(adaptiveSubTimeStep) {
-        if (emitter && emitter->enable) throw std::runtime_error("the inflow emitter is not part of this library");
+          mEmitter(emitter), bAdaptiveSubTimeStep(adaptiveSubTimeStep) {
         const bool multi = CUDA_SEEPAGEFLOW_PARAMS.sf_type == SF_TYPE_MULTI_SF;
         sf_check(sf_create(&CUDA_SEEPAGEFLOW_PARAMS, &CUDA_BOUNDARY_PARAMS, &mSolver->Options(),
                            mParticles->MaxSize(), -1, &mHandle),
@@ -300,6 +375,21 @@ class CudaSFSystem {
             if (i < mBoundaries->mLabel.size()) bl[i] = mBoundaries->mLabel[i];
         }
         sf_check(sf_upload_boundary(mHandle, mBoundaries->mPos.size(), bp.data(), bl.data()), "sf_upload_boundary");
+        if (mEmitter && CUDA_SPH_EMITTER_PARAMS.enable) {  // cuda_sf_system.cu:36-58
+            const float r = CUDA_SEEPAGEFLOW_PARAMS.sph_particle_radius;
+            switch (CUDA_SPH_EMITTER_PARAMS.emit_type) {
+            case SQUARE: mEmitter->BuildSquareEmitter(r, CUDA_SPH_EMITTER_PARAMS.emit_radius); break;
+            case CIRCLE: mEmitter->BuildCircleEmitter(r, CUDA_SPH_EMITTER_PARAMS.emit_radius); break;
+            case RECTANGLE:
+                mEmitter->BuildRectangleEmitter(r, CUDA_SPH_EMITTER_PARAMS.emit_width,
+                                                CUDA_SPH_EMITTER_PARAMS.emit_height);
+                break;
+            }
+        }
+        if (mEmitter) {
+            mEmitter->Push(mHandle);
+            mEmitter->Pull(mHandle);
+        }
     }
     CudaSFSystem(const CudaSFSystem&) = delete;
     CudaSFSystem& operator=(const CudaSFSystem&) = delete;
@@ -310,9 +400,11 @@ class CudaSFSystem {
     // printed and the step returns its measured time in ms.
     float UpdateSystem(float renderInterval) {
         float ms = 0.f;
+        if (mEmitter) mEmitter->Push(mHandle);  // status, velocity and the run switch may have changed
         if (sf_set_params(mHandle, &CUDA_SEEPAGEFLOW_PARAMS) != SF_OK ||
             sf_update_system(mHandle, renderInterval, &ms) != SF_OK)
             std::printf("UpdateSystem failed: %s\n", sf_last_error());
+        if (mEmitter) mEmitter->Pull(mHandle);
         return ms;
     }
     size_t GetNumOfSubTimeSteps() const {
@@ -332,6 +424,7 @@ class CudaSFSystem {
     CudaSphSFSolverPtr mSolver;
     CudaGNSearcherPtr mSearcher;
     CudaGNBoundarySearcherPtr mBoundarySearcher;
+    CudaEmitterPtr mEmitter;
     bool bAdaptiveSubTimeStep;
     sf_handle mHandle = nullptr;
 };

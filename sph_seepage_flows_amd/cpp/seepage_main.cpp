@@ -9,7 +9,11 @@
 // jitter rules as sph_seepage_flows_amd/scenes.py, so its frames can be
 // compared bit for bit with the Python host.
 //
-//   seepage_main [--frames N] [--out FOLDER] [--render-interval S] [--fixed] [--export-initial]
+//   seepage_main [--frames N] [--out FOLDER] [--render-interval S] [--fixed] [--export-initial] [--emit]
+//
+// --emit switches on main.cpp's inflow emitter (main.cpp:85-95: a circle of
+// radius 0.17 at the box centre + (size/10, size/5, size/2.3), velocity
+// (0, 0, -5)) with room for 20,000 more particles.
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -51,9 +55,9 @@ void lattice(Vec_Float3& out, float3 o, float spacing, int nx, int ny, int nz, s
             }
 }
 
-CudaSFSystemPtr SetupParams() {
+CudaSFSystemPtr SetupParams(bool emit) {
     const float3 lo = make_float3(0.f, 0.f, 0.f), hi = make_float3(1.f, 1.f, 1.f);
-    CUDA_SEEPAGEFLOW_APP_PARAMS.max_num = 24576;
+    CUDA_SEEPAGEFLOW_APP_PARAMS.max_num = emit ? 24576 + 20000 : 24576;
     auto& P = CUDA_SEEPAGEFLOW_PARAMS;
     const float r = 0.01f, rs = 0.01f, rho0 = 1000.f, cs = 100.f;
     P.sph_density = rho0;
@@ -113,7 +117,20 @@ CudaSFSystemPtr SetupParams() {
     auto boundary_particles = std::make_shared<CudaBoundaryParticles>(boundaryData.pos, boundaryData.label);
     auto solver = std::make_shared<CudaWCSphSFSolver>(particles->MaxSize(), 0.f, 3.f, 40.f);
     auto boundary_searcher = std::make_shared<CudaGNBoundarySearcher>(lo, hi, boundary_particles->MaxSize(), B.kernel_radius);
-    CudaEmitterPtr emitter = std::make_shared<CudaEmitter>();
+    // sph emitter (main.cpp:85-95, 106-111)
+    const float3 size = make_float3(hi.x - lo.x, hi.y - lo.y, hi.z - lo.z);
+    const float3 center = make_float3((hi.x + lo.x) / 2.f, (hi.y + lo.y) / 2.f, (hi.z + lo.z) / 2.f);
+    auto& E = CUDA_SPH_EMITTER_PARAMS;
+    E.enable = true;
+    E.run = emit;
+    E.emit_pos = make_float3(center.x + size.x / 10.f, center.y + size.y / 5.f, center.z + size.z / 2.3f);
+    E.emit_vel = make_float3(0.f, 0.f, -5.f);
+    E.emit_col = make_float3(127.f / 255.f, 205.f / 255.f, 255.f / 255.f);
+    E.emit_radius = 0.17f;
+    E.emit_width = 0.22f;
+    E.emit_height = 0.18f;
+    E.emit_type = CIRCLE;
+    CudaEmitterPtr emitter = std::make_shared<CudaEmitter>(E.emit_pos, E.emit_vel, E.enable);
     std::printf("Number of total particles = %zu, boundary particles = %zu\n", particles->Size(), boundary_particles->Size());
     return std::make_shared<CudaSFSystem>(particles, boundary_particles, solver, searcher, boundary_searcher, emitter,
                                           true);
@@ -123,7 +140,7 @@ CudaSFSystemPtr SetupParams() {
 
 int main(int argc, char** argv) {
     int frames = 2;
-    bool adaptive = true, export_initial = false;
+    bool adaptive = true, export_initial = false, emit = false;
     float render_interval = 1.f / 30.f;
     std::string out = "export/bgeo/seepage_small_block";
     for (int i = 1; i < argc; ++i) {
@@ -132,13 +149,14 @@ int main(int argc, char** argv) {
         else if (!std::strcmp(argv[i], "--render-interval") && i + 1 < argc) render_interval = (float)std::atof(argv[++i]);
         else if (!std::strcmp(argv[i], "--fixed")) adaptive = false;
         else if (!std::strcmp(argv[i], "--export-initial")) export_initial = true;
+        else if (!std::strcmp(argv[i], "--emit")) emit = true;
     }
     std::strncpy(CUDA_SEEPAGEFLOW_APP_PARAMS.bgeo_export_folder, out.c_str(),
                  sizeof(CUDA_SEEPAGEFLOW_APP_PARAMS.bgeo_export_folder) - 1);
     CUDA_SEEPAGEFLOW_APP_PARAMS.bgeo_export = true;
     CudaSFSystemPtr system;
     try {
-        system = SetupParams();
+        system = SetupParams(emit);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "setup failed: %s\n", e.what());
         return 1;

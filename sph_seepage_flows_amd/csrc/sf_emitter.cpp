@@ -143,6 +143,11 @@ int sf_set_emitter(sf_handle h, const sf_emitter_params* p) {
     const float r = h->P.sph_particle_radius;
     if (!(r > 0.f)) return set_error(SF_E_INVALID, "sph_particle_radius must be > 0");
     EmitterState e;
+    // the emission counter, elapsed time and count belong to the system
+    // (CudaSFSystem members, zero from its construction), not to the emitter
+    e.counter = h->emitter.counter;
+    e.elapsed = h->emitter.elapsed;
+    e.emitted = h->emitter.emitted;
     e.enable = p->enable != 0;
     e.run = p->run != 0;
     e.pos = p->emit_pos;
