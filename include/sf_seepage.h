@@ -26,6 +26,10 @@
  *   sf_export_bgeo       <- ExportBgeoFileCUDA  seepage_flows/include/kiri_utils.h:364-448
  *   sf_write_bgeo        <- Partio::writeBGEO (uncompressed V5)  extlib/include/partio/io/BGEO.cpp:311-476
  *   sf_build_world_boundary <- CudaBoundaryEmitter::BuildWorldBoundary  cuda_boundary_emitter.cpp:12-80
+ *   sf_read_bgeo         <- ReadBgeoFileForGPU  seepage_flows/include/kiri_utils.h:252-308
+ *                           over Partio::readBGEO  extlib/include/partio/io/BGEO.cpp:196-309
+ *   sf_build_seepage_shape_volume / sf_build_seepage_shape_multi_volume
+ *                        <- CudaVolumeEmitter::BuildSeepageflowShape(Multi)Volume  cuda_volume_emitter.cpp:109-175
  *   sf_grid_size         <- make_int3((highest - lowest) / kernel_radius)  main.cpp:104, cuda_neighbor_searcher.cu:26
  *   sf_set_emitter / sf_get_emitter <- CudaEmitter + CudaSphEmitterParams (see their declarations)
  *   sf_batch_*, sf_step_graph, sf_*_launch_config, sf_autotune, sf_set_field, sf_diff_*,
@@ -176,6 +180,43 @@ int sf_build_world_boundary(sf_float3 lowest, sf_float3 highest, float particle_
 /* Writes an uncompressed Houdini bgeo V5 file with position, Cd, pscale, label. */
 int sf_write_bgeo(const char* path, size_t n, const float* pos3, const float* col3,
                   const float* radius, const uint64_t* label);
+
+/* ---- scene input (host only) --------------------------------------------- */
+/* Every point of a classic Houdini bgeo V5 file (uncompressed or gzip) as
+ * (x, y, z, pscale) float4 rows; pscale is 0.01 where the file has no
+ * "pscale" point attribute, and flip_yz stores (x, z, y).  shape4 == NULL
+ * only counts (*n_out).  Attribute types float / int / vector / indexed
+ * string are read past; a string attribute, another version or the newer
+ * JSON bgeo are SF_E_IO. */
+int sf_read_bgeo(const char* path, int flip_yz, float* shape4, size_t capacity, size_t* n_out);
+
+/* SeepageflowVolumeData / SeepageflowMultiVolumeData (cuda_volume_emitter.cuh:43-63)
+ * as caller-owned arrays of `capacity` entries, `size` of them in use. */
+typedef struct sf_volume_data {
+    float* pos3;
+    float* col3;
+    uint64_t* label;
+    float* mass;
+    float* radius;
+    float* cda0asat3;       /* multi-material builder only */
+    float* amcamcp2;        /* multi-material builder only */
+    size_t size;            /* in/out */
+    size_t capacity;
+    float sand_min_radius;  /* in/out */
+} sf_volume_data;
+/* Appends the n shape points (sf_read_bgeo rows) as sand, label 1, with
+ * mass = sand_density * ((4/3) pi r^3).  sand_min_radius restarts from
+ * float(SIZE_MAX) on every call (the multi builder: only while size == 0)
+ * and takes the smallest r.  offset_y != 0 then lowers every particle held
+ * (earlier shapes too) by min_y - (world_lowest_y + 5 sand_min_radius), with
+ * min_y this shape's lowest y.  SF_E_CAPACITY if size + n > capacity. */
+int sf_build_seepage_shape_volume(sf_volume_data* d, const float* shape4, size_t n, sf_float3 color,
+                                  float sand_density, int offset_y, float world_lowest_y);
+/* The same, also writing (cd, a0, asat) and (amc, amcp) for the shape's
+ * particles (the MULTI_SF per-particle coefficients). */
+int sf_build_seepage_shape_multi_volume(sf_volume_data* d, const float* shape4, size_t n, sf_float3 color,
+                                        float sand_density, sf_float3 cda0asat, float amc, float amcp,
+                                        int offset_y, float world_lowest_y);
 
 /* ---- scene lifetime ----------------------------------------------------- */
 /* opts may be NULL (reference defaults). device < 0 keeps the current device. */

@@ -106,6 +106,13 @@ class sf_emitter_params(C.Structure):
                 ("emit_width", C.c_float), ("emit_height", C.c_float)]
 
 
+class sf_volume_data(C.Structure):
+    """SeepageflowVolumeData / SeepageflowMultiVolumeData (cuda_volume_emitter.cuh:43-63) as caller arrays."""
+    _fields_ = [("pos3", C.c_void_p), ("col3", C.c_void_p), ("label", C.c_void_p), ("mass", C.c_void_p),
+                ("radius", C.c_void_p), ("cda0asat3", C.c_void_p), ("amcamcp2", C.c_void_p), ("size", C.c_size_t),
+                ("capacity", C.c_size_t), ("sand_min_radius", C.c_float)]
+
+
 class sf_scene_params(C.Structure):
     """Per-scene overrides of a batched handle (names as in sf_params)."""
     _fields_ = [(n, C.c_float) for n in ("dem_tan_friction_angle", "sf_c0", "sf_cmc", "sf_cmc_p", "sf_csat",
@@ -123,6 +130,12 @@ _SIG = {
     "sf_build_world_boundary": (C.c_int, [sf_float3, sf_float3, C.c_float, C.c_size_t, _VP, _VP,
                                           C.POINTER(C.c_size_t)]),
     "sf_write_bgeo": (C.c_int, [C.c_char_p, C.c_size_t, _VP, _VP, _VP, _VP]),
+    "sf_read_bgeo": (C.c_int, [C.c_char_p, C.c_int, _VP, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "sf_build_seepage_shape_volume": (C.c_int, [C.POINTER(sf_volume_data), _VP, C.c_size_t, sf_float3, C.c_float,
+                                                C.c_int, C.c_float]),
+    "sf_build_seepage_shape_multi_volume": (C.c_int, [C.POINTER(sf_volume_data), _VP, C.c_size_t, sf_float3,
+                                                      C.c_float, sf_float3, C.c_float, C.c_float, C.c_int,
+                                                      C.c_float]),
     "sf_create": (C.c_int, [C.POINTER(sf_params), C.POINTER(sf_boundary_params), C.POINTER(sf_solver_options),
                             C.c_size_t, C.c_int, C.POINTER(_VP)]),
     "sf_destroy": (C.c_int, [_VP]),

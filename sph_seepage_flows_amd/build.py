@@ -20,7 +20,7 @@ LIB = LIBDIR / "libsf_seepage.so"
 INCLUDE = PKG.parent / "include"
 
 SOURCES = ["sf_searcher.hip", "sf_solver.hip", "sf_adjoint.hip", "sf_api.cpp", "sf_diff.cpp", "sf_emitter.cpp",
-           "sf_host.cpp"]
+           "sf_host.cpp", "sf_scene.cpp"]
 ARCH = os.environ.get("SF_OFFLOAD_ARCH", "gfx950")
 
 
@@ -73,7 +73,7 @@ def build(verbose: bool = False) -> Path:
             if rc:
                 raise RuntimeError(f"hipcc failed ({rc}): {cmd}\n{out}")
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-lz", "-o", str(tmp)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}{r.stderr}")

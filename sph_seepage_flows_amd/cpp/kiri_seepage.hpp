@@ -24,6 +24,12 @@
 // export takes the particle object instead of four device pointers.
 //   CudaEmitter / CUDA_SPH_EMITTER_PARAMS             -> sf_set_emitter
 //     (emitter/cuda_emitter.cpp:12-97, system/cuda_sf_system.cu:36-58,86-115)
+//   ReadBgeoFileForGPU / ReadMultiBgeoFilesForGPU     -> sf_read_bgeo
+//     (seepage_flows/include/kiri_utils.h:252-361); the root folder is
+//     DB_PBR_PATH when the caller defines it (as the reference's build does),
+//     else DbPbrPath(), settable at run time
+//   CudaVolumeEmitter::BuildSeepageflowShape(Multi)Volume
+//     -> sf_build_seepage_shape_(multi_)volume (emitter/cuda_volume_emitter.cpp:109-175)
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -45,7 +51,12 @@ using SharedPtr = std::shared_ptr<T>;
 using Vec_Float = std::vector<float>;
 using Vec_Float2 = std::vector<float2>;
 using Vec_Float3 = std::vector<float3>;
+using Vec_Float4 = std::vector<float4>;
 using Vec_SizeT = std::vector<size_t>;
+using String = std::string;
+using Vec_String = std::vector<std::string>;
+static_assert(sizeof(float3) == 12 && sizeof(float2) == 8 && sizeof(float4) == 16 && sizeof(size_t) == 8,
+              "the volume builders write the vectors' storage as packed float / uint64 arrays");
 
 enum CudaSphSolverType { SPH_SOLVER = SF_SPH_SOLVER, WCSPH_SOLVER = SF_WCSPH_SOLVER, IISPH_SOLVER = 2 };
 enum CudaSeepageflowType { SF = SF_TYPE_SF, MULTI_SF = SF_TYPE_MULTI_SF };
@@ -120,6 +131,118 @@ class CudaBoundaryEmitter {
   private:
     bool bEnable;
 };
+
+// Root folder of the shape files: <root>bgeo/<Folder>/<Name>.bgeo.
+inline std::string& DbPbrPath() {
+#ifdef DB_PBR_PATH
+    static std::string root = DB_PBR_PATH;
+#else
+    static std::string root = "./";
+#endif
+    return root;
+}
+
+// ReadBgeoFileForGPU (kiri_utils.h:252-308): (x, y, z, pscale) of every point,
+// pscale 0.01 where the file has none, y and z swapped with flip_yz.  A file
+// that cannot be read throws (the reference dereferenced a null reader).
+inline Vec_Float4 ReadBgeoFileForGPU(String Folder, String Name, bool flip_yz = false) {
+    const std::string path = DbPbrPath() + "bgeo/" + Folder + "/" + Name + ".bgeo";
+    size_t n = 0;
+    sf_check(sf_read_bgeo(path.c_str(), flip_yz ? 1 : 0, nullptr, 0, &n), "sf_read_bgeo");
+    Vec_Float4 out(n);
+    sf_check(sf_read_bgeo(path.c_str(), flip_yz ? 1 : 0, reinterpret_cast<float*>(out.data()), n, &n),
+             "sf_read_bgeo");
+    if (n > 0) std::printf("pscale=%g\n", out[0].w);
+    return out;
+}
+
+// ReadMultiBgeoFilesForGPU (kiri_utils.h:310-361): the files' points in order.
+inline Vec_Float4 ReadMultiBgeoFilesForGPU(Vec_String folders, Vec_String file_names, bool flip_yz = false) {
+    Vec_Float4 all;
+    for (size_t k = 0; k < folders.size() && k < file_names.size(); ++k) {
+        Vec_Float4 one = ReadBgeoFileForGPU(folders[k], file_names[k], flip_yz);
+        std::printf("Loaded Bgeo File=%s, Number of Particles=%zu\n", file_names[k].c_str(), one.size());
+        all.insert(all.end(), one.begin(), one.end());
+    }
+    return all;
+}
+
+// SeepageflowVolumeData / SeepageflowMultiVolumeData (cuda_volume_emitter.cuh:43-63)
+struct SeepageflowVolumeData {
+    float sandMinRadius = 0.f;
+    Vec_Float3 pos;
+    Vec_Float3 col;
+    Vec_Float mass;
+    Vec_Float radius;
+    Vec_SizeT label;
+};
+struct SeepageflowMultiVolumeData {
+    float sandMinRadius = 0.f;
+    Vec_Float3 pos;
+    Vec_Float3 col;
+    Vec_Float mass;
+    Vec_Float radius;
+    Vec_SizeT label;
+    Vec_Float2 amcamcp;
+    Vec_Float3 cda0asat;
+};
+
+class CudaVolumeEmitter {
+  public:
+    explicit CudaVolumeEmitter(bool enable = true) : bEnable(enable) {}
+    CudaVolumeEmitter(const CudaVolumeEmitter&) = delete;
+    CudaVolumeEmitter& operator=(const CudaVolumeEmitter&) = delete;
+
+    void BuildSeepageflowShapeVolume(SeepageflowVolumeData& data, Vec_Float4 shape, float3 color, float sandDensity,
+                                     bool offsetY = false, float worldLowestY = 0.f) {
+        if (!bEnable) return;
+        sf_volume_data d = grow(data, shape.size());
+        sf_check(sf_build_seepage_shape_volume(&d, reinterpret_cast<const float*>(shape.data()), shape.size(),
+                                               to_sf(color), sandDensity, offsetY ? 1 : 0, worldLowestY),
+                 "sf_build_seepage_shape_volume");
+        data.sandMinRadius = d.sand_min_radius;
+    }
+    void BuildSeepageflowShapeMultiVolume(SeepageflowMultiVolumeData& data, Vec_Float4 shape, float3 color,
+                                          float sandDensity, float3 cda0asat, float2 amcamcp, bool offsetY = false,
+                                          float worldLowestY = 0.f) {
+        if (!bEnable) return;
+        sf_volume_data d = grow(data, shape.size());
+        data.cda0asat.resize(data.pos.size());
+        data.amcamcp.resize(data.pos.size());
+        d.cda0asat3 = reinterpret_cast<float*>(data.cda0asat.data());
+        d.amcamcp2 = reinterpret_cast<float*>(data.amcamcp.data());
+        sf_check(sf_build_seepage_shape_multi_volume(&d, reinterpret_cast<const float*>(shape.data()), shape.size(),
+                                                     to_sf(color), sandDensity, to_sf(cda0asat), amcamcp.x,
+                                                     amcamcp.y, offsetY ? 1 : 0, worldLowestY),
+                 "sf_build_seepage_shape_multi_volume");
+        data.sandMinRadius = d.sand_min_radius;
+    }
+    bool GetEmitterStatus() const { return bEnable; }
+
+  private:
+    // room for n more particles; the arrays of `data` become the C-ABI's
+    template <class V>
+    static sf_volume_data grow(V& data, size_t n) {
+        const size_t held = data.pos.size();
+        data.pos.resize(held + n);
+        data.col.resize(held + n);
+        data.mass.resize(held + n);
+        data.radius.resize(held + n);
+        data.label.resize(held + n);
+        sf_volume_data d{};
+        d.pos3 = reinterpret_cast<float*>(data.pos.data());
+        d.col3 = reinterpret_cast<float*>(data.col.data());
+        d.label = reinterpret_cast<uint64_t*>(data.label.data());
+        d.mass = data.mass.data();
+        d.radius = data.radius.data();
+        d.size = held;
+        d.capacity = held + n;
+        d.sand_min_radius = data.sandMinRadius;
+        return d;
+    }
+    bool bEnable;
+};
+typedef SharedPtr<CudaVolumeEmitter> CudaVolumeEmitterPtr;
 
 class CudaSFSystem;
 

@@ -76,6 +76,9 @@ class Scene:
     blabel: np.ndarray | None = None
     meta: dict = field(default_factory=dict)
     max_sat: np.ndarray | None = None  # initial maxSaturation history (None: zero)
+    # sf_params fields set after the Physics derivations (as a driver that
+    # changes CUDA_SEEPAGEFLOW_PARAMS after computing it does)
+    overrides: dict = field(default_factory=dict)
 
     @property
     def n(self) -> int:
@@ -86,7 +89,10 @@ class Scene:
         return 0 if self.bpos is None else int(self.bpos.shape[0])
 
     def params(self):
-        return P.make_params(self.physics)
+        p, b, o = P.make_params(self.physics)
+        for k, v in self.overrides.items():
+            setattr(p, k, v)
+        return p, b, o
 
 
 def world_boundary(lo, hi, r) -> tuple[np.ndarray, np.ndarray]:
@@ -324,3 +330,41 @@ def column(seed: int = 4, n_sand: int = 48_000, n_water: int = 16_000, width: fl
 
 
 SCENES = {"small_block": small_block, "drop_in_1m": drop_in_1m, "mixed": mixed, "dam": dam, "column": column}
+
+
+def main_emitter(ph: P.Physics) -> dict:
+    """The reference driver's inflow emitter (main.cpp:85-95) for the box of
+    `ph`: a disc of radius 0.17 at centre + (size.x / 10, size.y / 5,
+    size.z / 2.3) shooting along -z at 5 m/s; keyword arguments of
+    SFSystem.set_emitter."""
+    from ._abi import SF_EMITTER_CIRCLE
+    lo, hi = np.array(ph.lo, np.float32), np.array(ph.hi, np.float32)
+    size, centre = hi - lo, (hi + lo) / np.float32(2.0)
+    pos = centre + size / np.array([10.0, 5.0, 2.3], np.float32)
+    col = np.array([127.0, 205.0, 255.0], np.float32) / np.float32(255.0)
+    return dict(pos=tuple(float(v) for v in pos), vel=(0.0, 0.0, -5.0), col=tuple(float(v) for v in col),
+                kind=SF_EMITTER_CIRCLE, radius=0.17, width=0.22, height=0.18)
+
+
+def shape_scene(volume, name: str = "shape", lo=(0.0, 0.0, 0.0), hi=(1.5, 1.5, 2.5)) -> Scene:
+    """The reference driver's bunny scene (main.cpp:25-221) around sand built
+    from bgeo shapes (`volume`, a scene_io.VolumeData of multi-material
+    shapes): 1.5 x 1.5 x 2.5 m box, water r = 0.01 (h = 0.04, no water at
+    start: it comes from `main_emitter`), sand density 2700, E = 1e5,
+    MULTI_SF, gravity -9.8, solver speed of sound 100 (the solver's default),
+    dem_particle_radius = the smallest shape radius and
+    dt = 0.5 R / sqrt(E / rho_s) with that R (main.cpp:171-172); dem_mass
+    keeps its value for r = 0.005 as main.cpp:61 computes it first."""
+    ph = P.Physics(lo=tuple(lo), hi=tuple(hi), r_water=0.01, r_sand=0.005, rho_sand=2700.0, young=1e5,
+                   gravity=(0.0, -9.8, 0.0), c0=100.0, sf_type=SF_TYPE_MULTI_SF)
+    R = np.float32(volume.sand_min_radius)
+    dt = np.float32(0.5) * R / np.sqrt(np.float32(ph.young) / np.float32(ph.rho_sand))
+    ph.dt = float(dt)
+    if len(volume.cda0asat) != volume.n:
+        raise ValueError("shape_scene takes a multi-material volume (build_shape_multi_volume)")
+    bpos, blab = world_boundary(ph.lo, ph.hi, ph.r_water)
+    return Scene(name, ph, np.ascontiguousarray(volume.pos), volume.label.copy(), volume.mass.copy(),
+                 volume.radius.copy(), np.ascontiguousarray(volume.col), None, volume.cda0asat.copy(),
+                 volume.amcamcp.copy(), bpos, blab, meta={"emitter": main_emitter(ph), "max_num": 500_000},
+                 overrides={"dem_particle_radius": float(R)})
+

@@ -77,3 +77,61 @@ def test_cpp_driver_with_emitter_matches_python_host(tmp_path):
     np.testing.assert_array_equal(p1, g.get("pos"))
     np.testing.assert_array_equal(c1, g.get("col"))
     np.testing.assert_array_equal(l1.astype(np.uint64), g.get("label"))
+
+
+def _ball(centre, radius, spacing):
+    k = np.arange(-radius, radius + 1e-9, spacing)
+    g = np.stack(np.meshgrid(k, k, k, indexing="ij"), -1).reshape(-1, 3)
+    return (g[(g ** 2).sum(1) <= radius * radius] + np.asarray(centre)).astype(np.float32)
+
+
+def test_cpp_driver_shape_scene_matches_python_host(tmp_path):
+    """main.cpp's own scene set-up (shape objects read from bgeo, multi-material
+    volume on the floor, circle emitter from the first frame): the driver's
+    initial particles and first frame equal the Python host's
+    (scene_io + scenes.shape_scene) bit for bit."""
+    from sph_seepage_flows_amd import scene_io, write_bgeo
+    res = tmp_path / "res"
+    # a sand ball in place of the bunny asset, and a block of smaller grains
+    ball = _ball((0.75, 0.4, 1.25), 0.12, 0.01)
+    i, j, k = np.meshgrid(np.arange(20), np.arange(6), np.arange(20), indexing="ij")
+    block = (np.stack([i, j, k], -1).reshape(-1, 3) * np.float32(0.008) + np.float32([0.2, 0.025, 0.3])).astype(
+        np.float32)
+    objs = [("bunny", "bunny", ball, 0.005, 0.5), ("dam", "dam2", block, 0.004, 0.015)]
+    for folder, name, pts, ps, _ in objs:
+        path = Path(scene_io.pbr_path(res, folder, name))
+        path.parent.mkdir(parents=True)
+        write_bgeo(str(path), pts, np.zeros_like(pts), np.full(len(pts), ps, np.float32), np.ones(len(pts), np.uint64))
+    out = tmp_path / "frames"
+    r = subprocess.run([str(DRIVER), "--frames", "1", "--out", str(out), "--export-initial", "--resources", str(res),
+                        "--shape", "bunny/bunny", "--shape", "dam/dam2:0.015"], capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    v = scene_io.VolumeData()
+    dry = (0.88, 0.79552, 0.5984)
+    for folder, name, _, _, cd in objs:
+        shape = scene_io.read_bgeo(scene_io.pbr_path(res, folder, name))
+        v.build_shape_multi_volume(shape, dry, 2700.0, (cd, 0.0, 0.8), (1.5, 0.5), offset_y=True, world_lowest_y=0.0)
+    sc = scenes.shape_scene(v)
+    assert sc.n == len(ball) + len(block) and v.sand_min_radius == np.float32(0.004)
+    p0, c0, rad0, l0 = read_bgeo(out / "0000.bgeo")
+    np.testing.assert_array_equal(p0, sc.pos)
+    np.testing.assert_array_equal(c0, sc.col)
+    np.testing.assert_array_equal(rad0, sc.radius)
+    np.testing.assert_array_equal(l0.astype(np.uint64), sc.label)
+    g = SFSystem.from_scene(sc, max_num=sc.meta["max_num"])
+    g.set_emitter(**sc.meta["emitter"])
+    remaining = np.float32(1.0 / 30.0)
+    sub = 0
+    while remaining > np.float32(1e-6):
+        g.UpdateSystem(float(remaining))
+        remaining = np.float32(remaining - remaining / np.float32(g.GetNumOfSubTimeSteps()))
+        sub += 1
+    enabled, k, emitted = g.emitter_state()
+    print(f"shape scene: {sc.n} sand, {sub} sub-steps, {emitted} emitted", flush=True)
+    assert emitted > 0
+    p1, c1, rad1, l1 = read_bgeo(out / "0001.bgeo")
+    np.testing.assert_array_equal(p1, g.get("pos"))
+    np.testing.assert_array_equal(c1, g.get("col"))
+    np.testing.assert_array_equal(l1.astype(np.uint64), g.get("label"))
