@@ -190,6 +190,43 @@ def diff_bench(dt: float, check: bool) -> dict:
     return out
 
 
+def frame_bench(scene, frames: int) -> dict:
+    """The reference driver's adaptive frame loop (main.cpp:235-247, 1/30 s
+    frames): sf_update_system called per sub-step from a Python host loop
+    (the loop main.cpp runs, one host read of the sub-step count per call)
+    against sf_update_frame (the same loop inside the library; identical
+    results).  Wall-clock ms per frame after one untimed frame each."""
+    import numpy as np
+
+    import sph_seepage_flows_amd as sf
+
+    ri = 1.0 / 30.0
+
+    def host_frame(g):
+        rem, k = np.float32(ri), 0
+        while rem > np.float32(1e-6):
+            g.UpdateSystem(float(rem))
+            rem = np.float32(rem - rem / np.float32(g.GetNumOfSubTimeSteps()))
+            k += 1
+        return k
+
+    res = {"workload": f"{scene.name}: {scene.n} particles, 1/30 s frames", "frames": frames}
+    for mode in ("host_loop", "device_frame"):
+        g = sf.SFSystem.from_scene(scene, device=0)
+        run = host_frame if mode == "host_loop" else (lambda g: g.update_frame(ri)[0])
+        run(g)
+        g.synchronize()
+        t0 = time.perf_counter()
+        subs = sum(run(g) for _ in range(frames))
+        g.synchronize()
+        wall = time.perf_counter() - t0
+        g.close()
+        res[mode] = {"ms_per_frame": wall * 1e3 / frames, "sub_steps_per_frame": subs / frames,
+                     "us_per_sub_step": wall * 1e6 / subs, "particle_steps_per_s": scene.n * subs / wall}
+    res["speedup"] = res["host_loop"]["ms_per_frame"] / res["device_frame"]["ms_per_frame"]
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,8 +237,8 @@ def main() -> int:
     ap.add_argument("--workload", default="drop_in_1m", choices=["drop_in_1m", "small_block", "dam_4m"])
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-window measurement")
-    ap.add_argument("--parts", default="step,ensemble,diff",
-                    help="comma list of step (the headline workload), ensemble, diff; a profiler can run one")
+    ap.add_argument("--parts", default="step,ensemble,diff,frame",
+                    help="comma list of step (the headline workload), ensemble, diff, frame; a profiler can run one")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -235,6 +272,7 @@ def main() -> int:
         "cpu_baseline": None,
         "batched": None,
         "differentiation": None,
+        "frame_loop": None,
     }
     if "step" in parts:
         print(f"[bench] scene {scene.name}: n={scene.n} nb={scene.nb}", file=sys.stderr, flush=True)
@@ -302,6 +340,10 @@ def main() -> int:
         print("[bench] differentiable window...", file=sys.stderr, flush=True)
         out["differentiation"] = diff_bench(dt, check=not args.no_cpu_baseline)
         print(f"[bench] differentiation: {json.dumps(out['differentiation'])}", file=sys.stderr, flush=True)
+    if "frame" in parts:
+        print("[bench] frame loop...", file=sys.stderr, flush=True)
+        out["frame_loop"] = [frame_bench(scenes.small_block(), 5), frame_bench(scene, 3)]
+        print(f"[bench] frame loop: {json.dumps(out['frame_loop'])}", file=sys.stderr, flush=True)
     if rank == 0 and "step" in parts and not args.no_cpu_baseline:
         print(f"[bench] cpu baseline ({args.cpu_steps} oracle steps)...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_steps, dt)

@@ -21,6 +21,7 @@
  *   sf_update_system     <- CudaBaseSystem::UpdateSystem  cuda_base_system.cu:29-56 (-> CudaSFSystem::OnUpdateSolver cuda_sf_system.cu:66-119)
  *   sf_step              <- the same per-step pass sequence with a caller-fixed dt (cuda_wcsph_sf_solver.cpp:44-177)
  *   sf_num_sub_steps     <- CudaBaseSystem::GetNumOfSubTimeSteps  cuda_base_system.cuh:38
+ *   sf_update_frame      <- the frame loop of Update()  seepage_flows/src/seepageflow/main.cpp:235-259
  *   sf_download / sf_device_buffer <- CudaSFParticles::Get*Ptr()  cuda_sf_particles.cuh:170-190,
  *                           CudaGNBaseSearcher::GetCellStart()  cuda_neighbor_searcher.cuh:44-50
  *   sf_export_bgeo       <- ExportBgeoFileCUDA  seepage_flows/include/kiri_utils.h:364-448
@@ -30,6 +31,7 @@
  *                           over Partio::readBGEO  extlib/include/partio/io/BGEO.cpp:196-309
  *   sf_build_seepage_shape_volume / sf_build_seepage_shape_multi_volume
  *                        <- CudaVolumeEmitter::BuildSeepageflowShape(Multi)Volume  cuda_volume_emitter.cpp:109-175
+ *   sf_bgeo_to_vtk       <- Partio2VTK  seepage_flows/include/kiri_utils.h:54-250
  *   sf_grid_size         <- make_int3((highest - lowest) / kernel_radius)  main.cpp:104, cuda_neighbor_searcher.cu:26
  *   sf_set_emitter / sf_get_emitter <- CudaEmitter + CudaSphEmitterParams (see their declarations)
  *   sf_batch_*, sf_step_graph, sf_*_launch_config, sf_autotune, sf_set_field, sf_diff_*,
@@ -189,6 +191,12 @@ int sf_write_bgeo(const char* path, size_t n, const float* pos3, const float* co
  * string are read past; a string attribute, another version or the newer
  * JSON bgeo are SF_E_IO. */
 int sf_read_bgeo(const char* path, int flip_yz, float* shape4, size_t capacity, size_t* n_out);
+/* Converts a bgeo V5 file to a binary legacy VTK 4.1 unstructured grid: the
+ * positions as POINTS, one vertex cell per particle (its point is the "id"
+ * attribute when the file has one, else the particle index), and every other
+ * float / int / vector point attribute as POINT_DATA FIELD arrays with all
+ * their components.  An empty file writes nothing. */
+int sf_bgeo_to_vtk(const char* bgeo_path, const char* vtk_path);
 
 /* SeepageflowVolumeData / SeepageflowMultiVolumeData (cuda_volume_emitter.cuh:43-63)
  * as caller-owned arrays of `capacity` entries, `size` of them in use. */
@@ -244,6 +252,19 @@ int sf_step(sf_handle h, float dt);
 int sf_step_n(sf_handle h, float dt, int n_steps);
 int sf_update_system(sf_handle h, float render_interval, float* ms_out);
 int sf_num_sub_steps(sf_handle h, size_t* out);
+/* One frame of the reference driver's loop, with the results of calling
+ * sf_update_system from the host the way main.cpp:235-259 does:
+ *   adaptive != 0:  remaining = render_interval;
+ *                   while (remaining > 1e-6f) { sf_update_system(h, remaining);
+ *                                               remaining -= remaining / nsub; }
+ *   adaptive == 0:  k = nsub (as left by the previous call);
+ *                   k times sf_update_system(h, render_interval);
+ * The remaining-time update and the emitter clock run on the device, and the
+ * host reads them back only after as many sub-steps as the loop certainly
+ * still runs (the CFL dt has a fixed upper bound); the particle state, nsub
+ * and the emitter state afterwards are those of the host loop, bit for bit.
+ * *sub_steps (may be NULL) = sub-steps run; *ms_out (may be NULL) = GPU time. */
+int sf_update_frame(sf_handle h, float render_interval, int adaptive, size_t* sub_steps, float* ms_out);
 /* Passes of one step, in launch order, for sf_step_profiled. */
 enum sf_pass {
     SF_PASS_HASH = 0, SF_PASS_SORT = 1, SF_PASS_GATHER = 2, SF_PASS_CELL_START = 3,

@@ -131,6 +131,7 @@ _SIG = {
                                           C.POINTER(C.c_size_t)]),
     "sf_write_bgeo": (C.c_int, [C.c_char_p, C.c_size_t, _VP, _VP, _VP, _VP]),
     "sf_read_bgeo": (C.c_int, [C.c_char_p, C.c_int, _VP, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "sf_bgeo_to_vtk": (C.c_int, [C.c_char_p, C.c_char_p]),
     "sf_build_seepage_shape_volume": (C.c_int, [C.POINTER(sf_volume_data), _VP, C.c_size_t, sf_float3, C.c_float,
                                                 C.c_int, C.c_float]),
     "sf_build_seepage_shape_multi_volume": (C.c_int, [C.POINTER(sf_volume_data), _VP, C.c_size_t, sf_float3,
@@ -150,6 +151,7 @@ _SIG = {
     "sf_update_system": (C.c_int, [_VP, C.c_float, C.POINTER(C.c_float)]),
     "sf_step_profiled": (C.c_int, [_VP, C.c_float, C.c_int, _VP, C.POINTER(C.c_float)]),
     "sf_num_sub_steps": (C.c_int, [_VP, C.POINTER(C.c_size_t)]),
+    "sf_update_frame": (C.c_int, [_VP, C.c_float, C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_float)]),
     "sf_size": (C.c_int, [_VP, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "sf_download": (C.c_int, [_VP, C.c_int, _VP, C.c_size_t]),
     "sf_device_buffer": (C.c_int, [_VP, C.c_int, C.POINTER(_VP), C.POINTER(C.c_size_t)]),

@@ -28,6 +28,7 @@
 //     (seepage_flows/include/kiri_utils.h:252-361); the root folder is
 //     DB_PBR_PATH when the caller defines it (as the reference's build does),
 //     else DbPbrPath(), settable at run time
+//   Partio2VTK(vtk, bgeo)                             -> sf_bgeo_to_vtk (kiri_utils.h:54-250)
 //   CudaVolumeEmitter::BuildSeepageflowShape(Multi)Volume
 //     -> sf_build_seepage_shape_(multi_)volume (emitter/cuda_volume_emitter.cpp:109-175)
 #pragma once
@@ -165,6 +166,12 @@ inline Vec_Float4 ReadMultiBgeoFilesForGPU(Vec_String folders, Vec_String file_n
         all.insert(all.end(), one.begin(), one.end());
     }
     return all;
+}
+
+// Partio2VTK (kiri_utils.h:54-250), from a bgeo file instead of a partio
+// object: a binary legacy VTK unstructured grid of the particles.
+inline void Partio2VTK(const std::string& vtk_path, const std::string& bgeo_path) {
+    sf_check(sf_bgeo_to_vtk(bgeo_path.c_str(), vtk_path.c_str()), "sf_bgeo_to_vtk");
 }
 
 // SeepageflowVolumeData / SeepageflowMultiVolumeData (cuda_volume_emitter.cuh:43-63)
@@ -529,6 +536,20 @@ class CudaSFSystem {
             std::printf("UpdateSystem failed: %s\n", sf_last_error());
         if (mEmitter) mEmitter->Pull(mHandle);
         return ms;
+    }
+    // One frame of main.cpp's Update() loop (main.cpp:235-259) run by the
+    // library (sf_update_frame): the same state as calling UpdateSystem from
+    // the loop, with the loop's bookkeeping on the device.  Returns the
+    // number of sub-steps.
+    size_t UpdateFrame(float renderInterval) {
+        size_t k = 0;
+        float ms = 0.f;
+        if (mEmitter) mEmitter->Push(mHandle);
+        if (sf_set_params(mHandle, &CUDA_SEEPAGEFLOW_PARAMS) != SF_OK ||
+            sf_update_frame(mHandle, renderInterval, bAdaptiveSubTimeStep ? 1 : 0, &k, &ms) != SF_OK)
+            std::printf("UpdateFrame failed: %s\n", sf_last_error());
+        if (mEmitter) mEmitter->Pull(mHandle);
+        return k;
     }
     size_t GetNumOfSubTimeSteps() const {
         size_t k = 1;

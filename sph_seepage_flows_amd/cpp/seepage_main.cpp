@@ -14,13 +14,15 @@
 // be compared bit for bit with the Python host.
 //
 //   seepage_main [--frames N] [--out FOLDER] [--render-interval S] [--fixed] [--export-initial] [--emit]
-//                [--resources DIR] [--shape FOLDER/NAME[:CD]]... [--flip-yz] [--no-emit]
+//                [--resources DIR] [--shape FOLDER/NAME[:CD]]... [--flip-yz] [--no-emit] [--device-frame]
 //
 // --emit switches on main.cpp's inflow emitter (main.cpp:85-95: a circle of
 // radius 0.17 at the box centre + (size/10, size/5, size/2.3), velocity
 // (0, 0, -5)) in the small scene, with room for 20,000 more particles.
 // --shape adds one sand object (CD: its drag coefficient, 0.5 by default as
 // main.cpp:137-138); --no-emit keeps the shape scene's emitter off.
+// --device-frame runs each frame with CudaSFSystem::UpdateFrame (the same
+// loop inside the library) instead of calling UpdateSystem per sub-step.
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -35,6 +37,8 @@
 using namespace KIRI;
 
 namespace {
+
+bool gAdaptive = true;  // adaptive_sub_timestep of main.cpp:198
 
 uint64_t splitmix64(uint64_t x) {
     x += 0x9E3779B97F4A7C15ull;
@@ -142,7 +146,7 @@ CudaSFSystemPtr SetupParams(bool emit) {
     CudaEmitterPtr emitter = std::make_shared<CudaEmitter>(E.emit_pos, E.emit_vel, E.enable);
     std::printf("Number of total particles = %zu, boundary particles = %zu\n", particles->Size(), boundary_particles->Size());
     return std::make_shared<CudaSFSystem>(particles, boundary_particles, solver, searcher, boundary_searcher, emitter,
-                                          true);
+                                          gAdaptive);
 }
 
 struct ShapeSpec {
@@ -252,7 +256,7 @@ CudaSFSystemPtr SetupShapeScene(const std::vector<ShapeSpec>& shapes, bool flip_
     auto boundary_searcher = std::make_shared<CudaGNBoundarySearcher>(lo, hi, boundary_particles->MaxSize(),
                                                                       B.kernel_radius);
     return std::make_shared<CudaSFSystem>(particles, boundary_particles, solver, searcher, boundary_searcher, emitter,
-                                          true);
+                                          gAdaptive);
 }
 
 bool parse_shape(const char* arg, ShapeSpec* out) {
@@ -274,7 +278,7 @@ bool parse_shape(const char* arg, ShapeSpec* out) {
 
 int main(int argc, char** argv) {
     int frames = 2;
-    bool adaptive = true, export_initial = false, emit = false, no_emit = false, flip_yz = false;
+    bool export_initial = false, emit = false, no_emit = false, flip_yz = false, device_frame = false;
     std::vector<ShapeSpec> shapes;
     float render_interval = 1.f / 30.f;
     std::string out = "export/bgeo/seepage_small_block";
@@ -282,11 +286,12 @@ int main(int argc, char** argv) {
         if (!std::strcmp(argv[i], "--frames") && i + 1 < argc) frames = std::atoi(argv[++i]);
         else if (!std::strcmp(argv[i], "--out") && i + 1 < argc) out = argv[++i];
         else if (!std::strcmp(argv[i], "--render-interval") && i + 1 < argc) render_interval = (float)std::atof(argv[++i]);
-        else if (!std::strcmp(argv[i], "--fixed")) adaptive = false;
+        else if (!std::strcmp(argv[i], "--fixed")) gAdaptive = false;
         else if (!std::strcmp(argv[i], "--export-initial")) export_initial = true;
         else if (!std::strcmp(argv[i], "--emit")) emit = true;
         else if (!std::strcmp(argv[i], "--no-emit")) no_emit = true;
         else if (!std::strcmp(argv[i], "--flip-yz")) flip_yz = true;
+        else if (!std::strcmp(argv[i], "--device-frame")) device_frame = true;
         else if (!std::strcmp(argv[i], "--resources") && i + 1 < argc) DbPbrPath() = argv[++i];
         else if (!std::strcmp(argv[i], "--shape") && i + 1 < argc) {
             ShapeSpec sp;
@@ -318,7 +323,9 @@ int main(int argc, char** argv) {
     for (int frame = 1; frame <= frames; ++frame) {  // Update() main.cpp:223-283
         auto t0 = std::chrono::steady_clock::now();
         size_t sub = 0;
-        if (adaptive) {
+        if (device_frame) {
+            sub = system->UpdateFrame(render_interval);
+        } else if (system->GetAdaptiveSubTimeStep()) {
             float remaining_time = render_interval;
             while (remaining_time > 1e-6f) {
                 system->UpdateSystem(remaining_time);

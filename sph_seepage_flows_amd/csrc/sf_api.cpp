@@ -226,7 +226,7 @@ int solve_with(sf_system* h, const Fields& f, hipEvent_t* ev) {
     return SF_OK;
 }
 
-int advect(sf_system* h, float dt, const CflOut* dt_dev, float render_interval, hipEvent_t* ev) {
+int advect(sf_system* h, float dt, const CflOut* dt_dev, hipEvent_t* ev) {
     hipStream_t s = h->stream;
     SF_MARK(ev, SF_PASS_ADVECT);
     AdvectArgs a;
@@ -239,7 +239,6 @@ int advect(sf_system* h, float dt, const CflOut* dt_dev, float render_interval, 
     a.bcell = h->bcell.p;
     a.dt = dt;
     a.dt_dev = dt_dev;
-    a.render_interval = render_interval;
     launch_advect(h->K, a, (int)h->n, s);
     h->cur ^= 1;
     SF_MARK(ev, SF_PASS_COUNT);
@@ -251,7 +250,19 @@ int one_step(sf_system* h, float dt, hipEvent_t* ev) {
     int rc;
     if ((rc = build_neighbors(h, ev))) return rc;
     if ((rc = solve(h, ev))) return rc;
-    return advect(h, dt, nullptr, 0.f, ev);
+    return advect(h, dt, nullptr, ev);
+}
+
+// CudaWCSphSFSolver::UpdateSolver after the neighbour build
+// (cuda_wcsph_sf_solver.cpp:34-179): the passes, the CFL sub-step count for
+// render_interval (or the frame's remaining time, see FrameDev) and the
+// advect with dt = render_interval / nsub, all without a host round trip.
+int wcsph_substep_tail(sf_system* h, float render_interval, FrameDev* frame) {
+    int rc;
+    if ((rc = solve(h))) return rc;
+    launch_cfl(h->K, h->acc.p, (int)h->n, h->cfl_part.p, h->cfl_out.p, render_interval, h->P.sph_mass,
+               h->O.speed_of_sound, h->O.time_step_limit_scale, h->P.dt, frame, h->stream);
+    return advect(h, 0.f, h->cfl_out.p);
 }
 
 int check_handle(sf_system* h) {
@@ -378,7 +389,9 @@ int sf_create(const sf_params* p, const sf_boundary_params* b, const sf_solver_o
     if ((rc = h->cell.alloc((size_t)h->C + 1)) || (rc = h->bcell.alloc((size_t)h->local_cells() + 1)) ||
         (rc = h->ct.alloc((size_t)h->C + 1)))
         return bail(rc);
-    if ((rc = h->cfl_part.alloc(cfl_partials(0))) || (rc = h->cfl_out.alloc(1))) return bail(rc);
+    if ((rc = h->cfl_part.alloc(cfl_partials(0))) || (rc = h->cfl_out.alloc(1)) ||
+        (rc = h->frame_dev.alloc(1)))
+        return bail(rc);
     if ((rc = ensure_sort_capacity(h, c))) return bail(rc);
     if ((rc = h->bpv.alloc(1))) return bail(rc);
     // zero everything like CudaArray::Clear (cuda_array.cuh:51-54)
@@ -626,10 +639,7 @@ int sf_update_system(sf_handle h, float render_interval, float* ms_out) {
     SF_HIP(hipEventRecord(h->ev0, h->stream));
     if ((rc = build_neighbors(h))) return rc;
     if (h->P.solver_type == SF_WCSPH_SOLVER) {
-        if ((rc = solve(h))) return rc;
-        launch_cfl(h->K, h->acc.p, (int)h->n, h->cfl_part.p, h->cfl_out.p, render_interval, h->P.sph_mass,
-                   h->O.speed_of_sound, h->O.time_step_limit_scale, h->P.dt, h->stream);
-        if ((rc = advect(h, 0.f, h->cfl_out.p, render_interval))) return rc;
+        if ((rc = wcsph_substep_tail(h, render_interval, nullptr))) return rc;
         CflOut o;
         SF_HIP(hipMemcpyAsync(&o, h->cfl_out.p, sizeof(CflOut), hipMemcpyDeviceToHost, h->stream));
         SF_HIP(hipEventRecord(h->ev1, h->stream));
@@ -640,12 +650,149 @@ int sf_update_system(sf_handle h, float render_interval, float* ms_out) {
         const float q = render_interval / h->P.dt;
         h->nsub = q >= 1.f ? (size_t)q : 1;
         if ((rc = solve(h))) return rc;
-        if ((rc = advect(h, h->P.dt, nullptr, 0.f))) return rc;
+        if ((rc = advect(h, h->P.dt, nullptr))) return rc;
         SF_HIP(hipEventRecord(h->ev1, h->stream));
         SF_HIP(hipEventSynchronize(h->ev1));
     }
     // CudaSFSystem::OnUpdateSolver emits after the solver step (cuda_sf_system.cu:86-115)
     if ((rc = emit_after_update(h, render_interval))) return rc;
+    if (ms_out) {
+        float ms = 0.f;
+        SF_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        *ms_out = ms;
+    }
+    return SF_OK;
+}
+
+}  // extern "C"
+
+namespace sf {
+
+// Largest chunk of sub-steps enqueued between two host reads.
+constexpr size_t kFrameChunk = 4096;
+
+// k_cfl_final's sub-step count for remaining time r and CFL dt d.
+static uint32_t cfl_count(float r, float d) {
+    const float q = r / d;
+    return q >= 4294967295.f ? 0xffffffffu : (q >= 1.f ? (uint32_t)q : 1u);
+}
+
+// Sub-steps the adaptive loop certainly still runs from remaining time r.
+// The CFL dt never exceeds dt_max, so every count is at least
+// cfl_count(r, dt_max), and r - r / k grows with r and with k: the loop run
+// with dt_max ends no later than the real one.  This replays it in the same
+// float arithmetic.
+static size_t certain_sub_steps(float r, float dt_max, size_t cap) {
+#pragma clang fp contract(off)
+    size_t k = 0;
+    while (r > 1e-6f && k < cap) {
+        r = r - r / (float)cfl_count(r, dt_max);
+        ++k;
+    }
+    return k;
+}
+
+// One WCSPH frame (main.cpp:235-259): the host enqueues as many sub-steps as
+// the loop certainly runs (or, with a running emitter, as many as certainly
+// pass without an emission), then reads the device loop state once.
+static int frame_wcsph(sf_system* h, float ri, bool adaptive, size_t* steps) {
+#pragma clang fp contract(off)
+    int rc;
+    // k_cfl_final: dt = min(params dt, scale * min(dtf, 0.4 h / c)) <= dt_max
+    const float dts = 0.4f * h->K.h / h->O.speed_of_sound;
+    const float lim = h->O.time_step_limit_scale * dts;
+    const float dt_max = h->P.dt < lim ? h->P.dt : lim;
+    // the fixed loop runs the count the previous call left (main.cpp:251-257)
+    const size_t todo = h->nsub;
+    bool clock = emitter_wcsph_clock(h);
+    FrameDev fd{ri, h->emitter.elapsed, 0u, (adaptive ? kFrameAdaptive : 0u) | (clock ? kFrameElapsed : 0u)};
+    SF_HIP(hipMemcpyAsync(h->frame_dev.p, &fd, sizeof(fd), hipMemcpyHostToDevice, h->stream));
+    float r = ri;
+    size_t done = 0;
+    for (;;) {
+        size_t len;
+        if (adaptive) {
+            if (!(r > 1e-6f)) break;
+            len = certain_sub_steps(r, dt_max, kFrameChunk);
+        } else {
+            if (done >= todo) break;
+            len = std::min(todo - done, kFrameChunk);
+        }
+        if (clock) {
+            // a sub-step advances the clock by r / k <= 2 dt_max (k = floor(r / dt) >= 1),
+            // or, with a fixed interval, by at most ri / cfl_count(ri, dt_max)
+            const double step_max = adaptive ? 2.0 * (double)dt_max : (double)(ri / (float)cfl_count(ri, dt_max));
+            len = std::min(len, emitter_quiet_steps(h, step_max * (1.0 + 1.0 / 4194304.0), len) + 1);
+        }
+        for (size_t k = 0; k < len; ++k) {
+            if ((rc = build_neighbors(h))) return rc;
+            if ((rc = wcsph_substep_tail(h, ri, h->frame_dev.p))) return rc;
+        }
+        CflOut o;
+        SF_HIP(hipMemcpyAsync(&fd, h->frame_dev.p, sizeof(fd), hipMemcpyDeviceToHost, h->stream));
+        SF_HIP(hipMemcpyAsync(&o, h->cfl_out.p, sizeof(o), hipMemcpyDeviceToHost, h->stream));
+        SF_HIP(hipStreamSynchronize(h->stream));
+        done += len;
+        h->nsub = o.nsub;
+        r = fd.remaining;
+        if (clock) {
+            h->emitter.elapsed = fd.elapsed;
+            if ((rc = emitter_wcsph_check(h))) return rc;
+            clock = emitter_wcsph_clock(h);  // off once the emitter met the capacity
+        }
+    }
+    *steps = done;
+    return SF_OK;
+}
+
+// One SPH-solver frame: the sub-step count is ri / params.dt on the host
+// (CudaSphSFSolver::UpdateSolver, cuda_sph_sf_solver.cpp:19-30), so nothing
+// is read back.
+static int frame_sph(sf_system* h, float ri, bool adaptive, size_t* steps) {
+#pragma clang fp contract(off)
+    int rc;
+    const size_t todo = h->nsub;
+    float r = ri;
+    size_t done = 0;
+    for (;;) {
+        float cur;
+        if (adaptive) {
+            if (!(r > 1e-6f)) break;
+            cur = r;
+        } else {
+            if (done >= todo) break;
+            cur = ri;
+        }
+        const float q = cur / h->P.dt;
+        h->nsub = q >= 1.f ? (size_t)q : 1;
+        if ((rc = build_neighbors(h)) || (rc = solve(h)) || (rc = advect(h, h->P.dt, nullptr))) return rc;
+        if ((rc = emit_after_update(h, cur))) return rc;
+        if (adaptive) r = r - r / static_cast<float>(h->nsub);
+        ++done;
+    }
+    *steps = done;
+    return SF_OK;
+}
+
+}  // namespace sf
+
+extern "C" {
+
+int sf_update_frame(sf_handle h, float render_interval, int adaptive, size_t* sub_steps, float* ms_out) {
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!h->uploaded) return set_error(SF_E_STATE, "no particles uploaded");
+    if (!std::isfinite(render_interval) || !(render_interval > 0.f))
+        return set_error(SF_E_INVALID, "render interval must be finite and > 0");
+    if (h->nscenes) return set_error(SF_E_STATE, "batched handles step with a fixed dt (sf_step, sf_step_graph)");
+    SF_HIP(hipEventRecord(h->ev0, h->stream));
+    size_t steps = 0;
+    rc = h->P.solver_type == SF_WCSPH_SOLVER ? frame_wcsph(h, render_interval, adaptive != 0, &steps)
+                                             : frame_sph(h, render_interval, adaptive != 0, &steps);
+    if (rc) return rc;
+    SF_HIP(hipEventRecord(h->ev1, h->stream));
+    SF_HIP(hipEventSynchronize(h->ev1));
+    if (sub_steps) *sub_steps = steps;
     if (ms_out) {
         float ms = 0.f;
         SF_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));

@@ -91,7 +91,7 @@ int diff_step(sf_system* h, float dt, StepTapeBufs* tape) {
     std::swap(d->id.n, d->id_tmp.n);
     if (!tape) {
         if ((rc = solve(h))) return rc;
-        return advect(h, dt, nullptr, 0.f);
+        return advect(h, dt, nullptr);
     }
     StepTapeBufs& T = *tape;
     if ((rc = copy(T.pm.p, h->pm[h->cur].p, n * sizeof(float4), s)) ||
@@ -112,7 +112,7 @@ int diff_step(sf_system* h, float dt, StepTapeBufs* tape) {
     // the handle's own transient fields keep the last step's values for downloads
     if ((rc = copy(h->rp.p, T.rp.p, n * sizeof(float2), s)) || (rc = copy(h->es.p, T.es.p, n * sizeof(float2), s)))
         return rc;
-    return advect(h, dt, nullptr, 0.f);
+    return advect(h, dt, nullptr);
 }
 
 int check_diff_handle(sf_system* h) {

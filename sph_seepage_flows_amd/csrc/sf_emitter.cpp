@@ -97,6 +97,47 @@ int emit(sf_system* h) {
 
 }  // namespace
 
+bool emitter_wcsph_clock(const sf_system* h) {
+    const EmitterState& e = h->emitter;
+    return e.enable && e.run && !e.samples.empty() && h->P.solver_type == SF_WCSPH_SOLVER;
+}
+
+// The WCSPH schedule's test, on the clock as it stands (cuda_sf_system.cu:95-104).
+int emitter_wcsph_check(sf_system* h) {
+#pragma clang fp contract(off)
+    EmitterState& e = h->emitter;
+    const float speed = std::sqrt(dot3(f3(e.vel), f3(e.vel)));
+    const float two_r = 2.f * h->P.sph_particle_radius;
+    if ((speed * e.elapsed) / two_r >= static_cast<float>(e.counter)) {
+        const int rc = emit(h);
+        if (rc) return rc;
+        ++e.counter;
+    }
+    return SF_OK;
+}
+
+// Sub-steps after which the WCSPH test is certainly still false when each
+// advances the clock by at most step_max: the float test
+// fl(fl(speed * t) / 2r) >= counter is below counter while
+// speed * t / 2r * (1 + u)^2 < counter, and t grows by at most
+// (t + step_max)(1 + u) per sub-step (u = 2^-24, the float rounding bound).
+size_t emitter_quiet_steps(const sf_system* h, double step_max, size_t cap) {
+    const EmitterState& e = h->emitter;
+    const double speed = (double)std::sqrt(dot3(f3(e.vel), f3(e.vel)));
+    const double two_r = (double)(2.f * h->P.sph_particle_radius);
+    const double u = 1.0 / 16777216.0;
+    const double counter = (double)e.counter;
+    if (speed * (double)e.elapsed / two_r * (1 + u) * (1 + u) >= counter) return 0;
+    double t = (double)e.elapsed;
+    size_t j = 0;
+    while (j < cap) {
+        t = (t + step_max) * (1 + u);
+        if (speed * t / two_r * (1 + u) * (1 + u) >= counter) break;
+        ++j;
+    }
+    return j;
+}
+
 int emit_after_update(sf_system* h, float render_interval) {
 #pragma clang fp contract(off)
     EmitterState& e = h->emitter;
@@ -105,11 +146,7 @@ int emit_after_update(sf_system* h, float render_interval) {
     const float two_r = 2.f * h->P.sph_particle_radius;
     if (h->P.solver_type == SF_WCSPH_SOLVER) {
         e.elapsed += render_interval / static_cast<float>(h->nsub);
-        if ((speed * e.elapsed) / two_r >= static_cast<float>(e.counter)) {
-            const int rc = emit(h);
-            if (rc) return rc;
-            ++e.counter;
-        }
+        return emitter_wcsph_check(h);
     } else {
         // size_t(2 r / (|v| dt)); a period below one step emits every call
         // (the reference would divide by zero)

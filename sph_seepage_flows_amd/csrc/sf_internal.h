@@ -175,11 +175,25 @@ struct CflOut {
     float dt;
     uint32_t nsub;
     float amax;
-    uint32_t pad;
+    float ri;  // the render interval this sub-step used (dt of the advect = ri / nsub)
 };
+// Device-side state of a frame loop (main.cpp:235-247 run without a host
+// round trip per sub-step).  With kFrameAdaptive the finalize takes the
+// render interval from `remaining` and then applies the loop's update
+// remaining -= remaining / nsub; with kFrameElapsed it also advances the
+// emitter clock elapsed += ri / nsub.  `count` counts the sub-steps.
+struct FrameDev {
+    float remaining;
+    float elapsed;
+    uint32_t count;
+    uint32_t flags;
+};
+constexpr uint32_t kFrameAdaptive = 1u;
+constexpr uint32_t kFrameElapsed = 2u;
 int cfl_partials(int n);
+// frame (may be null): see FrameDev; when null the render interval is the argument
 void launch_cfl(const SFConst& k, const float4* acc, int n, float* partials, CflOut* out, float render_interval,
-                float sph_mass, float speed_of_sound, float scale, float params_dt, hipStream_t s);
+                float sph_mass, float speed_of_sound, float scale, float params_dt, FrameDev* frame, hipStream_t s);
 struct AdvectArgs {
     const float4* pm_in;
     const float4* vr_in;
@@ -189,8 +203,7 @@ struct AdvectArgs {
     const float4* bpv;
     const uint32_t* bcell;
     float dt;
-    const CflOut* dt_dev;  // when non-null, dt = render_interval / nsub from the device
-    float render_interval;
+    const CflOut* dt_dev;  // when non-null, dt = ri / nsub from the device
 };
 void launch_advect(const SFConst& k, const AdvectArgs& a, int n, hipStream_t s);
 

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdint>
 #include <cstring>
 #include <limits>
@@ -85,13 +86,22 @@ float word_as_float(uint32_t w) {
     return f;
 }
 
+// One point attribute: Houdini type (0 float, 1 int, 4 indexed string,
+// 5 vector), component count and first word within a point record.
+struct PointAttr {
+    std::string name;
+    int32_t type;
+    uint16_t size;
+    size_t word;
+};
+
 // Partio's readBGEO (extlib/include/partio/io/BGEO.cpp:196-309) with its
 // point-attribute table (getAttributes, :85-150): each point is the
 // homogeneous position (4 words) followed by every point attribute's words.
-// Returns the word offset of the first "pscale" attribute (or -1) and the
-// point record size in words; `points` is the offset of the first record.
+// Returns the point record size in words and the attribute table; `points`
+// is the byte offset of the first record.
 int parse_header(const std::vector<unsigned char>& file, const char* path, size_t* n_points, size_t* record_words,
-                 long* pscale_word, size_t* points) {
+                 std::vector<PointAttr>* attrs, size_t* points) {
     BeIn in{file.data(), file.size()};
     const std::string magic = in.str(4);
     if (!in.ok) return set_error(SF_E_IO, std::string(path) + ": not a bgeo file (too short)");
@@ -115,7 +125,7 @@ int parse_header(const std::vector<unsigned char>& file, const char* path, size_
     if (version != 5) return set_error(SF_E_IO, std::string(path) + ": bgeo version must be 5");
     if (np < 0 || n_point_attr < 0) return set_error(SF_E_IO, std::string(path) + ": negative bgeo counts");
     size_t words = 4;
-    long pscale = -1;
+    attrs->clear();
     for (int32_t a = 0; a < n_point_attr; ++a) {
         const std::string name = in.str(in.u16());
         const uint16_t size = in.u16();
@@ -134,17 +144,28 @@ int parse_header(const std::vector<unsigned char>& file, const char* path, size_
                                           std::to_string(type));
         }
         if (!in.ok) return set_error(SF_E_IO, std::string(path) + ": truncated bgeo attribute table");
-        // the first definition of a name is the one a lookup by name finds
-        if (name == "pscale" && pscale < 0 && size >= 1) pscale = (long)words;
+        attrs->push_back(PointAttr{name, type, size, words});
         words += size;
     }
     if ((file.size() - in.at) / 4 / words < (size_t)np)
         return set_error(SF_E_IO, std::string(path) + ": truncated bgeo point data");
     *n_points = (size_t)np;
     *record_words = words;
-    *pscale_word = pscale;
     *points = in.at;
     return SF_OK;
+}
+
+// first attribute of that name: the one a lookup by name finds
+const PointAttr* find_attr(const std::vector<PointAttr>& attrs, const char* name) {
+    for (const PointAttr& a : attrs)
+        if (a.name == name && a.size >= 1) return &a;
+    return nullptr;
+}
+
+int read_bgeo_file(const char* path, std::vector<unsigned char>& file, size_t* np, size_t* words,
+                   std::vector<PointAttr>* attrs, size_t* at) {
+    if (int rc = slurp(path, file)) return rc;
+    return parse_header(file, path, np, words, attrs, at);
 }
 
 uint32_t word_at(const unsigned char* p) {
@@ -213,10 +234,11 @@ extern "C" {
 int sf_read_bgeo(const char* path, int flip_yz, float* shape4, size_t capacity, size_t* n_out) {
     if (!path || !n_out) return set_error(SF_E_INVALID, "null path or n_out");
     std::vector<unsigned char> file;
-    if (int rc = slurp(path, file)) return rc;
+    std::vector<PointAttr> attrs;
     size_t np = 0, words = 0, at = 0;
-    long pscale = -1;
-    if (int rc = parse_header(file, path, &np, &words, &pscale, &at)) return rc;
+    if (int rc = read_bgeo_file(path, file, &np, &words, &attrs, &at)) return rc;
+    const PointAttr* ps_attr = find_attr(attrs, "pscale");
+    const long pscale = ps_attr ? (long)ps_attr->word : -1;
     *n_out = np;
     if (!shape4) return SF_OK;
     if (capacity < np) return set_error(SF_E_CAPACITY, "shape capacity too small for the file's points");
@@ -231,6 +253,73 @@ int sf_read_bgeo(const char* path, int flip_yz, float* shape4, size_t capacity, 
         o[2] = flip_yz ? y : z;
         o[3] = ps;
     }
+    return SF_OK;
+}
+
+// Partio2VTK (kiri_utils.h:54-250): legacy VTK 4.1, binary (big-endian), an
+// unstructured grid with one vertex cell per particle (cell point = the "id"
+// attribute, else the index), then every other attribute as FIELD data:
+// float and vector as float, int as int.
+int sf_bgeo_to_vtk(const char* bgeo_path, const char* vtk_path) {
+    if (!bgeo_path || !vtk_path) return set_error(SF_E_INVALID, "null path");
+    std::vector<unsigned char> file;
+    std::vector<PointAttr> attrs;
+    size_t np = 0, words = 0, at = 0;
+    if (int rc = read_bgeo_file(bgeo_path, file, &np, &words, &attrs, &at)) return rc;
+    if (np == 0) return SF_OK;  // nothing is written for an empty file
+    if (np > 0x7fffffffu) return set_error(SF_E_INVALID, "too many particles for a VTK file");
+    const unsigned char* rec0 = file.data() + at;
+    const size_t stride = 4 * words;
+    std::string out;
+    out.reserve(np * (12 + 8 + 4) + 256);
+    auto text = [&](const std::string& t) { out += t; };
+    auto word = [&](uint32_t w) {  // big-endian, as the file stores it
+        out.push_back((char)(w >> 24));
+        out.push_back((char)(w >> 16));
+        out.push_back((char)(w >> 8));
+        out.push_back((char)w);
+    };
+    const std::string n = std::to_string(np);
+    text("# vtk DataFile Version 4.1\nSeepage Flow Framework Particles Data\nBINARY\nDATASET UNSTRUCTURED_GRID\n");
+    text("POINTS " + n + " float\n");
+    for (size_t i = 0; i < np; ++i)
+        for (int c = 0; c < 3; ++c) word(word_at(rec0 + stride * i + 4 * c));
+    text("\n");
+    const PointAttr* id = find_attr(attrs, "id");
+    text("CELLS " + n + " " + std::to_string(2 * np) + "\n");
+    for (size_t i = 0; i < np; ++i) {
+        word(1u);
+        word(id ? word_at(rec0 + stride * i + 4 * id->word) : (uint32_t)i);
+    }
+    text("\n");
+    text("CELL_TYPES " + n + "\n");
+    for (size_t i = 0; i < np; ++i) word(1u);  // VTK_VERTEX
+    text("\n");
+    std::vector<const PointAttr*> fields;
+    for (const PointAttr& a : attrs)
+        if (&a != id && (a.type == 0 || a.type == 1 || a.type == 5)) fields.push_back(&a);
+    text("POINT_DATA " + n + "\n");
+    text("FIELD FieldData " + std::to_string(fields.size()) + "\n");
+    for (const PointAttr* a : fields) {
+        std::string name;
+        bool space = false;
+        for (char ch : a->name) {  // runs of whitespace become one '_'
+            const bool ws = ch == ' ' || ch == '\t' || ch == '\n' || ch == '\r' || ch == '\f' || ch == '\v';
+            if (ws && !space) name += '_';
+            if (!ws) name += ch;
+            space = ws;
+        }
+        const int count = a->type == 5 ? 3 : (int)a->size;
+        text(name + " " + std::to_string(a->size) + " " + n + (a->type == 1 ? " int\n" : " float\n"));
+        for (size_t i = 0; i < np; ++i)
+            for (int c = 0; c < count && (size_t)c < a->size; ++c) word(word_at(rec0 + stride * i + 4 * (a->word + c)));
+        text("\n");
+    }
+    FILE* f = std::fopen(vtk_path, "wb");
+    if (!f) return set_error(SF_E_IO, std::string("cannot open ") + vtk_path + " for writing");
+    const size_t wrote = std::fwrite(out.data(), 1, out.size(), f);
+    const int closed = std::fclose(f);
+    if (wrote != out.size() || closed != 0) return set_error(SF_E_IO, std::string("write error on ") + vtk_path);
     return SF_OK;
 }
 

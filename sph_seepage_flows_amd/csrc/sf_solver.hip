@@ -732,7 +732,8 @@ __global__ __launch_bounds__(kBlock) void k_cfl_partial(const float4* __restrict
 }
 __global__ __launch_bounds__(kBlock) void k_cfl_final(const float* __restrict__ part, int np, CflOut* out, float h,
                                                       float render_interval, float m, float c, float scale,
-                                                      float params_dt) {
+                                                      float params_dt, FrameDev* frame) {
+#pragma clang fp contract(off)
     __shared__ float sm[kBlock / 64];
     float r = 0.f;
     for (int q = threadIdx.x; q < np; q += kBlock) r = fmaxf(r, part[q]);
@@ -749,24 +750,36 @@ __global__ __launch_bounds__(kBlock) void k_cfl_final(const float* __restrict__ 
         const float inner = dtf < dts ? dtf : dts;
         const float lim = scale * inner;
         const float dt = params_dt < lim ? params_dt : lim;
-        const float q = render_interval / dt;
+        FrameDev fr;
+        if (frame) fr = *frame;
+        const float ri = (frame && (fr.flags & kFrameAdaptive)) ? fr.remaining : render_interval;
+        const float q = ri / dt;
         uint32_t k = q >= 4294967295.f ? 0xffffffffu : (q >= 1.f ? (uint32_t)q : 1u);
         CflOut o;
         o.dt = dt;
         o.nsub = k;
         o.amax = amax;
-        o.pad = 0;
+        o.ri = ri;
         *out = o;
+        if (frame) {
+            // the host loop's updates, in its float arithmetic (main.cpp:246,
+            // cuda_sf_system.cu:95)
+            const float step = ri / (float)k;
+            if (fr.flags & kFrameElapsed) fr.elapsed = fr.elapsed + step;
+            if (fr.flags & kFrameAdaptive) fr.remaining = ri - step;
+            fr.count += 1;
+            *frame = fr;
+        }
     }
 }
 void launch_cfl(const SFConst& k, const float4* acc, int n, float* partials, CflOut* out, float render_interval,
-                float sph_mass, float speed_of_sound, float scale, float params_dt, hipStream_t s) {
+                float sph_mass, float speed_of_sound, float scale, float params_dt, FrameDev* frame, hipStream_t s) {
     int nb = (n + kBlock - 1) / kBlock;
     if (nb > kCflBlocks) nb = kCflBlocks;
     if (nb < 1) nb = 1;
     k_cfl_partial<<<nb, kBlock, 0, s>>>(acc, n, partials);
     k_cfl_final<<<1, kBlock, 0, s>>>(partials, nb, out, k.h, render_interval, sph_mass, speed_of_sound, scale,
-                                      params_dt);
+                                      params_dt, frame);
 }
 
 // ---------------------------------------------------------------------------
@@ -776,7 +789,7 @@ SF_DEV float sgnf(float v) { return (float)((0.f < v) - (v < 0.f)); }
 __global__ __launch_bounds__(kBlock) void k_advect(SFConst k, AdvectArgs a, int n) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const float dt = a.dt_dev ? a.render_interval / (float)a.dt_dev->nsub : a.dt;
+    const float dt = a.dt_dev ? a.dt_dev->ri / (float)a.dt_dev->nsub : a.dt;
     const float4 p = a.pm_in[i];
     const float4 v = a.vr_in[i];
     float4 ac = a.acc[i];

@@ -59,6 +59,7 @@ struct sf_system {
     // CFL
     DevBuf<float> cfl_part;
     DevBuf<CflOut> cfl_out;
+    DevBuf<FrameDev> frame_dev;  // sf_update_frame's loop state
     size_t nsub = 1;
     // batched scenes (nscenes == 0: one scene)
     size_t nscenes = 0;
@@ -124,12 +125,20 @@ int ensure_cell_capacity(sf_system* h);
 int build_neighbors(sf_system* h, hipEvent_t* ev = nullptr);
 int solve(sf_system* h, hipEvent_t* ev = nullptr);
 int solve_with(sf_system* h, const Fields& f, hipEvent_t* ev = nullptr);
-int advect(sf_system* h, float dt, const CflOut* dt_dev, float render_interval, hipEvent_t* ev = nullptr);
+int advect(sf_system* h, float dt, const CflOut* dt_dev, hipEvent_t* ev = nullptr);
+int wcsph_substep_tail(sf_system* h, float render_interval, FrameDev* frame);
 int one_step(sf_system* h, float dt, hipEvent_t* ev = nullptr);
 int check_handle(sf_system* h);
 // Inflow emitter after a sf_update_system step (sf_emitter.cpp);
 // render_interval as passed to that call.
 int emit_after_update(sf_system* h, float render_interval);
+// the WCSPH emission clock runs (emitter on, running, built, WCSPH solver)
+bool emitter_wcsph_clock(const sf_system* h);
+// the WCSPH emission test on the clock as it stands (emits when due)
+int emitter_wcsph_check(sf_system* h);
+// sub-steps, each advancing the clock by at most step_max, after which the
+// WCSPH emission test is certainly still false (at most cap)
+size_t emitter_quiet_steps(const sf_system* h, double step_max, size_t cap);
 int set_device(sf_system* h);
 int fetch4(sf_system* h, const float4* src, size_t n, std::vector<float4>& out);
 }  // namespace sf

@@ -7,7 +7,8 @@ what SeepageflowVolumeData / SeepageflowMultiVolumeData hold
 (cuda_volume_emitter.cuh:43-63), and its two builders are
 CudaVolumeEmitter::BuildSeepageflowShapeVolume / ...ShapeMultiVolume
 (cuda_volume_emitter.cpp:109-175) through `sf_build_seepage_shape_volume` /
-`sf_build_seepage_shape_multi_volume`.
+`sf_build_seepage_shape_multi_volume`.  `bgeo_to_vtk` is Partio2VTK
+(kiri_utils.h:54-250) through `sf_bgeo_to_vtk`.
 """
 from __future__ import annotations
 
@@ -34,6 +35,12 @@ def read_multi_bgeo(paths, flip_yz: bool = False) -> np.ndarray:
     """The points of several files, concatenated in the given order."""
     parts = [read_bgeo(p, flip_yz) for p in paths]
     return np.concatenate(parts) if parts else np.zeros((0, 4), np.float32)
+
+
+def bgeo_to_vtk(bgeo_path: str | os.PathLike, vtk_path: str | os.PathLike) -> None:
+    """Partio2VTK (kiri_utils.h:54-250): a binary legacy VTK unstructured grid
+    of the file's particles and point attributes (sf_bgeo_to_vtk)."""
+    check(lib().sf_bgeo_to_vtk(os.fsencode(bgeo_path), os.fsencode(vtk_path)))
 
 
 def pbr_path(root: str | os.PathLike, folder: str, name: str) -> str:

@@ -85,6 +85,13 @@ class SFSystem:
         check(lib().sf_update_system(self._h, float(render_interval), C.byref(ms)))
         return ms.value
 
+    def update_frame(self, render_interval: float, adaptive: bool = True) -> tuple[int, float]:
+        """One frame of the reference driver's sub-step loop in the library
+        (sf_update_frame): (sub-steps run, GPU ms)."""
+        k, ms = C.c_size_t(0), C.c_float(0.0)
+        check(lib().sf_update_frame(self._h, float(render_interval), int(adaptive), C.byref(k), C.byref(ms)))
+        return k.value, ms.value
+
     def GetNumOfSubTimeSteps(self) -> int:
         k = C.c_size_t(0)
         check(lib().sf_num_sub_steps(self._h, C.byref(k)))

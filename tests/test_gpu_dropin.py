@@ -135,3 +135,18 @@ def test_cpp_driver_shape_scene_matches_python_host(tmp_path):
     np.testing.assert_array_equal(p1, g.get("pos"))
     np.testing.assert_array_equal(c1, g.get("col"))
     np.testing.assert_array_equal(l1.astype(np.uint64), g.get("label"))
+
+
+def test_cpp_driver_device_frame_loop_is_identical(tmp_path):
+    """--device-frame (CudaSFSystem::UpdateFrame, sf_update_frame) writes the
+    same frames, byte for byte, as the UpdateSystem loop, emitter on."""
+    outs = []
+    for extra in ([], ["--device-frame"]):
+        out = tmp_path / ("dev" if extra else "host")
+        r = subprocess.run([str(DRIVER), "--frames", "2", "--out", str(out), "--emit", *extra], capture_output=True,
+                           text=True, timeout=300)
+        print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+        assert r.returncode == 0
+        outs.append(out)
+    for f in ("0001.bgeo", "0002.bgeo"):
+        assert (outs[0] / f).read_bytes() == (outs[1] / f).read_bytes(), f

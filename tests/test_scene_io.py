@@ -242,3 +242,92 @@ def test_shape_scene_follows_the_reference_driver(tmp_path):
     e = sc.meta["emitter"]
     np.testing.assert_allclose(e["pos"], (0.75 + 0.15, 0.75 + 0.3, 1.25 + 2.5 / 2.3), rtol=1e-6)
     assert sc.n == 300 and sc.nb > 0 and (sc.label == 1).all()
+
+
+def parse_vtk(data: bytes):
+    """Independent reader of the binary legacy VTK layout (big-endian)."""
+    out, at = {}, 0
+
+    def line():
+        nonlocal at
+        e = data.index(b"\n", at)
+        s = data[at:e].decode()
+        at = e + 1
+        return s
+
+    assert line() == "# vtk DataFile Version 4.1"
+    out["title"] = line()
+    assert line() == "BINARY" and line() == "DATASET UNSTRUCTURED_GRID"
+    h = line().split()
+    assert h[0] == "POINTS" and h[2] == "float"
+    n = int(h[1])
+    out["points"] = np.frombuffer(data[at:at + 12 * n], ">f4").reshape(n, 3)
+    at += 12 * n + 1
+    h = line().split()
+    assert h[0] == "CELLS" and int(h[1]) == n and int(h[2]) == 2 * n
+    out["cells"] = np.frombuffer(data[at:at + 8 * n], ">i4").reshape(n, 2)
+    at += 8 * n + 1
+    assert line() == f"CELL_TYPES {n}"
+    out["types"] = np.frombuffer(data[at:at + 4 * n], ">i4")
+    at += 4 * n + 1
+    assert line() == f"POINT_DATA {n}"
+    h = line().split()
+    assert h[:2] == ["FIELD", "FieldData"]
+    fields = {}
+    for _ in range(int(h[2])):
+        name, comps, cnt, typ = line().split()
+        assert int(cnt) == n
+        k = int(comps) * n
+        fields[name] = np.frombuffer(data[at:at + 4 * k], ">f4" if typ == "float" else ">i4").reshape(n, int(comps))
+        at += 4 * k + 1
+    assert at == len(data)
+    out["fields"] = fields
+    return out
+
+
+def test_bgeo_to_vtk_export_layout(tmp_path):
+    """A frame as the exporter writes it (Cd, pscale, label): points, vertex
+    cells over the particle index, and the three attributes with all their
+    components."""
+    n = 17
+    pos = _points(n, 8)
+    rng = np.random.default_rng(9)
+    col = rng.random((n, 3)).astype(np.float32)
+    rad = (rng.random(n) * 0.01).astype(np.float32)
+    lab = rng.integers(0, 2, n).astype(np.uint64)
+    src = tmp_path / "0001.bgeo"
+    write_bgeo(str(src), pos, col, rad, lab)
+    dst = tmp_path / "0001.vtk"
+    scene_io.bgeo_to_vtk(src, dst)
+    v = parse_vtk(dst.read_bytes())
+    assert v["title"] == "Seepage Flow Framework Particles Data"
+    np.testing.assert_array_equal(v["points"], pos)
+    np.testing.assert_array_equal(v["cells"], np.stack([np.ones(n), np.arange(n)], 1))
+    assert (v["types"] == 1).all()
+    assert list(v["fields"]) == ["Cd", "pscale", "label"]
+    np.testing.assert_array_equal(v["fields"]["Cd"], col)
+    np.testing.assert_array_equal(v["fields"]["pscale"][:, 0], rad)
+    np.testing.assert_array_equal(v["fields"]["label"][:, 0], lab.astype(np.int32))
+
+
+def test_bgeo_to_vtk_ids_vectors_and_skipped_kinds(tmp_path):
+    n = 6
+    pos = _points(n, 10)
+    ids = (np.arange(n) * 3 + 100)[:, None]
+    vel = np.arange(3 * n, dtype=np.float32).reshape(n, 3)
+    attrs = [("id", 1, 1, ids, None), ("v", 5, 3, vel, None), ("name", 4, 1, np.zeros((n, 1), int), ["a"]),
+             ("my attr", 0, 1, np.full((n, 1), 2.5, np.float32), None)]
+    src = tmp_path / "s.bgeo"
+    src.write_bytes(bgeo_bytes(pos, attrs))
+    dst = tmp_path / "s.vtk"
+    scene_io.bgeo_to_vtk(src, dst)
+    v = parse_vtk(dst.read_bytes())
+    np.testing.assert_array_equal(v["cells"][:, 1], ids[:, 0])  # cells point at the "id" values
+    assert list(v["fields"]) == ["v", "my_attr"]  # no id, no indexed string; whitespace -> '_'
+    np.testing.assert_array_equal(v["fields"]["v"], vel)
+    assert (v["fields"]["my_attr"] == np.float32(2.5)).all()
+    # an empty file writes nothing
+    e = tmp_path / "e.bgeo"
+    e.write_bytes(bgeo_bytes(np.zeros((0, 3), np.float32), []))
+    scene_io.bgeo_to_vtk(e, tmp_path / "e.vtk")
+    assert not (tmp_path / "e.vtk").exists()
