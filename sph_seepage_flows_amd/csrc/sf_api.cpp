@@ -1091,9 +1091,15 @@ int sf_step_graph(sf_handle h, float dt, int n_steps) {
     if (!h->uploaded) return set_error(SF_E_STATE, "no particles uploaded");
     if (!std::isfinite(dt) || !(dt > 0.f)) return set_error(SF_E_INVALID, "dt must be finite and > 0");
     if (n_steps < 0) return set_error(SF_E_INVALID, "n_steps must be >= 0");
-    if (n_steps >= 2 && (h->graph_dirty || !h->gexec || h->graph_dt != dt || h->graph_parity != h->parity())) {
+    while (n_steps >= 2 && !h->keys_cycle()) {  // at most two plain steps first: see keys_cycle
+        if ((rc = one_step(h, dt))) return rc;
+        --n_steps;
+    }
+    if (n_steps >= 2 && (h->graph_dirty || !h->gexec || h->graph_dt != dt || h->graph_parity != h->parity() ||
+                         h->graph_keys != h->sorted_keys)) {
         h->drop_graph();
         const int parity = h->parity();
+        const uint32_t* keys = h->sorted_keys;
         SF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
         rc = one_step(h, dt);
         if (!rc) rc = one_step(h, dt);
@@ -1107,8 +1113,10 @@ int sf_step_graph(sf_handle h, float dt, int n_steps) {
         SF_HIP(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
         h->graph_dt = dt;
         h->graph_parity = parity;
+        h->graph_keys = keys;
         h->graph_dirty = false;
-        if (h->parity() != parity) return set_error(SF_E_STATE, "two steps did not restore the buffer parity");
+        if (h->parity() != parity || h->sorted_keys != keys)
+            return set_error(SF_E_STATE, "two steps did not restore the buffer parity");
     }
     for (int q = 0; q + 1 < n_steps; q += 2) SF_HIP(hipGraphLaunch(h->gexec, h->stream));
     if (n_steps & 1)

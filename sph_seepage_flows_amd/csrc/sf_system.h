@@ -74,6 +74,7 @@ struct sf_system {
     hipGraphExec_t gexec = nullptr;
     float graph_dt = 0.f;
     int graph_parity = -1;
+    const uint32_t* graph_keys = nullptr;  // sorted_keys when the graph was captured
     bool graph_dirty = true;
     // differentiable window (sf_diff_*), null until first used
     DiffState* diff = nullptr;
@@ -110,6 +111,14 @@ struct sf_system {
     }
     uint32_t local_cells() const { return K.C; }
     int parity() const { return cur | (cur_cs << 1) | (cur_aux << 2); }
+    // Two steps leave every buffer where they found it, so that a captured
+    // pair of steps can be replayed: with the incremental sort this holds once
+    // the previous sorted keys sit in its two buffers (after a first build and
+    // one incremental sort; before, the first sort is a radix sort whose keys
+    // sit elsewhere); the radix sort always ends in the same buffer.
+    bool keys_cycle() const {
+        return !sort_inc || (built && (sorted_keys == skeys[0].p || sorted_keys == skeys[1].p));
+    }
     void drop_graph() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);

@@ -92,3 +92,21 @@ def test_profiled_step_matches_plain_step():
         d = SFSystem.from_scene(sc)
         assert d.step_timed(1e-4, 2, name)[1] > 0
     assert _abi.lib().sf_step_timed(c._h, 1e-4, 1, len(_abi.PASSES), None, None) == _abi.SF_E_INVALID
+
+
+def test_graph_after_an_eager_radix_step_matches_plain_steps():
+    """One plain step (the first build: a radix sort), then graph replays: the
+    replays must read the previous keys from the buffers the incremental sort
+    rewrites, not from the radix sort's."""
+    sc = scenes.small_block()
+    a = SFSystem.from_scene(sc)
+    b = SFSystem.from_scene(sc)
+    a.step(1e-4, 8)
+    b.step(1e-4, 1)
+    b.step_graph(1e-4, 7)
+    for f in ("pos", "vel", "col", "sort_perm", "sorted_keys"):
+        np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
+    # an odd count and a second call reuse the graph
+    a.step(1e-4, 5)
+    b.step_graph(1e-4, 5)
+    np.testing.assert_array_equal(a.get("pos"), b.get("pos"))
