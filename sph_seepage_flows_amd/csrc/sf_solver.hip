@@ -40,14 +40,17 @@ SF_DEV float eos(const SFConst& k, float rho) {
 // the survivors to the lane's list; the expensive pair bodies then run over
 // the compacted lists, so a wave no longer executes every body for every
 // candidate that some lane happens to keep.  The flush is decided wave-wide
-// before appending, so every list stays within CAP.  NB is the workgroup size.
+// before appending (a single append, or a round of four in filter_push), so
+// every list stays within CAP + 3.  NB is the workgroup size.
 template <int NB, int CAP>
 struct NList {
     uint32_t* slot;  // this lane's column of the LDS list: slot[k * NB]
     int cnt;
 };
+// CAP entries per lane trigger a flush; three more slots take the overflow
+// of a four-candidate round (filter_push checks once per round).
 #define SF_NLIST(name)                                   \
-    __shared__ uint32_t name##_lds[CAP * NB];            \
+    __shared__ uint32_t name##_lds[(CAP + 3) * NB];      \
     NList<NB, CAP> name{name##_lds + threadIdx.x, 0}
 
 // The pair body is split in two: fetch(j) issues every load a pair needs,
@@ -72,7 +75,7 @@ SF_DEV void nl_flush2(NList<NB, CAP>& L, Fetch&& fetch, Apply&& apply) {
 }
 template <int NB, int CAP, class Fetch, class Apply>
 SF_DEV void nl_push2(NList<NB, CAP>& L, bool keep, uint32_t j, Fetch&& fetch, Apply&& apply) {
-    if (__ballot(L.cnt == CAP)) nl_flush2(L, fetch, apply);
+    if (__ballot(L.cnt >= CAP)) nl_flush2(L, fetch, apply);
     L.slot[L.cnt * NB] = j;
     L.cnt += keep ? 1 : 0;
 }
@@ -84,19 +87,37 @@ SF_DEV uint32_t scene_base(const SFConst& k, const Fields& f, int i) {
     else return 0u;
 }
 
-// Every S-th slot of [s, e) with four independent loads in flight per lane
-// (the loop is otherwise bound by one dependent global load per candidate).
-template <uint32_t S, class F>
-SF_DEV void for_range4s(const float4* __restrict__ pm, uint32_t s, uint32_t e, F&& fn) {
+// Every S-th slot j of [s, e) whose test(j, pm[j]) holds goes to the list,
+// in slot order.  Four candidates per round: four independent loads in
+// flight per lane (the loop is otherwise bound by one dependent global load
+// per candidate) from one base address with immediate offsets, and one
+// list-full check per round (a list at CAP or more is flushed first; a round
+// then leaves at most CAP + 3 entries, which the list has room for).
+// The pairs reach apply in the same order whenever the flushes fall.
+template <uint32_t S, int NB, int CAP, class Test, class Fetch, class Apply>
+SF_DEV void filter_push(NList<NB, CAP>& L, const float4* __restrict__ pm, uint32_t s, uint32_t e, Test&& test,
+                        Fetch&& fetch, Apply&& apply) {
+    static_assert(CAP >= 4, "a round pushes up to four candidates");
     uint32_t j = s;
     for (; j + 3u * S < e; j += 4u * S) {
-        const float4 a = ldo(pm, j), b = ldo(pm, j + S), c = ldo(pm, j + 2u * S), d = ldo(pm, j + 3u * S);
-        fn(j, a);
-        fn(j + S, b);
-        fn(j + 2u * S, c);
-        fn(j + 3u * S, d);
+        // slot offsets stay below 2^31 bytes, so base + 16 q S does not wrap
+        const float4* q = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(pm) + j * 16u);
+        const float4 a = q[0], b = q[S], c = q[2u * S], d = q[3u * S];
+        const bool ka = test(j, a), kb = test(j + S, b), kc = test(j + 2u * S, c), kd = test(j + 3u * S, d);
+        if (__ballot(L.cnt >= CAP)) nl_flush2(L, fetch, apply);
+        L.slot[L.cnt * NB] = j;
+        L.cnt += ka ? 1 : 0;
+        L.slot[L.cnt * NB] = j + S;
+        L.cnt += kb ? 1 : 0;
+        L.slot[L.cnt * NB] = j + 2u * S;
+        L.cnt += kc ? 1 : 0;
+        L.slot[L.cnt * NB] = j + 3u * S;
+        L.cnt += kd ? 1 : 0;
     }
-    for (; j < e; j += S) fn(j, ldo(pm, j));
+    for (; j < e; j += S) {
+        const float4 a = ldo(pm, j);
+        nl_push2(L, test(j, a), j, fetch, apply);
+    }
 }
 
 // column ranges plus the number of water particles in each: the cell table
@@ -506,11 +527,11 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
             for (int y = ya; y <= yb; ++y) {
                 const uint32_t base = cb + (uint32_t)((x * k.gy + y) * k.gz);
                 const uint32_t s0 = ldo(f.ct, base + (uint32_t)za).x, e0 = ldo(f.ct, base + (uint32_t)zb + 1u).x;
-                for_range4s<kSmL>(f.pm, s0 + sub, e0, [&](uint32_t j, const float4& pj) {
+                filter_push<kSmL>(L, f.pm, s0 + sub, e0, [&](uint32_t j, const float4& pj) {
                     const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
                     const float r2 = dx * dx + dy * dy + dz * dz;
-                    nl_push2(L, pj.w < 0.f && (int)j != i && r2 < cap_r2, j, fetch_sand, contact);
-                });
+                    return pj.w < 0.f && (int)j != i && r2 < cap_r2;
+                }, fetch_sand, contact);
             }
         nl_flush2(L, fetch_sand, contact);
     }
@@ -541,11 +562,11 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
     };
     for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t) {
         if (nw == 0) return;
-        for_range4s<kSmL>(f.pm, s + sub, e, [&](uint32_t j, const float4& pj) {
+        filter_push<kSmL>(L, f.pm, s + sub, e, [&](uint32_t, const float4& pj) {
             const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            nl_push2(L, pj.w > 0.f && r2 < k.h2, j, fetch_water, water);
-        });
+            return pj.w > 0.f && r2 < k.h2;
+        }, fetch_water, water);
     });
     nl_flush2(L, fetch_water, water);
     if (k.blabel1) {  // ComputeSFUniRadiusDemBoundaryForces: boundary label 1 only
@@ -662,11 +683,13 @@ __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int 
         }
     };
     for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t, uint32_t) {
-        for_range4s<kWmL>(f.pm, s + sub, e, [&](uint32_t j, const float4& pj) {
+        // the particle's own slot passes too: d = 0 makes its term exactly zero
+        // (the spiky factor is 0 below r = eps), and acc adds g afterwards
+        filter_push<kWmL>(L, f.pm, s + sub, e, [&](uint32_t, const float4& pj) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            nl_push2(L, r2 < k.h2 && (int)j != i, j, fetch, pair);
-        });
+            return r2 < k.h2;
+        }, fetch, pair);
     });
     nl_flush2(L, fetch, pair);
     for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
