@@ -128,7 +128,8 @@ int ensure_sort_capacity(sf_system* h, size_t m) {
     size_t nblocks = (c + kRsTile - 1) / kRsTile;
     if ((rc = h->hist.alloc(256 * nblocks))) return rc;
     if ((rc = h->wpre.alloc(c + 1)) || (rc = h->wpart.alloc((size_t)wprefix_blocks((int)c + 1) + 1)) ||
-        (rc = h->plist.alloc(c)) || (rc = h->wflag.alloc(c)))
+        (rc = h->plist.alloc(c)) || (rc = h->wflag.alloc(c)) || (rc = h->pph.alloc(c)) || (rc = h->ilist.alloc(c)) ||
+        (rc = h->icount.alloc(2)))
         return rc;
     if ((rc = h->skeys[0].alloc(c)) || (rc = h->skeys[1].alloc(c)) || (rc = h->sperm.alloc(c)) ||
         (rc = h->slot.alloc(c)) || (rc = h->in_list.alloc(c)) || (rc = h->stay_pre.alloc(c + 1)))
@@ -196,7 +197,7 @@ int build_neighbors(sf_system* h, hipEvent_t* ev) {
     g.wflag = h->wflag.p;
     g.multi = h->K.multi;
     SF_MARK(ev, SF_PASS_GATHER);
-    launch_gather_wprefix(g, n, h->wpart.p, h->wpre.p, h->plist.p, s);
+    launch_gather_wprefix(g, n, h->wpart.p, h->wpre.p, h->plist.p, h->pph.p, s);
     SF_MARK(ev, SF_PASS_CELL_START);
     launch_cell_start(h->sorted_keys, n, h->C, h->wpre.p, h->cell.p, h->ct.p, h->in_count.p, s);
     h->cur ^= 1;

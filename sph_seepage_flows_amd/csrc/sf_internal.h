@@ -102,10 +102,10 @@ struct GatherArgs {
     int multi;
 };
 // Reorders the persistent state by perm, then wpre[0..n] = water count among
-// slots [0, i) and plist = water slots then sand slots, each in slot order.
-// wpart holds wprefix_blocks(n + 1) + 1 entries.
+// slots [0, i), plist = water slots then sand slots, each in slot order, and
+// pph[t] = (reordered) pm[plist[t]].  wpart holds wprefix_blocks(n + 1) + 1 entries.
 void launch_gather_wprefix(const GatherArgs& a, int n, uint32_t* wpart, uint32_t* wpre, uint32_t* plist,
-                           hipStream_t s);
+                           float4* pph, hipStream_t s);
 void launch_boundary_gather(const uint32_t* perm, const float4* bp_in, float4* bp_out, int nb, hipStream_t s);
 void launch_boundary_volume(const SFConst& k, float4* bpv, const uint32_t* bcs, int nb, hipStream_t s);
 void launch_nbr_count(const SFConst& k, const float4* pm, const uint16_t* scene, const uint32_t* cs, uint32_t* cnt,
@@ -155,13 +155,21 @@ struct Fields {
     // slots, each in slot order; *nwater = nw (device)
     const uint32_t* plist;
     const uint32_t* nwater;
+    // pph[t] = pm[plist[t]]: positions and masses in phase order, so the
+    // water candidates of a column range [s, e) are pph[wpre[s], wpre[e]) and
+    // the sand ones pph[nw + s - wpre[s], nw + e - wpre[e])
+    const float4* pph;
+    // interface particles of the avg-flow / voidage-adhesion passes (phase
+    // indices t): water at ilist[0, icount[0]), sand at ilist[nw, nw + icount[1])
+    uint32_t* ilist;
+    uint32_t* icount;
 };
 // Launch shape of a neighbour pass: workgroup size and the per-lane LDS
 // candidate-list capacity (the latter only for the passes that use one).
 struct PassCfg {
     int nb = 256;
     int cap = 16;
-    int lanes = 1;  // lanes per particle (interface passes P2, P3)
+    int lanes = 4;  // lanes per particle (interface passes P2, P3)
 };
 constexpr int kNbChoices[3] = {64, 128, 256};
 constexpr int kCapChoices[2] = {8, 16};

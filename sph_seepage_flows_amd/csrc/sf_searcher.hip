@@ -487,12 +487,20 @@ __global__ __launch_bounds__(kBlock) void k_wprefix_plist(const uint8_t* __restr
         plist[water ? pre : nwater + (uint32_t)i - pre] = (uint32_t)i;
     });
 }
+// pph[t] = pm[plist[t]]: coalesced stores, and loads that walk each phase's
+// slots in increasing order
+__global__ __launch_bounds__(kBlock) void k_phase_pos(const uint32_t* __restrict__ plist, const float4* __restrict__ pm,
+                                                      int n, float4* __restrict__ pph) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t < (uint32_t)n) pph[t] = ldo(pm, ldo(plist, t));
+}
 void launch_gather_wprefix(const GatherArgs& a, int n, uint32_t* wpart, uint32_t* wpre, uint32_t* plist,
-                           hipStream_t s) {
+                           float4* pph, hipStream_t s) {
     const int nb = wprefix_blocks(n + 1);
     k_gather<<<nb, kBlock, 0, s>>>(a, n, wpart);
     k_rs_scan2<<<1, 1024, 0, s>>>(wpart, nb, 0, wpart + nb);
     k_wprefix_plist<<<nb, kBlock, 0, s>>>(a.wflag, n, wpart, wpre, plist);
+    if (n > 0) k_phase_pos<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(plist, a.pm_out, n, pph);
 }
 
 // ---------------------------------------------------------------------------

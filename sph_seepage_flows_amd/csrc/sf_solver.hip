@@ -133,6 +133,20 @@ SF_DEV void for_ranges(const SFConst& k, const Fields& f, uint32_t cb, int cx, i
     });
 }
 
+// Phase-index ranges of one phase over the 9 neighbour columns, in the order
+// of for_ranges: a column's slot range [s, e) holds the water candidates
+// pph[wpre[s], wpre[e]) and the sand candidates pph[nw + s - wpre[s],
+// nw + e - wpre[e]), each in slot order (the cell table holds both ends).
+template <class F>
+SF_DEV void for_phase_ranges(const SFConst& k, const Fields& f, uint32_t cb, int cx, int cy, int cz, bool water,
+                             uint32_t nw, F&& fn) {
+    for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+        const uint2 a = ldo(f.ct, cb + c0), b = ldo(f.ct, cb + c1);
+        const uint32_t t0 = water ? a.y : nw + (a.x - a.y), t1 = water ? b.y : nw + (b.x - b.y);
+        if (t0 < t1) fn(t0, t1);
+    });
+}
+
 SF_DEV float sq3(float d) { return d * d * d; }
 
 // ---------------------------------------------------------------------------
@@ -166,7 +180,9 @@ SF_DEV T lane_value(T v, int lane) {
 // (uniform control flow).
 // With L > 1 lanes per particle (adjacent lanes, same cell), lane sub of a
 // particle takes the window's slots j = first + sub, first + sub + L, ...
-template <int L = 1, class Prep, class Need, class Fn>
+// With Water set, the walk covers only the water candidates: phase indices
+// [wpre[s], wpre[e]) of each column range, staged from pph.
+template <int L = 1, bool Water = false, class Prep, class Need, class Fn>
 SF_DEV void staged_walk(const SFConst& k, const Fields& f, float4* __restrict__ stage, bool active, uint32_t cb,
                         int cx, int cy, int cz, Prep&& prep, Need&& need, Fn&& fn) {
     const int lane = (int)(threadIdx.x & 63u);
@@ -196,15 +212,17 @@ SF_DEV void staged_walk(const SFConst& k, const Fields& f, float4* __restrict__ 
                     a = ldo(f.ct, base + (uint32_t)zlo);
                     b = ldo(f.ct, base + (uint32_t)zhi + 1u);
                 }
-                const uint32_t us = lane_value(a.x, lead), ue = lane_value(b.x, last);
+                const uint32_t ax = Water ? a.y : a.x, bx = Water ? b.y : b.x;
+                const uint32_t us = lane_value(ax, lead), ue = lane_value(bx, last);
                 if (us >= ue) continue;
                 const uint32_t nw = lane_value(b.y, last) - lane_value(a.y, lead);
-                if (!need(nw, (ue - us) - nw)) continue;
+                if (!need(nw, (lane_value(b.x, last) - lane_value(a.x, lead)) - nw)) continue;
+                const float4* __restrict__ src = Water ? f.pph : f.pm;
                 for (uint32_t u = us; u < ue; u += (uint32_t)kStage) {
                     const uint32_t m = ue - u < (uint32_t)kStage ? ue - u : (uint32_t)kStage;
-                    for (uint32_t q = (uint32_t)lane; q < m; q += 64u) stage[q] = prep(ldo(f.pm, u + q));
+                    for (uint32_t q = (uint32_t)lane; q < m; q += 64u) stage[q] = prep(ldo(src, u + q));
                     wave_sync();
-                    const uint32_t wlo = a.x > u ? a.x : u, whi = b.x < u + m ? b.x : u + m;
+                    const uint32_t wlo = ax > u ? ax : u, whi = bx < u + m ? bx : u + m;
                     constexpr uint32_t S = (uint32_t)L;
                     uint32_t j = wlo + sub;
                     for (; j + 3u * S < whi; j += 4u * S) {
@@ -263,8 +281,9 @@ void dispatch(const Fields& f, PassCfg c, L&& l) {
 
 // ---------------------------------------------------------------------------
 // P1: water density (self counted twice, :466 + loop) and pressure; sand gets
-// rho_sand and p = 0.  Branch-free: max(h^2 - r^2, 0) vanishes outside the
-// support and max(w, 0) drops sand (w = -mass).
+// rho_sand and p = 0.  Only water candidates are walked (sand adds nothing);
+// branch-free: max(h^2 - r^2, 0) vanishes outside the support.  The kDenL
+// lanes of a particle split both the candidates and the boundary particles.
 constexpr int kDenL = 2;  // lanes per water particle in the density pass
 template <bool B, int NB>
 __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
@@ -288,31 +307,32 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
     int cx, cy, cz;
     cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
     float sf = 0.f, sb = 0.f;
-    staged_walk<kDenL>(
-        k, f, stage, water, cb, cx, cy, cz,
-        [](float4 p) {
-            p.w = fmaxf(p.w, 0.f);
-            return p;
-        },
-        [](uint32_t nw, uint32_t) { return nw != 0u; },
+    staged_walk<kDenL, true>(
+        k, f, stage, water, cb, cx, cy, cz, [](float4 p) { return p; }, [](uint32_t, uint32_t) { return true; },
         [&](uint32_t, const float4& pj) {
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             // h^2 - r^2 as one fused chain (three FMAs, one rounding each)
             const float d = fmaxf(fmaf(-dz, dz, fmaf(-dy, dy, fmaf(-dx, dx, k.h2))), 0.f);
             sf += pj.w * sq3(d);
         });
+    if (water) {
+        const uint32_t sub = threadIdx.x % kDenL;
+        for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+            const uint32_t be = ldo(f.bcell, c1);
+            for (uint32_t j = ldo(f.bcell, c0) + sub; j < be; j += kDenL) {
+                const float4 b = ldo(f.bpv, j);
+                const float dx = pi.x - b.x, dy = pi.y - b.y, dz = pi.z - b.z;
+                const float d = fmaxf(k.h2 - (dx * dx + dy * dy + dz * dz), 0.f);
+                sb += fabsf(b.w) * sq3(d);
+            }
+        });
+    }
 #pragma unroll
-    for (int o = 1; o < kDenL; o <<= 1) sf += __shfl_xor(sf, o);
+    for (int o = 1; o < kDenL; o <<= 1) {
+        sf += __shfl_xor(sf, o);
+        sb += __shfl_xor(sb, o);
+    }
     if (!water || !lead) return;
-    for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
-        const uint32_t be = ldo(f.bcell, c1);
-        for (uint32_t j = ldo(f.bcell, c0); j < be; ++j) {
-            const float4 b = ldo(f.bpv, j);
-            const float dx = pi.x - b.x, dy = pi.y - b.y, dz = pi.z - b.z;
-            const float d = fmaxf(k.h2 - (dx * dx + dy * dy + dz * dz), 0.f);
-            sb += fabsf(b.w) * sq3(d);
-        }
-    });
     const float rho = pi.w * k.w0 + k.w_coef * (sf + k.rho0 * sb);
     f.rp[i] = make_float2(rho, eos(k, rho));
 }
@@ -324,40 +344,154 @@ void launch_density(const SFConst& k, const Fields& f, int n, PassCfg c, hipStre
 }
 
 // ---------------------------------------------------------------------------
+// Interface passes (P2, P3).  Each particle needs only the other phase's
+// neighbours.  Most particles have none in their 9 column ranges (the cell
+// table gives each range's water and sand counts): k_interface writes the
+// empty sums' results for those and lists the rest, water and sand apart.
+// The passes then run over the lists only, on a phase-balanced launch (water
+// blocks, then sand blocks: no wave mixes the two phases' branches) with LP
+// adjacent lanes per particle, and walk just the other phase's candidates.
+// Which thread takes a listed particle does not matter: each particle's sums
+// are its own lanes' work in candidate order, so the results do not depend
+// on the list order the atomics leave.
+__global__ void k_zero_counts(uint32_t* __restrict__ c) {
+    if (threadIdx.x < 2) c[threadIdx.x] = 0u;
+}
+template <bool B>
+__global__ __launch_bounds__(kBlock) void k_interface(SFConst k, Fields f, int n) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = t < (uint32_t)n;
+    const uint32_t nw = *f.nwater;
+    const bool sand_i = t >= nw;
+    int i = 0;
+    uint32_t other = 0;
+    if (valid) {
+        i = (int)ldo(f.plist, t);
+        const float4 pi = ldo(f.pph, t);
+        const uint32_t cb = scene_base<B>(k, f, i);
+        int cx, cy, cz;
+        cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
+        for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+            const uint2 a = ldo(f.ct, cb + c0), b = ldo(f.ct, cb + c1);
+            const uint32_t w = b.y - a.y;
+            other += sand_i ? w : (b.x - a.x) - w;
+        });
+    }
+    // one atomic per wave and phase (a wave holds both phases at most once,
+    // where t crosses nw); each listed lane takes its rank among the wave's
+    const bool listed = valid && other != 0u;
+    const int lane = (int)(threadIdx.x & 63u);
+    for (int ph = 0; ph < 2; ++ph) {
+        const bool mine = listed && sand_i == (ph == 1);
+        const uint64_t m = __ballot(mine);
+        if (m == 0) continue;
+        const int leader = __builtin_ctzll(m);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(f.icount + ph, (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, leader, 64);
+        if (mine) f.ilist[(ph ? nw : 0u) + base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = t;
+    }
+    if (!valid || listed) return;
+    // what P2 then P3 leave with empty sums
+    f.flow[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    f.adh[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (sand_i) {
+        f.es[i] = make_float2(0.f, 0.f);
+    } else {
+        f.es[i] = make_float2(1.f, 0.f);
+        f.xf[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        const float2 rpi = ldo(f.rp, (uint32_t)i);
+        const float rt = rpi.x / fmaxf(kEps, 1.f);
+        f.wt[i] = make_float2(rt, rpi.y / fmaxf(kEps, rt * rt));
+    }
+    if (f.den4) {
+        reinterpret_cast<float*>(f.den4)[4 * i] = 0.f;
+        reinterpret_cast<float*>(f.den4)[4 * i + 1] = 0.f;
+    }
+}
+void launch_interface(const SFConst& k, const Fields& f, int n, hipStream_t s) {
+    k_zero_counts<<<1, 64, 0, s>>>(f.icount);
+    if (f.scene) k_interface<true><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, f, n);
+    else k_interface<false><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, f, n);
+}
+
+template <int NB, int LP>
+struct PhaseSlot {
+    int i;         // the particle's slot
+    uint32_t sub;  // its lane among LP
+    bool sand;
+};
+// The listed particle of this thread (false: none).
+template <int NB, int LP>
+SF_DEV bool phase_slot(const Fields& f, uint32_t nw, PhaseSlot<NB, LP>& ps) {
+    constexpr uint32_t PB = NB / LP;  // particles per block
+    const uint32_t cw = f.icount[0], cs = f.icount[1];
+    const PhaseBlock pb = phase_block((cw + PB - 1) / PB, (cs + PB - 1) / PB);
+    if (pb.phase < 0) return false;
+    ps.sand = pb.phase == 1;
+    const uint32_t q = pb.blk * PB + threadIdx.x / LP;
+    ps.sub = threadIdx.x % LP;
+    if (q >= (ps.sand ? cs : cw)) return false;
+    ps.i = (int)ldo(f.plist, ldo(f.ilist, (ps.sand ? nw : 0u) + q));
+    return true;
+}
+// enough blocks for every particle listed
+inline int phase_slot_grid(int n, int lanes, int nb) { return phase_grid(n * lanes, nb); }
+
+// ---------------------------------------------------------------------------
 // P2 (ComputeSFAvgFlow_CUDA): water i -> voidage 1 - sum_sand V W and the
 // average sand velocity; sand i -> average water velocity and saturation.
 // (Sand voidage from this pass is overwritten by P3 before anything reads it,
 // so a sand particle only needs its water neighbours, a water particle only
-// its sand neighbours: ranges without the other phase are skipped.)
+// its sand neighbours.)
 // Water particles also store wt = (rho / max(eps, voidage), p / max(eps, that^2)),
 // the per-particle factors the water momentum pass needs from every neighbour.
-template <bool B, int NB, int LP>
+// Candidates are filtered four at a time into the lane's list; the flush
+// fetches each kept neighbour's fields (slot j = plist[u]) ahead of its use.
+template <bool B, int NB, int CAP, int LP>
 __global__ __launch_bounds__(NB) void k_avg_flow(SFConst k, Fields f, int n) {
-    const int i = (int)(xcd_block() * (NB / LP) + threadIdx.x / LP);
-    const uint32_t sub = threadIdx.x % LP;
-    if (i >= n) return;
+    SF_NLIST(L);
+    const uint32_t nw = *f.nwater;
+    PhaseSlot<NB, LP> ps;
+    if (!phase_slot(f, nw, ps)) return;
+    const int i = ps.i;
+    const uint32_t sub = ps.sub;
+    const bool sand_i = ps.sand;
     const float4 pi = ldo(f.pm, (uint32_t)i);
-    const bool sand_i = pi.w < 0.f;
     const uint32_t cb = scene_base<B>(k, f, i);
     int cx, cy, cz;
     cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
     float sandv = 0.f, vsum = 0.f;
     v3 av = mk3(0.f, 0.f, 0.f);
     const float inv_rho_s = k.rho_s != 0.f ? 1.f / k.rho_s : 0.f;
-    for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t ns) {
-        if ((sand_i ? nw : ns) == 0) return;
-        for (uint32_t j = s + sub; j < e; j += LP) {
-            const float4 pj = ldo(f.pm, j);
-            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 >= k.h2 || (pj.w < 0.f) == sand_i) continue;
-            const float w = k.w_coef * sq3(k.h2 - r2);
-            const float vw = (pj.w < 0.f ? -pj.w * inv_rho_s : pj.w * frcp(ldo(f.rp, j).x)) * w;
-            if (!sand_i) sandv += vw;
-            av += xyz(ldo(f.vr, j)) * vw;
-            vsum += vw;
-        }
+    auto in_range = [&](uint32_t, const float4& pj) {
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        return r2 < k.h2;
+    };
+    struct Nbr {
+        float4 pj, vj;
+        float rho;
+    };
+    // sand i: water neighbours (rho_j needed); water i: sand neighbours
+    auto fetch = [&](uint32_t u) {
+        const uint32_t j = ldo(f.plist, u);
+        return Nbr{ldo(f.pph, u), ldo(f.vr, j), sand_i ? ldo(f.rp, j).x : 0.f};
+    };
+    auto pair = [&](uint32_t, const Nbr& q) {
+        const float4 pj = q.pj;
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float w = k.w_coef * sq3(k.h2 - r2);
+        const float vw = (sand_i ? pj.w * frcp(q.rho) : -pj.w * inv_rho_s) * w;
+        if (!sand_i) sandv += vw;
+        av += xyz(q.vj) * vw;
+        vsum += vw;
+    };
+    for_phase_ranges(k, f, cb, cx, cy, cz, sand_i, nw, [&](uint32_t t0, uint32_t t1) {
+        filter_push<LP>(L, f.pph, t0 + sub, t1, in_range, fetch, pair);
     });
+    nl_flush2(L, fetch, pair);
     sandv = lane_sum<LP>(sandv);
     vsum = lane_sum<LP>(vsum);
     av = mk3(lane_sum<LP>(av.x), lane_sum<LP>(av.y), lane_sum<LP>(av.z));
@@ -375,40 +509,57 @@ __global__ __launch_bounds__(NB) void k_avg_flow(SFConst k, Fields f, int n) {
 }
 void launch_avg_flow(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
     if (n <= 0) return;
-    dispatch(f, c, [&](auto B, auto NB, auto) {
+    launch_interface(k, f, n, s);
+    dispatch(f, c, [&](auto B, auto NB, auto CAP) {
         with_lanes(c.lanes, [&](auto LP) {
-            k_avg_flow<decltype(B)::value, decltype(NB)::value, decltype(LP)::value>
-                <<<(n * LP + NB - 1) / NB, (int)NB, 0, s>>>(k, f, n);
+            k_avg_flow<decltype(B)::value, decltype(NB)::value, decltype(CAP)::value, decltype(LP)::value>
+                <<<phase_slot_grid(n, LP, NB), (int)NB, 0, s>>>(k, f, n);
         });
     });
 }
 
 // ---------------------------------------------------------------------------
 // P3: sand i -> voidage from water neighbours; water i -> adhesion force.
-template <bool B, int NB, int LP>
+// Same walk as P2.
+template <bool B, int NB, int CAP, int LP>
 __global__ __launch_bounds__(NB) void k_voidage_adhesion(SFConst k, Fields f, int n) {
-    const int i = (int)(xcd_block() * (NB / LP) + threadIdx.x / LP);
-    const uint32_t sub = threadIdx.x % LP;
-    if (i >= n) return;
+    SF_NLIST(L);
+    const uint32_t nw = *f.nwater;
+    PhaseSlot<NB, LP> ps;
+    if (!phase_slot(f, nw, ps)) return;
+    const int i = ps.i;
+    const uint32_t sub = ps.sub;
     const float4 pi = ldo(f.pm, (uint32_t)i);
-    const bool sand_i = pi.w < 0.f;
     const uint32_t cb = scene_base<B>(k, f, i);
     int cx, cy, cz;
     cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
-    if (sand_i) {
+    auto in_range = [&](uint32_t, const float4& pj) {
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        return r2 < k.h2;
+    };
+    struct Nbr {
+        float4 pj;
+        float2 x;  // sand i: (rho_j, voidage_j) of a water j; water i: saturation of a sand j in y
+    };
+    auto fetch = [&](uint32_t u) {
+        const uint32_t j = ldo(f.plist, u);
+        return Nbr{ldo(f.pph, u), ps.sand ? make_float2(ldo(f.rp, j).x, ldo(f.es, j).x) : ldo(f.es, j)};
+    };
+    if (ps.sand) {
         float num = 0.f, den = 0.f;
-        for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t) {
-            if (nw == 0) return;
-            for (uint32_t j = s + sub; j < e; j += LP) {
-                const float4 pj = ldo(f.pm, j);
-                const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                if (pj.w < 0.f || r2 >= k.h2) continue;
-                const float vw = (pj.w * frcp(ldo(f.rp, j).x)) * (k.w_coef * sq3(k.h2 - r2));
-                num += ldo(f.es, j).x * vw;
-                den += vw;
-            }
+        auto pair = [&](uint32_t, const Nbr& q) {
+            const float4 pj = q.pj;
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const float vw = (pj.w * frcp(q.x.x)) * (k.w_coef * sq3(k.h2 - r2));
+            num += q.x.y * vw;
+            den += vw;
+        };
+        for_phase_ranges(k, f, cb, cx, cy, cz, true, nw, [&](uint32_t t0, uint32_t t1) {
+            filter_push<LP>(L, f.pph, t0 + sub, t1, in_range, fetch, pair);
         });
+        nl_flush2(L, fetch, pair);
         num = lane_sum<LP>(num);
         den = lane_sum<LP>(den);
         if (sub != 0) return;
@@ -421,23 +572,23 @@ __global__ __launch_bounds__(NB) void k_voidage_adhesion(SFConst k, Fields f, in
     const float inv_rho_s = k.rho_s != 0.f ? 1.f / k.rho_s : 0.f;
     float vs = 0.f;
     v3 F = mk3(0.f, 0.f, 0.f);
-    for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t, uint32_t ns) {
-        if (ns == 0) return;
-        for (uint32_t j = s + sub; j < e; j += LP) {
-            const float4 pj = ldo(f.pm, j);
-            const v3 dp = mk3(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
-            const float r2 = dot(dp, dp);
-            if (pj.w >= 0.f || r2 >= k.h2) continue;
-            const float mj = -pj.w;
-            vs += (mj * inv_rho_s) * (k.w_coef * sq3(k.h2 - r2));
-            const float l = fsqrt(r2);
-            const float A = adhesion_A(k, l);
-            if (A != 0.f) {
-                const float G = bezier(k.gadh, ldo(f.es, j).y);
-                F += dp * (G * mi * mj * A * frcp(fmaxf(kEps, l)));
-            }
+    auto pair = [&](uint32_t, const Nbr& q) {
+        const float4 pj = q.pj;
+        const v3 dp = mk3(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
+        const float r2 = dot(dp, dp);
+        const float mj = -pj.w;
+        vs += (mj * inv_rho_s) * (k.w_coef * sq3(k.h2 - r2));
+        const float l = fsqrt(r2);
+        const float A = adhesion_A(k, l);
+        if (A != 0.f) {
+            const float G = bezier(k.gadh, q.x.y);
+            F += dp * (G * mi * mj * A * frcp(fmaxf(kEps, l)));
         }
+    };
+    for_phase_ranges(k, f, cb, cx, cy, cz, false, nw, [&](uint32_t t0, uint32_t t1) {
+        filter_push<LP>(L, f.pph, t0 + sub, t1, in_range, fetch, pair);
     });
+    nl_flush2(L, fetch, pair);
     vs = lane_sum<LP>(vs);
     F = mk3(lane_sum<LP>(F.x), lane_sum<LP>(F.y), lane_sum<LP>(F.z));
     if (sub != 0) return;
@@ -447,10 +598,10 @@ __global__ __launch_bounds__(NB) void k_voidage_adhesion(SFConst k, Fields f, in
 }
 void launch_voidage_adhesion(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
     if (n <= 0) return;
-    dispatch(f, c, [&](auto B, auto NB, auto) {
+    dispatch(f, c, [&](auto B, auto NB, auto CAP) {
         with_lanes(c.lanes, [&](auto LP) {
-            k_voidage_adhesion<decltype(B)::value, decltype(NB)::value, decltype(LP)::value>
-                <<<(n * LP + NB - 1) / NB, (int)NB, 0, s>>>(k, f, n);
+            k_voidage_adhesion<decltype(B)::value, decltype(NB)::value, decltype(CAP)::value, decltype(LP)::value>
+                <<<phase_slot_grid(n, LP, NB), (int)NB, 0, s>>>(k, f, n);
         });
     });
 }
@@ -475,6 +626,7 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
     const uint32_t sub = threadIdx.x % kSmL;  // this lane's share of the particle's candidates
     if (pb.phase < 0 || t >= ns) return;
     const int i = (int)ldo(f.plist, nw + t);
+    const uint32_t self = nw + t;  // this particle's phase index
     const float4 pi = ldo(f.pm, (uint32_t)i);
     if (pi.w >= 0.f) return;
     const uint32_t cb = scene_base<B>(k, f, i);
@@ -505,7 +657,11 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
         float4 pj, vj;
         float satj;
     };
-    auto fetch_sand = [&](uint32_t j) { return SandNbr{ldo(f.pm, j), ldo(f.vr, j), ldo(f.es, j).y}; };
+    // candidates are phase indices u (slot plist[u]; pph[u] = pm[plist[u]])
+    auto fetch_sand = [&](uint32_t u) {
+        const uint32_t j = ldo(f.plist, u);
+        return SandNbr{ldo(f.pph, u), ldo(f.vr, j), ldo(f.es, j).y};
+    };
     auto contact = [&](uint32_t, const SandNbr& q) {
         const float4 pj = q.pj;
         const v3 dji = mk3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
@@ -526,11 +682,12 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
         for (int x = xa; x <= xb; ++x)
             for (int y = ya; y <= yb; ++y) {
                 const uint32_t base = cb + (uint32_t)((x * k.gy + y) * k.gz);
-                const uint32_t s0 = ldo(f.ct, base + (uint32_t)za).x, e0 = ldo(f.ct, base + (uint32_t)zb + 1u).x;
-                filter_push<kSmL>(L, f.pm, s0 + sub, e0, [&](uint32_t j, const float4& pj) {
+                const uint2 a = ldo(f.ct, base + (uint32_t)za), b = ldo(f.ct, base + (uint32_t)zb + 1u);
+                // the sand candidates of the range, as phase indices
+                filter_push<kSmL>(L, f.pph, nw + (a.x - a.y) + sub, nw + (b.x - b.y), [&](uint32_t u, const float4& pj) {
                     const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
                     const float r2 = dx * dx + dy * dy + dz * dz;
-                    return pj.w < 0.f && (int)j != i && r2 < cap_r2;
+                    return u != self && r2 < cap_r2;
                 }, fetch_sand, contact);
             }
         nl_flush2(L, fetch_sand, contact);
@@ -542,7 +699,10 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
         float2 rpj;
         float eps_j;
     };
-    auto fetch_water = [&](uint32_t j) { return WaterNbr{ldo(f.pm, j), ldo(f.xf, j), ldo(f.rp, j), ldo(f.es, j).x}; };
+    auto fetch_water = [&](uint32_t u) {
+        const uint32_t j = ldo(f.plist, u);
+        return WaterNbr{ldo(f.pph, u), ldo(f.xf, j), ldo(f.rp, j), ldo(f.es, j).x};
+    };
     auto water = [&](uint32_t, const WaterNbr& q) {
         const float4 pj = q.pj;
         const v3 dji = mk3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
@@ -560,12 +720,11 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
         }
         fsum += xyz(q.xfj) * (w * vol_i);
     };
-    for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t nw, uint32_t) {
-        if (nw == 0) return;
-        filter_push<kSmL>(L, f.pm, s + sub, e, [&](uint32_t, const float4& pj) {
+    for_phase_ranges(k, f, cb, cx, cy, cz, true, nw, [&](uint32_t t0, uint32_t t1) {
+        filter_push<kSmL>(L, f.pph, t0 + sub, t1, [&](uint32_t, const float4& pj) {
             const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            return pj.w > 0.f && r2 < k.h2;
+            return r2 < k.h2;
         }, fetch_water, water);
     });
     nl_flush2(L, fetch_water, water);

@@ -43,6 +43,8 @@ struct sf_system {
     int cur_aux = 0;  // cda / amc
     // searcher
     DevBuf<uint32_t> keys, kA, kB, vA, vB, hist, cell, bcell, wpre, wpart, plist;
+    DevBuf<float4> pph;  // positions in phase order (Fields::pph)
+    DevBuf<uint32_t> ilist, icount;  // interface particles (Fields::ilist)
     DevBuf<uint8_t> wflag;
     DevBuf<uint2> ct;
     // incremental sort (sf_set_sort_strategy): sorted-key ping-pong, permutation, work arrays
@@ -107,6 +109,9 @@ struct sf_system {
         f.sct = nscenes ? sct.p : nullptr;
         f.plist = plist.p;
         f.nwater = wpre.p + n;
+        f.pph = pph.p;
+        f.ilist = ilist.p;
+        f.icount = icount.p;
         return f;
     }
     uint32_t local_cells() const { return K.C; }
