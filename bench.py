@@ -289,6 +289,19 @@ def main() -> int:
         t0 = time.perf_counter()
         total_ms, dom_ms = g.step_timed(dt, args.steps, dom)
         wall = time.perf_counter() - t0
+        # The same scene after 0.2 s of simulated time (six 1/30 s frames of
+        # the reference driver's loop): the water column has run into the
+        # sand pile and the interface passes carry real work.
+        print("[bench] advancing 6 frames for the mixed state...", file=sys.stderr, flush=True)
+        for _ in range(6):
+            g.update_frame(1.0 / 30.0)
+        _, mixed_pass = g.step_profiled(dt, PROFILE_STEPS, per_pass=True)
+        mixed_ms, _ = g.step_timed(dt, 100, dom)
+        mixed = {"sim_time_s": 0.2 + (args.warmup + PROFILE_STEPS + args.steps) * dt, "steps": 100,
+                 "ms_per_step": mixed_ms / 100, "particle_steps_per_s": n * 100 / (mixed_ms / 1e3),
+                 "pass_ms_profiled": mixed_pass}
+        print(f"[bench] mixed state: {mixed_ms / 100:.3f} ms/step; passes "
+              f"{json.dumps({k: round(v, 4) for k, v in mixed_pass.items()})}", file=sys.stderr, flush=True)
         g.close()
         ms_per_step = total_ms / args.steps
         print(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step (events), wall {wall * 1e3 / args.steps:.3f} "
@@ -331,6 +344,7 @@ def main() -> int:
                               "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
                               "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
             "pass_ms_profiled": pass_ms,
+            "mixed_state": mixed,
         })
     if "ensemble" in parts and not args.no_ensemble:
         print("[bench] batched ensemble...", file=sys.stderr, flush=True)

@@ -380,12 +380,12 @@ int sf_set_sort_strategy(sf_handle h, int strategy);
  * (sand / water momentum only).  Results do not depend on the shape. */
 int sf_set_launch_config(sf_handle h, int pass, int nb, int cap);
 int sf_get_launch_config(sf_handle h, int pass, int* nb, int* cap);
-/* Lanes per particle (1, 2 or 4; default 1) of the two interface passes
+/* Lanes per particle (1, 2 or 4; default 4) of the two interface passes
  * (SF_PASS_AVG_FLOW, SF_PASS_VOIDAGE_ADHESION), whose neighbour sums run only
- * where water and sand meet.  Several lanes per particle speed up small or
- * well-mixed scenes; they regroup those sums, so results change at rounding
- * level, and handles that must agree bit for bit use the same value.  The
- * autotuner keeps it. */
+ * for the particles with neighbours of the other phase.  More lanes shorten
+ * the walks where the phases mix; they regroup those sums, so results change
+ * at rounding level, and handles that must agree bit for bit use the same
+ * value.  The autotuner keeps it. */
 int sf_set_interface_lanes(sf_handle h, int lanes);
 /* Times every shape of every neighbour pass on the handle's current particles
  * (reps launches each, after one neighbour build) and keeps the fastest.

@@ -118,9 +118,9 @@ class SFSystem:
         return bool(e.value), p.value, m.value
 
     def set_interface_lanes(self, lanes: int) -> None:
-        """Lanes per particle (1, 2 or 4) of the avg-flow and voidage/adhesion
-        passes (sf_set_interface_lanes).  More lanes speed up small or mixed
-        scenes; results change at rounding level."""
+        """Lanes per particle (1, 2 or 4; default 4) of the avg-flow and
+        voidage/adhesion passes (sf_set_interface_lanes).  More lanes shorten
+        the walks where the phases mix; results change at rounding level."""
         check(lib().sf_set_interface_lanes(self._h, int(lanes)))
 
     def step_graph(self, dt: float, n: int) -> None:
