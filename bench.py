@@ -227,6 +227,26 @@ def frame_bench(scene, frames: int) -> dict:
     return res
 
 
+def mixed_bench(scene, dt: float, steps: int) -> dict:
+    """The headline scene after 0.2 s of simulated time (six 1/30 s frames of
+    the reference driver's loop from the initial state): the water column has
+    run into the sand pile, so every neighbour pass carries interface work.
+    Then the per-pass breakdown of 10 profiled steps and `steps` steps timed
+    with events around the water momentum pass only."""
+    import sph_seepage_flows_amd as sf
+
+    g = sf.SFSystem.from_scene(scene, device=0)
+    for _ in range(6):
+        g.update_frame(1.0 / 30.0)
+    _, passes = g.step_profiled(dt, PROFILE_STEPS, per_pass=True)
+    total_ms, _ = g.step_timed(dt, steps, "water_momentum")
+    n = g.sizes()[0]
+    g.close()
+    return {"workload": f"{scene.name} after 0.2 s (six 1/30 s frames)", "steps": steps,
+            "ms_per_step": total_ms / steps, "particle_steps_per_s": n * steps / (total_ms / 1e3),
+            "pass_ms_profiled": passes}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,8 +257,9 @@ def main() -> int:
     ap.add_argument("--workload", default="drop_in_1m", choices=["drop_in_1m", "small_block", "dam_4m"])
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-window measurement")
-    ap.add_argument("--parts", default="step,ensemble,diff,frame",
-                    help="comma list of step (the headline workload), ensemble, diff, frame; a profiler can run one")
+    ap.add_argument("--parts", default="step,mixed,ensemble,diff,frame",
+                    help="comma list of step (the headline workload), mixed (the same scene after 0.2 s), ensemble, "
+                         "diff, frame; a profiler can run one")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -273,6 +294,7 @@ def main() -> int:
         "batched": None,
         "differentiation": None,
         "frame_loop": None,
+        "mixed_state": None,
     }
     if "step" in parts:
         print(f"[bench] scene {scene.name}: n={scene.n} nb={scene.nb}", file=sys.stderr, flush=True)
@@ -289,19 +311,6 @@ def main() -> int:
         t0 = time.perf_counter()
         total_ms, dom_ms = g.step_timed(dt, args.steps, dom)
         wall = time.perf_counter() - t0
-        # The same scene after 0.2 s of simulated time (six 1/30 s frames of
-        # the reference driver's loop): the water column has run into the
-        # sand pile and the interface passes carry real work.
-        print("[bench] advancing 6 frames for the mixed state...", file=sys.stderr, flush=True)
-        for _ in range(6):
-            g.update_frame(1.0 / 30.0)
-        _, mixed_pass = g.step_profiled(dt, PROFILE_STEPS, per_pass=True)
-        mixed_ms, _ = g.step_timed(dt, 100, dom)
-        mixed = {"sim_time_s": 0.2 + (args.warmup + PROFILE_STEPS + args.steps) * dt, "steps": 100,
-                 "ms_per_step": mixed_ms / 100, "particle_steps_per_s": n * 100 / (mixed_ms / 1e3),
-                 "pass_ms_profiled": mixed_pass}
-        print(f"[bench] mixed state: {mixed_ms / 100:.3f} ms/step; passes "
-              f"{json.dumps({k: round(v, 4) for k, v in mixed_pass.items()})}", file=sys.stderr, flush=True)
         g.close()
         ms_per_step = total_ms / args.steps
         print(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step (events), wall {wall * 1e3 / args.steps:.3f} "
@@ -344,8 +353,11 @@ def main() -> int:
                               "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
                               "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
             "pass_ms_profiled": pass_ms,
-            "mixed_state": mixed,
         })
+    if "mixed" in parts:
+        print("[bench] mixed state (6 frames first)...", file=sys.stderr, flush=True)
+        out["mixed_state"] = mixed_bench(scene, dt, 100)
+        print(f"[bench] mixed state: {json.dumps(out['mixed_state'])}", file=sys.stderr, flush=True)
     if "ensemble" in parts and not args.no_ensemble:
         print("[bench] batched ensemble...", file=sys.stderr, flush=True)
         out["batched"] = ensemble_bench(dt, 50, 5)
