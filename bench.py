@@ -311,6 +311,17 @@ def main() -> int:
         t0 = time.perf_counter()
         total_ms, dom_ms = g.step_timed(dt, args.steps, dom)
         wall = time.perf_counter() - t0
+        # neighbour-pair throughput of the dominant pass: its particles' pairs
+        # within h (self included: the passes evaluate it, with a zero term)
+        # and their 27-cell candidates, counted on the final state
+        cnt = g.get("nbr_count").astype(np.int64)
+        cand = g.get("nbr_candidates").astype(np.int64)
+        lab = g.get("label")
+        mine = {"water_momentum": lab == 0, "density": lab == 0, "sand_momentum": lab == 1}.get(dom, lab >= 0)
+        pairs, cands = int(cnt[mine].sum()), int(cand[mine].sum())
+        nbr_pairs = {"particles": int(mine.sum()), "pairs_within_h_per_launch": pairs,
+                     "candidates_per_launch": cands, "pairs_per_s": pairs / (dom_ms * 1e-3),
+                     "candidates_per_s": cands / (dom_ms * 1e-3)}
         g.close()
         ms_per_step = total_ms / args.steps
         print(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step (events), wall {wall * 1e3 / args.steps:.3f} "
@@ -348,7 +359,7 @@ def main() -> int:
                        "parallelism": "single-gpu"},
             "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "alg_bytes_per_launch": pb[dom],
-                         "mean_launch_ms": dom_ms, "valu": valu},
+                         "mean_launch_ms": dom_ms, "valu": valu, "neighbour_pairs": nbr_pairs},
             "step_roofline": {"alg_bytes_per_particle_step": step_bytes / n,
                               "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
                               "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
