@@ -297,7 +297,11 @@ int sf_download(sf_handle h, int field, void* dst, size_t dst_bytes);
  * it gives scenes an initial saturation history. */
 int sf_set_field(sf_handle h, int field, const void* src, size_t src_bytes);
 /* Raw device pointer and element stride in bytes of a field's storage
- * (float3 fields are stored as float4, stride 16). Valid until the next step. */
+ * (float3 fields are stored as float4, stride 16). Valid until the next step.
+ * Once the position buffer (SF_FIELD_POS / MASS / LABEL) has been handed out,
+ * every later neighbour build of the handle hashes the positions again, so
+ * writes through it between steps are seen (otherwise the step's advect
+ * already computed the next build's cell keys). */
 int sf_device_buffer(sf_handle h, int field, void** dptr, size_t* stride_bytes);
 /* Writes folder/name.bgeo from the current state (creates folder). */
 int sf_export_bgeo(sf_handle h, const char* folder, const char* name);

@@ -166,16 +166,19 @@ int zero(sf_system* h, DevBuf<T>& b) {
 int build_neighbors(sf_system* h, hipEvent_t* ev) {
     const int n = (int)h->n;
     hipStream_t s = h->stream;
+    const bool inc = h->sort_inc && h->built && h->sorted_keys;
+    const bool fused = inc && h->keys_fresh;  // the last advect hashed and flagged
+    h->keys_fresh = false;
     SF_MARK(ev, SF_PASS_HASH);
-    launch_hash(h->K, h->pm[h->cur].p, h->nscenes ? h->scene.p : nullptr, h->keys.p, n, s);
+    if (!fused) launch_hash(h->K, h->pm[h->cur].p, h->nscenes ? h->scene.p : nullptr, h->keys.p, n, s);
     SortBuffers sb{h->kA.p, h->vA.p, h->kB.p, h->vB.p, h->hist.p};
     SF_MARK(ev, SF_PASS_SORT);
-    if (h->sort_inc && h->built && h->sorted_keys) {
+    if (inc) {
         // the particles are in the last sort's order: rebuild the stable sort
         // from the previous keys and cellStart
         uint32_t* out = h->sorted_keys == h->skeys[0].p ? h->skeys[1].p : h->skeys[0].p;
         IncSortBuffers ib{h->in_count.p, h->in_start.p, h->slot.p, h->in_list.p, h->stay_pre.p, h->part2.p};
-        launch_incremental_sort(h->keys.p, h->sorted_keys, h->cell.p, n, h->C, ib, h->sperm.p, out, s);
+        launch_incremental_sort(h->keys.p, h->sorted_keys, h->cell.p, n, h->C, ib, h->sperm.p, out, fused, s);
         h->sorted_keys = out;
         h->perm = h->sperm.p;
     } else {
@@ -240,7 +243,15 @@ int advect(sf_system* h, float dt, const CflOut* dt_dev, hipEvent_t* ev) {
     a.bcell = h->bcell.p;
     a.dt = dt;
     a.dt_dev = dt_dev;
+    // the next build's hash and change flags, when it will be incremental
+    const bool fuse = h->sort_inc && h->built && h->sorted_keys && !h->pm_shared;
+    a.keys_out = fuse ? h->keys.p : nullptr;
+    a.old_keys = h->sorted_keys;
+    a.in_count = h->in_count.p;
+    a.slot = h->slot.p;
+    a.scene = h->nscenes ? h->scene.p : nullptr;
     launch_advect(h->K, a, (int)h->n, s);
+    h->keys_fresh = fuse;
     h->cur ^= 1;
     SF_MARK(ev, SF_PASS_COUNT);
     SF_HIP(hipGetLastError());
@@ -978,7 +989,13 @@ int sf_device_buffer(sf_handle h, int field, void** dptr, size_t* stride) {
     void* p = nullptr;
     size_t s = 16;
     switch (field) {
-    case SF_FIELD_POS: case SF_FIELD_MASS: case SF_FIELD_LABEL: p = h->pm[h->cur].p; break;
+    case SF_FIELD_POS: case SF_FIELD_MASS: case SF_FIELD_LABEL:
+        // the caller may now write positions: every later build hashes them
+        h->pm_shared = true;
+        h->keys_fresh = false;
+        h->drop_graph();
+        p = h->pm[h->cur].p;
+        break;
     case SF_FIELD_VEL: case SF_FIELD_RADIUS: p = h->vr[h->cur].p; break;
     case SF_FIELD_ACC: p = h->acc.p; break;
     case SF_FIELD_COL: case SF_FIELD_MAX_SATURATION: p = h->cs[h->cur_cs].p; break;
@@ -1097,10 +1114,11 @@ int sf_step_graph(sf_handle h, float dt, int n_steps) {
         --n_steps;
     }
     if (n_steps >= 2 && (h->graph_dirty || !h->gexec || h->graph_dt != dt || h->graph_parity != h->parity() ||
-                         h->graph_keys != h->sorted_keys)) {
+                         h->graph_keys != h->sorted_keys || h->graph_fresh != h->keys_fresh)) {
         h->drop_graph();
         const int parity = h->parity();
         const uint32_t* keys = h->sorted_keys;
+        const bool fresh = h->keys_fresh;
         SF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
         rc = one_step(h, dt);
         if (!rc) rc = one_step(h, dt);
@@ -1115,8 +1133,9 @@ int sf_step_graph(sf_handle h, float dt, int n_steps) {
         h->graph_dt = dt;
         h->graph_parity = parity;
         h->graph_keys = keys;
+        h->graph_fresh = fresh;
         h->graph_dirty = false;
-        if (h->parity() != parity || h->sorted_keys != keys)
+        if (h->parity() != parity || h->sorted_keys != keys || h->keys_fresh != fresh)
             return set_error(SF_E_STATE, "two steps did not restore the buffer parity");
     }
     for (int q = 0; q + 1 < n_steps; q += 2) SF_HIP(hipGraphLaunch(h->gexec, h->stream));

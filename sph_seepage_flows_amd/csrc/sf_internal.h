@@ -121,9 +121,11 @@ struct IncSortBuffers {
     uint32_t* stay_pre;  // n + 1
     uint32_t* part;      // vprefix2_partials(n, C)
 };
+// flags_done: the newcomer ranks (slot) and counts (in_count) were already
+// taken, by the advect that wrote key
 void launch_incremental_sort(const uint32_t* key, const uint32_t* old_key, const uint32_t* old_start, int n,
                              uint32_t C, const IncSortBuffers& b, uint32_t* perm, uint32_t* sorted_keys,
-                             hipStream_t s);
+                             bool flags_done, hipStream_t s);
 // partial-sum counts of the prefix kernels
 int wprefix_blocks(int n);
 int vprefix2_partials(int n1, int n2);
@@ -212,6 +214,16 @@ struct AdvectArgs {
     const uint32_t* bcell;
     float dt;
     const CflOut* dt_dev;  // when non-null, dt = ri / nsub from the device
+    // The next neighbour build's hash and change flags, written here from the
+    // new positions when keys_out is non-null (incremental sort): keys_out[i]
+    // = cell key of the new position (offset by scene * C for batched
+    // handles), and where it differs from old_keys[i] (this step's sorted key
+    // of slot i) slot[i] = the newcomer rank taken from in_count.
+    uint32_t* keys_out;
+    const uint32_t* old_keys;
+    uint32_t* in_count;
+    uint32_t* slot;
+    const uint16_t* scene;
 };
 void launch_advect(const SFConst& k, const AdvectArgs& a, int n, hipStream_t s);
 

@@ -574,11 +574,11 @@ __global__ __launch_bounds__(kBlock) void k_inc_place(const uint32_t* __restrict
 
 void launch_incremental_sort(const uint32_t* key, const uint32_t* old_key, const uint32_t* old_start, int n,
                              uint32_t C, const IncSortBuffers& b, uint32_t* perm, uint32_t* sorted_keys,
-                             hipStream_t s) {
+                             bool flags_done, hipStream_t s) {
     if (n <= 0) return;
     const int gn = (n + kBlock - 1) / kBlock;
     // in_count is zero here: the cell-start kernel after every sort clears it
-    k_inc_flag<<<gn, kBlock, 0, s>>>(key, old_key, n, b.in_count, b.slot);
+    if (!flags_done) k_inc_flag<<<gn, kBlock, 0, s>>>(key, old_key, n, b.in_count, b.slot);
     launch_vprefix2(StayVal{key, old_key}, n, b.stay_pre, ArrayVal{b.in_count}, (int)C, b.in_start, b.part, s);
     k_inc_fill<<<gn, kBlock, 0, s>>>(key, old_key, n, b.in_start, b.slot, b.in_list);
     k_inc_order<<<(int)((C + kBlock - 1) / kBlock), kBlock, 0, s>>>(b.in_start, C, b.in_list);

@@ -1016,6 +1016,14 @@ __global__ __launch_bounds__(kBlock) void k_advect(SFConst k, AdvectArgs a, int 
     }
     a.pm_out[i] = f4(np, p.w);
     a.vr_out[i] = f4(nv, v.w);
+    if (a.keys_out) {  // the next build's hash and change flag (k_hash, k_inc_flag)
+        int cx, cy, cz;
+        cell_of(k, np.x, np.y, np.z, cx, cy, cz);
+        uint32_t key = cell_key(k, cx, cy, cz);
+        if (a.scene) key += (uint32_t)a.scene[i] * k.C;
+        a.keys_out[i] = key;
+        if (key != a.old_keys[i]) a.slot[i] = atomicAdd(a.in_count + key, 1u);
+    }
 }
 void launch_advect(const SFConst& k, const AdvectArgs& a, int n, hipStream_t s) {
     if (n <= 0) return;

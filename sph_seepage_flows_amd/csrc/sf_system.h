@@ -52,6 +52,11 @@ struct sf_system {
     bool sort_inc = true;
     size_t sort_cap = 0;
     const uint32_t* sorted_keys = nullptr;
+    // keys holds the hash of the current positions and the incremental
+    // sort's change flags are taken (the last advect wrote them); cleared by
+    // anything else that changes positions
+    bool keys_fresh = false;
+    bool pm_shared = false;  // the position buffer was handed out (sf_device_buffer)
     const uint32_t* perm = nullptr;
     // transient fields
     DevBuf<float2> rp, es, wt;
@@ -77,6 +82,7 @@ struct sf_system {
     float graph_dt = 0.f;
     int graph_parity = -1;
     const uint32_t* graph_keys = nullptr;  // sorted_keys when the graph was captured
+    bool graph_fresh = false;              // keys_fresh when the graph was captured
     bool graph_dirty = true;
     // differentiable window (sf_diff_*), null until first used
     DiffState* diff = nullptr;
@@ -116,13 +122,14 @@ struct sf_system {
     }
     uint32_t local_cells() const { return K.C; }
     int parity() const { return cur | (cur_cs << 1) | (cur_aux << 2); }
-    // Two steps leave every buffer where they found it, so that a captured
-    // pair of steps can be replayed: with the incremental sort this holds once
-    // the previous sorted keys sit in its two buffers (after a first build and
-    // one incremental sort; before, the first sort is a radix sort whose keys
-    // sit elsewhere); the radix sort always ends in the same buffer.
+    // Two steps leave every buffer and flag where they found them, so that a
+    // captured pair of steps can be replayed: with the incremental sort this
+    // holds once the previous sorted keys sit in its two buffers (after a
+    // first build and one incremental sort; before, the first sort is a radix
+    // sort whose keys sit elsewhere) and the last advect wrote the next keys;
+    // the radix sort always ends in the same buffer.
     bool keys_cycle() const {
-        return !sort_inc || (built && (sorted_keys == skeys[0].p || sorted_keys == skeys[1].p));
+        return !sort_inc || (built && keys_fresh && (sorted_keys == skeys[0].p || sorted_keys == skeys[1].p));
     }
     void drop_graph() {
         if (gexec) (void)hipGraphExecDestroy(gexec);

@@ -110,3 +110,35 @@ def test_graph_after_an_eager_radix_step_matches_plain_steps():
     a.step(1e-4, 5)
     b.step_graph(1e-4, 5)
     np.testing.assert_array_equal(a.get("pos"), b.get("pos"))
+
+
+def test_positions_written_between_steps_are_hashed_again():
+    """A step's advect writes the next build's cell keys; positions written
+    through the device buffer afterwards must be hashed again, also by a
+    captured graph.  Reference: the same run with the radix sort."""
+    sc = scenes.small_block()
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipMemcpy.restype = C.c_int
+    runs = {}
+    for mode in ("incremental", "incremental_graph", "radix"):
+        g = SFSystem.from_scene(sc)
+        if mode == "radix":
+            g.set_sort_strategy(False)
+        run = g.step_graph if mode == "incremental_graph" else g.step
+        g.step(1e-4, 3)
+        run(1e-4, 4)
+        pos, mass, lab = g.get("pos"), g.get("mass"), g.get("label")
+        pm = np.zeros((sc.n, 4), np.float32)
+        pm[:, :3] = pos
+        pm[:, 0] += np.float32(0.05)  # more than a cell (h = 0.04): every key changes
+        pm[:, 3] = np.where(lab == 1, -mass, mass)  # the label is the sign of the mass
+        ptr, stride = g.device_buffer("pos")
+        assert stride == 16
+        assert hip.hipMemcpy(ptr, pm.ctypes.data, pm.nbytes, 1) == 0  # host to device
+        np.testing.assert_array_equal(g.get("pos"), pm[:, :3])
+        run(1e-4, 4)
+        runs[mode] = {f: g.get(f) for f in ("pos", "vel", "sort_perm", "sorted_keys", "cell_start")}
+    for mode in ("incremental", "incremental_graph"):
+        for f, v in runs["radix"].items():
+            np.testing.assert_array_equal(runs[mode][f], v, err_msg=f"{mode} {f}")
