@@ -200,9 +200,10 @@ int build_neighbors(sf_system* h, hipEvent_t* ev) {
     g.wflag = h->wflag.p;
     g.multi = h->K.multi;
     SF_MARK(ev, SF_PASS_GATHER);
-    launch_gather_wprefix(g, n, h->wpart.p, h->wpre.p, h->plist.p, h->pph.p, s);
+    launch_gather_wprefix(g, n, h->wpart.p, h->wpre.p, h->plist.p, s);
     SF_MARK(ev, SF_PASS_CELL_START);
-    launch_cell_start(h->sorted_keys, n, h->C, h->wpre.p, h->cell.p, h->ct.p, h->in_count.p, s);
+    launch_cell_start(h->sorted_keys, n, h->C, h->wpre.p, h->cell.p, h->ct.p, h->in_count.p,
+                      PhasePos{h->plist.p, g.pm_out, h->pph.p}, s);
     h->cur ^= 1;
     h->cur_cs ^= 1;
     if (h->K.multi) h->cur_aux ^= 1;
@@ -309,7 +310,10 @@ void launch_pass(sf_system* h, int q, PassCfg c) {
     const int n = (int)h->n;
     switch (q) {
     case 0: launch_density(h->K, f, n, c, h->stream); break;
-    case 1: launch_avg_flow(h->K, f, n, c, h->stream); break;
+    case 1:  // the density pass clears the interface counters in a step
+        launch_zero_counts(f, h->stream);
+        launch_avg_flow(h->K, f, n, c, h->stream);
+        break;
     case 2: launch_voidage_adhesion(h->K, f, n, c, h->stream); break;
     case 3: launch_sand_momentum(h->K, f, n, c, h->stream); break;
     default: launch_water_momentum(h->K, f, n, c, h->stream); break;
@@ -534,7 +538,7 @@ int sf_upload_boundary(sf_handle h, size_t nb, const float* pos3, const uint64_t
     SortBuffers sb{h->kA.p, h->vA.p, h->kB.p, h->vB.p, h->hist.p};
     const uint32_t *sk = nullptr, *bperm = nullptr;
     if ((rc = radix_sort(h->keys.p, (int)nb, h->plan, sb, &sk, &bperm, h->stream))) return rc;
-    launch_cell_start(sk, (int)nb, h->local_cells(), nullptr, h->bcell.p, nullptr, nullptr, h->stream);
+    launch_cell_start(sk, (int)nb, h->local_cells(), nullptr, h->bcell.p, nullptr, nullptr, PhasePos{}, h->stream);
     launch_boundary_gather(bperm, tmp.p, h->bpv.p, (int)nb, h->stream);
     launch_boundary_volume(h->K, h->bpv.p, h->bcell.p, (int)nb, h->stream);
     SF_HIP(hipGetLastError());

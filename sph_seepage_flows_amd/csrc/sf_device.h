@@ -215,15 +215,6 @@ SF_DEV v3 world_walls(const SFConst& k, v3 p, v3 v) {
     return f;
 }
 
-// XCD-aware block index: the 8 XCDs receive workgroups round-robin, so map
-// them to 8 contiguous chunks of the cell-sorted particle range; spatial
-// neighbours then share one XCD's L2.
-SF_DEV uint32_t xcd_block() {
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
-    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, l = b >> 3;
-    return x < r ? x * (q + 1u) + l : r * (q + 1u) + (x - r) * q + l;
-}
-
 // Block mapping of a pass over two phases of a slot list: na blocks of phase
 // 0, then nb blocks of phase 1.  Each XCD (hardware blocks b with b % 8 = x)
 // first runs its contiguous 1/8 share of the phase-0 blocks, then its share

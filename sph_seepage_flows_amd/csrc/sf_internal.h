@@ -88,10 +88,17 @@ int radix_sort(const uint32_t* keys, int n, const RadixPlan& plan, const SortBuf
 // ---- searcher launchers (sf_searcher.hip) ----
 // scene (batched handles): per-slot scene id, keys offset by scene * C; null for one scene
 void launch_hash(const SFConst& k, const float4* pm, const uint16_t* scene, uint32_t* keys, int n, hipStream_t s);
+// Positions in phase order, written by the cell-start launch when pph is set:
+// pph[t] = pm[plist[t]] for t < n.
+struct PhasePos {
+    const uint32_t* plist;
+    const float4* pm;
+    float4* pph;
+};
 // cellStart[c] for c in [0, C] plus the cell table ct[c] = (cellStart[c], wpre[cellStart[c]]);
 // zero_c (may be null) is cleared over [0, C) (the incremental sort's newcomer counters)
 void launch_cell_start(const uint32_t* sorted_keys, int n, uint32_t C, const uint32_t* wpre, uint32_t* cell_start,
-                       uint2* ct, uint32_t* zero_c, hipStream_t s);
+                       uint2* ct, uint32_t* zero_c, PhasePos pp, hipStream_t s);
 struct GatherArgs {
     const uint32_t* perm;
     const float4 *pm_in, *vr_in, *cs_in, *cda_in;
@@ -102,10 +109,10 @@ struct GatherArgs {
     int multi;
 };
 // Reorders the persistent state by perm, then wpre[0..n] = water count among
-// slots [0, i), plist = water slots then sand slots, each in slot order, and
-// pph[t] = (reordered) pm[plist[t]].  wpart holds wprefix_blocks(n + 1) + 1 entries.
+// slots [0, i) and plist = water slots then sand slots, each in slot order.
+// wpart holds wprefix_blocks(n + 1) + 1 entries.
 void launch_gather_wprefix(const GatherArgs& a, int n, uint32_t* wpart, uint32_t* wpre, uint32_t* plist,
-                           float4* pph, hipStream_t s);
+                           hipStream_t s);
 void launch_boundary_gather(const uint32_t* perm, const float4* bp_in, float4* bp_out, int nb, hipStream_t s);
 void launch_boundary_volume(const SFConst& k, float4* bpv, const uint32_t* bcs, int nb, hipStream_t s);
 void launch_nbr_count(const SFConst& k, const float4* pm, const uint16_t* scene, const uint32_t* cs, uint32_t* cnt,
@@ -176,7 +183,11 @@ struct PassCfg {
 constexpr int kNbChoices[3] = {64, 128, 256};
 constexpr int kCapChoices[2] = {8, 16};
 void launch_density(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);
+// The avg-flow pass lists the interface particles into counters that start
+// at zero: the density pass clears them (a step runs it first); a caller that
+// repeats the avg-flow pass alone clears them with launch_zero_counts.
 void launch_avg_flow(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);
+void launch_zero_counts(const Fields& f, hipStream_t s);
 void launch_voidage_adhesion(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);
 void launch_sand_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);
 void launch_water_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);

@@ -237,8 +237,12 @@ int radix_sort(const uint32_t* keys, int n, const RadixPlan& plan, const SortBuf
 // (CountingInCell_CUDA + exclusive_scan, cuda_neighbor_searcher.cu:45-47)
 __global__ __launch_bounds__(kBlock) void k_cell_start(const uint32_t* __restrict__ sk, int n, uint32_t C,
                                                        const uint32_t* __restrict__ wpre, uint32_t* __restrict__ cs,
-                                                       uint2* __restrict__ ct, uint32_t* __restrict__ zero_c) {
+                                                       uint2* __restrict__ ct, uint32_t* __restrict__ zero_c,
+                                                       PhasePos pp) {
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    // pph[t] = pm[plist[t]]: coalesced stores, loads that walk each phase's
+    // slots in increasing order
+    if (pp.pph && c < (uint32_t)n) pp.pph[c] = ldo(pp.pm, ldo(pp.plist, c));
     if (c > C) return;
     if (zero_c && c < C) zero_c[c] = 0u;  // the next incremental sort's newcomer counters
     uint32_t lo = 0, hi = (uint32_t)n;
@@ -251,9 +255,11 @@ __global__ __launch_bounds__(kBlock) void k_cell_start(const uint32_t* __restric
     if (ct) ct[c] = make_uint2(lo, wpre ? wpre[lo] : 0u);
 }
 void launch_cell_start(const uint32_t* sorted_keys, int n, uint32_t C, const uint32_t* wpre, uint32_t* cell_start,
-                       uint2* ct, uint32_t* zero_c, hipStream_t s) {
-    const uint32_t total = C + 1;
-    k_cell_start<<<(total + kBlock - 1) / kBlock, kBlock, 0, s>>>(sorted_keys, n, C, wpre, cell_start, ct, zero_c);
+                       uint2* ct, uint32_t* zero_c, PhasePos pp, hipStream_t s) {
+    uint32_t total = C + 1;
+    if (pp.pph && (uint32_t)n > total) total = (uint32_t)n;
+    k_cell_start<<<(total + kBlock - 1) / kBlock, kBlock, 0, s>>>(sorted_keys, n, C, wpre, cell_start, ct, zero_c,
+                                                                  pp);
 }
 
 // ---------------------------------------------------------------------------
@@ -487,20 +493,12 @@ __global__ __launch_bounds__(kBlock) void k_wprefix_plist(const uint8_t* __restr
         plist[water ? pre : nwater + (uint32_t)i - pre] = (uint32_t)i;
     });
 }
-// pph[t] = pm[plist[t]]: coalesced stores, and loads that walk each phase's
-// slots in increasing order
-__global__ __launch_bounds__(kBlock) void k_phase_pos(const uint32_t* __restrict__ plist, const float4* __restrict__ pm,
-                                                      int n, float4* __restrict__ pph) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    if (t < (uint32_t)n) pph[t] = ldo(pm, ldo(plist, t));
-}
 void launch_gather_wprefix(const GatherArgs& a, int n, uint32_t* wpart, uint32_t* wpre, uint32_t* plist,
-                           float4* pph, hipStream_t s) {
+                           hipStream_t s) {
     const int nb = wprefix_blocks(n + 1);
     k_gather<<<nb, kBlock, 0, s>>>(a, n, wpart);
     k_rs_scan2<<<1, 1024, 0, s>>>(wpart, nb, 0, wpart + nb);
     k_wprefix_plist<<<nb, kBlock, 0, s>>>(a.wflag, n, wpart, wpre, plist);
-    if (n > 0) k_phase_pos<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(plist, a.pm_out, n, pph);
 }
 
 // ---------------------------------------------------------------------------

@@ -289,6 +289,8 @@ template <bool B, int NB>
 __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
     __shared__ float4 stage_lds[(NB / 64) * kStage];
     float4* stage = stage_lds + (threadIdx.x >> 6) * kStage;
+    // the interface counters of this step's avg-flow pass start from zero
+    if (blockIdx.x == 0 && threadIdx.x < 2) f.icount[threadIdx.x] = 0u;
     constexpr uint32_t PB = NB / kDenL;  // water particles per block
     const uint32_t nw = *f.nwater, ns = (uint32_t)n - nw;
     const PhaseBlock pb = phase_block((nw + PB - 1) / PB, (ns + NB - 1) / NB);
@@ -409,8 +411,8 @@ __global__ __launch_bounds__(kBlock) void k_interface(SFConst k, Fields f, int n
         reinterpret_cast<float*>(f.den4)[4 * i + 1] = 0.f;
     }
 }
+void launch_zero_counts(const Fields& f, hipStream_t s) { k_zero_counts<<<1, 64, 0, s>>>(f.icount); }
 void launch_interface(const SFConst& k, const Fields& f, int n, hipStream_t s) {
-    k_zero_counts<<<1, 64, 0, s>>>(f.icount);
     if (f.scene) k_interface<true><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, f, n);
     else k_interface<false><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, f, n);
 }
