@@ -120,21 +120,8 @@ SF_DEV void filter_push(NList<NB, CAP>& L, const float4* __restrict__ pm, uint32
     }
 }
 
-// column ranges plus the number of water particles in each: the cell table
-// holds (cellStart[c], water prefix at cellStart[c]) so one 8-byte load per
-// range end gives both.
-template <class F>
-SF_DEV void for_ranges(const SFConst& k, const Fields& f, uint32_t cb, int cx, int cy, int cz, F&& fn) {
-    for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
-        const uint2 a = ldo(f.ct, cb + c0), b = ldo(f.ct, cb + c1);
-        if (a.x == b.x) return;
-        const uint32_t nw = b.y - a.y;
-        fn(a.x, b.x, nw, (b.x - a.x) - nw);
-    });
-}
-
 // Phase-index ranges of one phase over the 9 neighbour columns, in the order
-// of for_ranges: a column's slot range [s, e) holds the water candidates
+// of for_columns: a column's slot range [s, e) holds the water candidates
 // pph[wpre[s], wpre[e]) and the sand candidates pph[nw + s - wpre[s],
 // nw + e - wpre[e]), each in slot order (the cell table holds both ends).
 template <class F>
@@ -239,6 +226,69 @@ SF_DEV void staged_walk(const SFConst& k, const Fields& f, float4* __restrict__ 
             }
         }
     }
+}
+
+// The filtered walk of the water momentum pass: every candidate j of the
+// particle's 9 column ranges with test(pm[j]) goes to the lane's list, lane
+// sub (of S per particle) taking j = s + sub + S m of each range [s, e) in
+// increasing j, exactly as filter_push over the ranges.  The filter is bound
+// by the texture address unit, which costs the same per wave load
+// instruction whatever it carries, so where the wave's ranges overlap it
+// loads them once: when every particle of the wave sits in one grid column
+// (most waves of an ordered region), the lanes walk the same 9 columns, and
+// a column whose union range (first lane's start to last lane's end) fits
+// KS candidates is staged in LDS with one coalesced load and filtered from
+// there.  Other columns, and waves spread over several columns, filter from
+// memory per lane.  Either way a lane's list gets the same entries in the
+// same order, so which way a column goes does not change the sums.  Must be
+// called by all lanes of the wave.
+template <uint32_t S, uint32_t KS, int NB, int CAP, class Test, class Fetch, class Apply>
+SF_DEV void column_filter(const SFConst& k, const Fields& f, float4* __restrict__ stage, bool active, uint32_t cb,
+                          int cx, int cy, int cz, NList<NB, CAP>& L, Test&& test, Fetch&& fetch, Apply&& apply) {
+    static_assert(CAP >= 4, "a round pushes up to four candidates");
+    static_assert(KS <= 64, "one staged candidate per lane");
+    const int lane = (int)(threadIdx.x & 63u);
+    const uint32_t sub = (uint32_t)lane % S;
+    const uint64_t act = __ballot(active);
+    if (!act) return;
+    const int lead = __builtin_ctzll(act), last = 63 - __builtin_clzll(act);
+    const uint32_t col = cb + (uint32_t)((cx * k.gy + cy) * k.gz);
+    const bool one = !__ballot(active && col != lane_value(col, lead));  // wave-uniform
+    if (!one && !active) return;
+    // lanes without a particle follow the lead's columns (with empty ranges)
+    const int wx = active ? cx : lane_value(cx, lead), wy = active ? cy : lane_value(cy, lead);
+    const int wz = active ? cz : lane_value(cz, lead);
+    for_columns(k, wx, wy, wz, [&](uint32_t c0, uint32_t c1) {
+        const uint32_t rs = active ? ldo(f.ct, cb + c0).x : 0u, re = active ? ldo(f.ct, cb + c1).x : 0u;
+        if (one) {
+            const uint32_t us = lane_value(rs, lead), ue = lane_value(re, last);
+            if (us >= ue) return;
+            if (ue - us <= KS) {
+                if ((uint32_t)lane < ue - us) stage[lane] = ldo(f.pm, us + (uint32_t)lane);
+                wave_sync();
+                uint32_t j = rs + sub;
+                for (; j + 3u * S < re; j += 4u * S) {
+                    const float4 a = stage[j - us], b = stage[j + S - us], c = stage[j + 2u * S - us],
+                                 d = stage[j + 3u * S - us];
+                    const bool ka = test(a), kb = test(b), kc = test(c), kd = test(d);
+                    if (__ballot(L.cnt >= CAP)) nl_flush2(L, fetch, apply);
+                    L.slot[L.cnt * NB] = j;
+                    L.cnt += ka ? 1 : 0;
+                    L.slot[L.cnt * NB] = j + S;
+                    L.cnt += kb ? 1 : 0;
+                    L.slot[L.cnt * NB] = j + 2u * S;
+                    L.cnt += kc ? 1 : 0;
+                    L.slot[L.cnt * NB] = j + 3u * S;
+                    L.cnt += kd ? 1 : 0;
+                }
+                for (; j < re; j += S) nl_push2(L, test(stage[j - us]), j, fetch, apply);
+                wave_sync();
+                return;
+            }
+        }
+        if (active)
+            filter_push<S>(L, f.pm, rs + sub, re, [&](uint32_t, const float4& pj) { return test(pj); }, fetch, apply);
+    });
 }
 
 // Lanes per particle of the interface passes (P2, P3), whose work sits at the
@@ -786,20 +836,25 @@ void launch_sand_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, h
 // P5: water linear momentum (pressure + artificial viscosity + seepage +
 // boundary pressure/viscosity), acc = g + a - F_adh / m.  kWmL adjacent lanes
 // share a particle: each filters every kWmL-th candidate of the ranges into
-// its own list, and the partial sums are combined with shuffles.
+// its own list (column_filter, which stages the candidates in LDS where the
+// wave's ranges overlap), and the partial sums are combined with shuffles.
 constexpr int kWmL = 4;
+constexpr uint32_t kWmStage = 48;  // longest union range staged per wave
 template <bool B, int NB, int CAP>
 __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int n) {
     SF_NLIST(L);
+    __shared__ float4 stage_lds[(NB / 64) * kWmStage];
+    float4* stage = stage_lds + (threadIdx.x >> 6) * kWmStage;
     constexpr uint32_t PB = NB / kWmL;  // water particles per block
     const uint32_t nw = *f.nwater;
     const PhaseBlock pb = phase_block((nw + PB - 1) / PB, 0u);
+    if (pb.phase < 0) return;  // the whole block
     const uint32_t t = pb.blk * PB + threadIdx.x / kWmL;
     const uint32_t sub = threadIdx.x % kWmL;  // this lane's share of the particle's candidates
-    if (pb.phase < 0 || t >= nw) return;
-    const int i = (int)ldo(f.plist, t);
+    // lanes past the last particle take part in the staged walk without a range
+    const int i = t < nw ? (int)ldo(f.plist, t) : 0;
     const float4 pi = ldo(f.pm, (uint32_t)i);
-    if (pi.w < 0.f) return;
+    const bool active = t < nw && pi.w >= 0.f;
     const uint32_t cb = scene_base<B>(k, f, i);
     const float mi = pi.w;
     const v3 vi = xyz(ldo(f.vr, (uint32_t)i));
@@ -843,16 +898,16 @@ __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int 
             a += xyz(ldo(f.xf, j)) * (-w * inv_rt_i);
         }
     };
-    for_ranges(k, f, cb, cx, cy, cz, [&](uint32_t s, uint32_t e, uint32_t, uint32_t) {
-        // the particle's own slot passes too: d = 0 makes its term exactly zero
-        // (the spiky factor is 0 below r = eps), and acc adds g afterwards
-        filter_push<kWmL>(L, f.pm, s + sub, e, [&](uint32_t, const float4& pj) {
-            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            return r2 < k.h2;
-        }, fetch, pair);
-    });
+    // the particle's own slot passes too: d = 0 makes its term exactly zero
+    // (the spiky factor is 0 below r = eps), and acc adds g afterwards
+    auto in_range = [&](const float4& pj) {
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        return r2 < k.h2;
+    };
+    column_filter<kWmL, kWmStage>(k, f, stage, active, cb, cx, cy, cz, L, in_range, fetch, pair);
     nl_flush2(L, fetch, pair);
+    if (!active) return;
     for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
         const uint32_t be = ldo(f.bcell, c1);
         for (uint32_t j = ldo(f.bcell, c0) + sub; j < be; j += kWmL) {
