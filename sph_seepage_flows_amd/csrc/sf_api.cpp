@@ -131,6 +131,9 @@ int ensure_sort_capacity(sf_system* h, size_t m) {
         (rc = h->plist.alloc(c)) || (rc = h->wflag.alloc(c)) || (rc = h->pph.alloc(c)) || (rc = h->ilist.alloc(c)) ||
         (rc = h->icount.alloc(2)))
         return rc;
+    // the interface counters start at zero (each step's water momentum pass
+    // clears them for the next)
+    SF_HIP(hipMemsetAsync(h->icount.p, 0, 2 * sizeof(uint32_t), h->stream));
     if ((rc = h->skeys[0].alloc(c)) || (rc = h->skeys[1].alloc(c)) || (rc = h->sperm.alloc(c)) ||
         (rc = h->slot.alloc(c)) || (rc = h->in_list.alloc(c)) || (rc = h->stay_pre.alloc(c + 1)))
         return rc;
@@ -309,9 +312,13 @@ void launch_pass(sf_system* h, int q, PassCfg c) {
     const Fields f = h->fields();
     const int n = (int)h->n;
     switch (q) {
-    case 0: launch_density(h->K, f, n, c, h->stream); break;
-    case 1:  // the density pass clears the interface counters in a step
+    case 0:  // the density pass lists the interface particles
         launch_zero_counts(f, h->stream);
+        launch_density(h->K, f, n, c, h->stream);
+        break;
+    case 1:  // in a step the density pass builds the lists
+        launch_zero_counts(f, h->stream);
+        launch_interface(h->K, f, n, h->stream);
         launch_avg_flow(h->K, f, n, c, h->stream);
         break;
     case 2: launch_voidage_adhesion(h->K, f, n, c, h->stream); break;

@@ -182,12 +182,16 @@ struct PassCfg {
 };
 constexpr int kNbChoices[3] = {64, 128, 256};
 constexpr int kCapChoices[2] = {8, 16};
+// The density pass also lists the interface particles of the avg-flow and
+// voidage/adhesion passes (and writes the empty sums' results of the rest)
+// into counters that start at zero; the water momentum pass clears them for
+// the next step.  A caller that runs the density pass alone clears them
+// first (launch_zero_counts); one that runs the avg-flow pass alone clears
+// them and builds the lists with launch_interface.
 void launch_density(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);
-// The avg-flow pass lists the interface particles into counters that start
-// at zero: the density pass clears them (a step runs it first); a caller that
-// repeats the avg-flow pass alone clears them with launch_zero_counts.
 void launch_avg_flow(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);
 void launch_zero_counts(const Fields& f, hipStream_t s);
+void launch_interface(const SFConst& k, const Fields& f, int n, hipStream_t s);
 void launch_voidage_adhesion(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);
 void launch_sand_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);
 void launch_water_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);

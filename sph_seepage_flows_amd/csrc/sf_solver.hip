@@ -168,10 +168,11 @@ SF_DEV T lane_value(T v, int lane) {
 // With L > 1 lanes per particle (adjacent lanes, same cell), lane sub of a
 // particle takes the window's slots j = first + sub, first + sub + L, ...
 // With Water set, the walk covers only the water candidates: phase indices
-// [wpre[s], wpre[e]) of each column range, staged from pph.
-template <int L = 1, bool Water = false, class Prep, class Need, class Fn>
+// [wpre[s], wpre[e]) of each column range, staged from pph.  range(a, b) sees
+// each of the lane's own 9 cell-table pairs (range start, range end).
+template <int L = 1, bool Water = false, class Prep, class Need, class Fn, class Range>
 SF_DEV void staged_walk(const SFConst& k, const Fields& f, float4* __restrict__ stage, bool active, uint32_t cb,
-                        int cx, int cy, int cz, Prep&& prep, Need&& need, Fn&& fn) {
+                        int cx, int cy, int cz, Prep&& prep, Need&& need, Fn&& fn, Range&& range) {
     const int lane = (int)(threadIdx.x & 63u);
     const uint32_t sub = (uint32_t)lane & (uint32_t)(L - 1);
     const uint32_t col = cb + (uint32_t)((cx * k.gy + cy) * k.gz);
@@ -198,6 +199,7 @@ SF_DEV void staged_walk(const SFConst& k, const Fields& f, float4* __restrict__ 
                 if (mine) {
                     a = ldo(f.ct, base + (uint32_t)zlo);
                     b = ldo(f.ct, base + (uint32_t)zhi + 1u);
+                    range(a, b);
                 }
                 const uint32_t ax = Water ? a.y : a.x, bx = Water ? b.y : b.x;
                 const uint32_t us = lane_value(ax, lead), ue = lane_value(bx, last);
@@ -330,6 +332,58 @@ void dispatch(const Fields& f, PassCfg c, L&& l) {
 }
 
 // ---------------------------------------------------------------------------
+// Interface lists.  The avg-flow and voidage/adhesion passes (P2, P3) need
+// each particle's neighbours of the other phase only.  Most particles have
+// none in their 9 column ranges (the cell table gives each range's water and
+// sand counts).  For those the empty sums' results are written directly
+// (write_no_interface); the rest are listed, water and sand apart
+// (list_interface), and P2 and P3 run over the lists.  In a step the density
+// pass does both, since it already loads every water particle's 9 cell-table
+// pairs; k_interface does the same for a lone avg-flow run.  The counters
+// start at zero: the water momentum pass (after P2 and P3 have read them)
+// clears them for the next step, and a lone density or avg-flow run clears
+// them first (launch_zero_counts).  The list order the atomics leave does
+// not matter: each listed particle's sums are its own lanes' work in
+// candidate order.
+
+// What P2 then P3 leave for particle i with empty sums; rpi = (rho, p).
+SF_DEV void write_no_interface(const Fields& f, int i, bool sand, float2 rpi) {
+    f.flow[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    f.adh[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (sand) {
+        f.es[i] = make_float2(0.f, 0.f);
+    } else {
+        f.es[i] = make_float2(1.f, 0.f);
+        f.xf[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        const float rt = rpi.x / fmaxf(kEps, 1.f);
+        f.wt[i] = make_float2(rt, rpi.y / fmaxf(kEps, rt * rt));
+    }
+    if (f.den4) {
+        reinterpret_cast<float*>(f.den4)[4 * i] = 0.f;
+        reinterpret_cast<float*>(f.den4)[4 * i + 1] = 0.f;
+    }
+}
+// Appends phase index t of every listed lane to its phase's list (water at
+// ilist[0, nw), sand at ilist[nw, n)): one atomic per wave and phase, each
+// listed lane taking its rank among the wave's.  A list never grows past its
+// phase's size (that would take a missed clear of the counters).  Must be
+// called by all lanes of the wave.
+SF_DEV void list_interface(const Fields& f, bool listed, bool sand, uint32_t t, uint32_t nw, uint32_t n) {
+    const int lane = (int)(threadIdx.x & 63u);
+    for (int ph = 0; ph < 2; ++ph) {
+        const bool mine = listed && sand == (ph == 1);
+        const uint64_t m = __ballot(mine);
+        if (m == 0) continue;
+        const int leader = __builtin_ctzll(m);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(f.icount + ph, (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, leader, 64);
+        const uint32_t q = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (mine && q < (ph ? n - nw : nw)) f.ilist[(ph ? nw : 0u) + q] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // P1: water density (self counted twice, :466 + loop) and pressure; sand gets
 // rho_sand and p = 0.  Only water candidates are walked (sand adds nothing);
 // branch-free: max(h^2 - r^2, 0) vanishes outside the support.  The kDenL
@@ -339,15 +393,29 @@ template <bool B, int NB>
 __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
     __shared__ float4 stage_lds[(NB / 64) * kStage];
     float4* stage = stage_lds + (threadIdx.x >> 6) * kStage;
-    // the interface counters of this step's avg-flow pass start from zero
-    if (blockIdx.x == 0 && threadIdx.x < 2) f.icount[threadIdx.x] = 0u;
     constexpr uint32_t PB = NB / kDenL;  // water particles per block
     const uint32_t nw = *f.nwater, ns = (uint32_t)n - nw;
     const PhaseBlock pb = phase_block((nw + PB - 1) / PB, (ns + NB - 1) / NB);
     if (pb.phase < 0) return;
-    if (pb.phase == 1) {  // sand: rho_sand, p = 0
+    if (pb.phase == 1) {  // sand: rho_sand, p = 0; listed if a column range holds water
         const uint32_t t = pb.blk * NB + threadIdx.x;
-        if (t < ns) f.rp[ldo(f.plist, nw + t)] = make_float2(k.rho_s, 0.f);
+        const bool valid = t < ns;
+        int i = 0;
+        uint32_t other = 0;
+        if (valid) {
+            i = (int)ldo(f.plist, nw + t);
+            f.rp[i] = make_float2(k.rho_s, 0.f);
+            const float4 p = ldo(f.pph, nw + t);
+            const uint32_t cb = scene_base<B>(k, f, i);
+            int cx, cy, cz;
+            cell_of(k, p.x, p.y, p.z, cx, cy, cz);
+            for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+                other += ldo(f.ct, cb + c1).y - ldo(f.ct, cb + c0).y;
+            });
+        }
+        const bool listed = valid && other != 0u;
+        list_interface(f, listed, true, nw + t, nw, (uint32_t)n);
+        if (valid && !listed) write_no_interface(f, i, true, make_float2(k.rho_s, 0.f));
         return;
     }
     const uint32_t t = pb.blk * PB + threadIdx.x / kDenL;
@@ -359,6 +427,7 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
     int cx, cy, cz;
     cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
     float sf = 0.f, sb = 0.f;
+    uint32_t other = 0;  // sand in the particle's 9 column ranges
     staged_walk<kDenL, true>(
         k, f, stage, water, cb, cx, cy, cz, [](float4 p) { return p; }, [](uint32_t, uint32_t) { return true; },
         [&](uint32_t, const float4& pj) {
@@ -366,7 +435,8 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
             // h^2 - r^2 as one fused chain (three FMAs, one rounding each)
             const float d = fmaxf(fmaf(-dz, dz, fmaf(-dy, dy, fmaf(-dx, dx, k.h2))), 0.f);
             sf += pj.w * sq3(d);
-        });
+        },
+        [&](uint2 a, uint2 b) { other += (b.x - a.x) - (b.y - a.y); });
     if (water) {
         const uint32_t sub = threadIdx.x % kDenL;
         for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
@@ -384,9 +454,13 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
         sf += __shfl_xor(sf, o);
         sb += __shfl_xor(sb, o);
     }
+    const bool listed = water && lead && other != 0u;
+    list_interface(f, listed, false, t, nw, (uint32_t)n);
     if (!water || !lead) return;
     const float rho = pi.w * k.w0 + k.w_coef * (sf + k.rho0 * sb);
-    f.rp[i] = make_float2(rho, eos(k, rho));
+    const float2 rpi = make_float2(rho, eos(k, rho));
+    f.rp[i] = rpi;
+    if (!listed) write_no_interface(f, i, false, rpi);
 }
 void launch_density(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
     if (n <= 0) return;
@@ -396,16 +470,8 @@ void launch_density(const SFConst& k, const Fields& f, int n, PassCfg c, hipStre
 }
 
 // ---------------------------------------------------------------------------
-// Interface passes (P2, P3).  Each particle needs only the other phase's
-// neighbours.  Most particles have none in their 9 column ranges (the cell
-// table gives each range's water and sand counts): k_interface writes the
-// empty sums' results for those and lists the rest, water and sand apart.
-// The passes then run over the lists only, on a phase-balanced launch (water
-// blocks, then sand blocks: no wave mixes the two phases' branches) with LP
-// adjacent lanes per particle, and walk just the other phase's candidates.
-// Which thread takes a listed particle does not matter: each particle's sums
-// are its own lanes' work in candidate order, so the results do not depend
-// on the list order the atomics leave.
+// k_interface: the interface lists and the empty sums' results, as the
+// density pass leaves them, for a lone avg-flow run.
 __global__ void k_zero_counts(uint32_t* __restrict__ c) {
     if (threadIdx.x < 2) c[threadIdx.x] = 0u;
 }
@@ -429,37 +495,9 @@ __global__ __launch_bounds__(kBlock) void k_interface(SFConst k, Fields f, int n
             other += sand_i ? w : (b.x - a.x) - w;
         });
     }
-    // one atomic per wave and phase (a wave holds both phases at most once,
-    // where t crosses nw); each listed lane takes its rank among the wave's
     const bool listed = valid && other != 0u;
-    const int lane = (int)(threadIdx.x & 63u);
-    for (int ph = 0; ph < 2; ++ph) {
-        const bool mine = listed && sand_i == (ph == 1);
-        const uint64_t m = __ballot(mine);
-        if (m == 0) continue;
-        const int leader = __builtin_ctzll(m);
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(f.icount + ph, (uint32_t)__popcll(m));
-        base = (uint32_t)__shfl((int)base, leader, 64);
-        if (mine) f.ilist[(ph ? nw : 0u) + base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = t;
-    }
-    if (!valid || listed) return;
-    // what P2 then P3 leave with empty sums
-    f.flow[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    f.adh[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (sand_i) {
-        f.es[i] = make_float2(0.f, 0.f);
-    } else {
-        f.es[i] = make_float2(1.f, 0.f);
-        f.xf[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        const float2 rpi = ldo(f.rp, (uint32_t)i);
-        const float rt = rpi.x / fmaxf(kEps, 1.f);
-        f.wt[i] = make_float2(rt, rpi.y / fmaxf(kEps, rt * rt));
-    }
-    if (f.den4) {
-        reinterpret_cast<float*>(f.den4)[4 * i] = 0.f;
-        reinterpret_cast<float*>(f.den4)[4 * i + 1] = 0.f;
-    }
+    list_interface(f, listed, sand_i, t, nw, (uint32_t)n);
+    if (valid && !listed) write_no_interface(f, i, sand_i, ldo(f.rp, (uint32_t)i));
 }
 void launch_zero_counts(const Fields& f, hipStream_t s) { k_zero_counts<<<1, 64, 0, s>>>(f.icount); }
 void launch_interface(const SFConst& k, const Fields& f, int n, hipStream_t s) {
@@ -561,7 +599,6 @@ __global__ __launch_bounds__(NB) void k_avg_flow(SFConst k, Fields f, int n) {
 }
 void launch_avg_flow(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
     if (n <= 0) return;
-    launch_interface(k, f, n, s);
     dispatch(f, c, [&](auto B, auto NB, auto CAP) {
         with_lanes(c.lanes, [&](auto LP) {
             k_avg_flow<decltype(B)::value, decltype(NB)::value, decltype(CAP)::value, decltype(LP)::value>
@@ -845,6 +882,9 @@ __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int 
     SF_NLIST(L);
     __shared__ float4 stage_lds[(NB / 64) * kWmStage];
     float4* stage = stage_lds + (threadIdx.x >> 6) * kWmStage;
+    // P2 and P3 have read the interface lists: the next density pass lists
+    // into counters that start at zero
+    if (blockIdx.x == 0 && threadIdx.x < 2) f.icount[threadIdx.x] = 0u;
     constexpr uint32_t PB = NB / kWmL;  // water particles per block
     const uint32_t nw = *f.nwater;
     const PhaseBlock pb = phase_block((nw + PB - 1) / PB, 0u);
