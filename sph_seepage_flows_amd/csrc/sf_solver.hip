@@ -166,13 +166,16 @@ SF_DEV T lane_value(T v, int lane) {
 // range, may skip the whole column.  Must be called by all lanes of the wave
 // (uniform control flow).
 // With L > 1 lanes per particle (adjacent lanes, same cell), lane sub of a
-// particle takes the window's slots j = first + sub, first + sub + L, ...
+// particle takes the slots j = first + sub + L m of each of its ranges
+// [first, last), in increasing j, however the ranges fall into stage chunks
+// (the same split as a per-lane walk of the ranges).
 // With Water set, the walk covers only the water candidates: phase indices
 // [wpre[s], wpre[e]) of each column range, staged from pph.  range(a, b) sees
 // each of the lane's own 9 cell-table pairs (range start, range end).
 template <int L = 1, bool Water = false, class Prep, class Need, class Fn, class Range>
 SF_DEV void staged_walk(const SFConst& k, const Fields& f, float4* __restrict__ stage, bool active, uint32_t cb,
                         int cx, int cy, int cz, Prep&& prep, Need&& need, Fn&& fn, Range&& range) {
+    static_assert((L & (L - 1)) == 0, "lanes per particle: a power of two");
     const int lane = (int)(threadIdx.x & 63u);
     const uint32_t sub = (uint32_t)lane & (uint32_t)(L - 1);
     const uint32_t col = cb + (uint32_t)((cx * k.gy + cy) * k.gz);
@@ -213,7 +216,7 @@ SF_DEV void staged_walk(const SFConst& k, const Fields& f, float4* __restrict__ 
                     wave_sync();
                     const uint32_t wlo = ax > u ? ax : u, whi = bx < u + m ? bx : u + m;
                     constexpr uint32_t S = (uint32_t)L;
-                    uint32_t j = wlo + sub;
+                    uint32_t j = wlo + ((ax + sub - wlo) & (S - 1u));  // j = ax + sub (mod L)
                     for (; j + 3u * S < whi; j += 4u * S) {
                         const float4 q0 = stage[j - u], q1 = stage[j + S - u], q2 = stage[j + 2u * S - u],
                                      q3 = stage[j + 3u * S - u];
@@ -428,15 +431,42 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
     cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
     float sf = 0.f, sb = 0.f;
     uint32_t other = 0;  // sand in the particle's 9 column ranges
-    staged_walk<kDenL, true>(
-        k, f, stage, water, cb, cx, cy, cz, [](float4 p) { return p; }, [](uint32_t, uint32_t) { return true; },
-        [&](uint32_t, const float4& pj) {
-            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            // h^2 - r^2 as one fused chain (three FMAs, one rounding each)
-            const float d = fmaxf(fmaf(-dz, dz, fmaf(-dy, dy, fmaf(-dx, dx, k.h2))), 0.f);
-            sf += pj.w * sq3(d);
-        },
-        [&](uint2 a, uint2 b) { other += (b.x - a.x) - (b.y - a.y); });
+    auto add = [&](uint32_t, const float4& pj) {
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        // h^2 - r^2 as one fused chain (three FMAs, one rounding each)
+        const float d = fmaxf(fmaf(-dz, dz, fmaf(-dy, dy, fmaf(-dx, dx, k.h2))), 0.f);
+        sf += pj.w * sq3(d);
+    };
+    auto count = [&](uint2 a, uint2 b) { other += (b.x - a.x) - (b.y - a.y); };
+    // A wave whose particles all sit in one grid column stages the water
+    // candidates of its 9 columns once; a wave over several columns would
+    // stage once per column it holds, one after another, so its lanes walk
+    // their own ranges instead.  Both give each lane the same candidates in
+    // the same order (staged_walk), hence the same sums.
+    const uint64_t act = __ballot(water);
+    const uint32_t col = cb + (uint32_t)((cx * k.gy + cy) * k.gz);
+    const bool one = !__ballot(water && col != lane_value(col, act ? __builtin_ctzll(act) : 0));
+    if (one) {
+        staged_walk<kDenL, true>(
+            k, f, stage, water, cb, cx, cy, cz, [](float4 p) { return p; }, [](uint32_t, uint32_t) { return true; },
+            add, count);
+    } else if (water) {
+        const uint32_t sub = threadIdx.x % kDenL;
+        for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+            const uint2 a = ldo(f.ct, cb + c0), b = ldo(f.ct, cb + c1);
+            count(a, b);
+            uint32_t j = a.y + sub;
+            for (; j + 3u * kDenL < b.y; j += 4u * kDenL) {
+                const float4 q0 = ldo(f.pph, j), q1 = ldo(f.pph, j + kDenL), q2 = ldo(f.pph, j + 2u * kDenL),
+                             q3 = ldo(f.pph, j + 3u * kDenL);
+                add(j, q0);
+                add(j + kDenL, q1);
+                add(j + 2u * kDenL, q2);
+                add(j + 3u * kDenL, q3);
+            }
+            for (; j < b.y; j += kDenL) add(j, ldo(f.pph, j));
+        });
+    }
     if (water) {
         const uint32_t sub = threadIdx.x % kDenL;
         for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
