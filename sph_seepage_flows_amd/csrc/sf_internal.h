@@ -172,6 +172,10 @@ struct Fields {
     // indices t): water at ilist[0, icount[0]), sand at ilist[nw, nw + icount[1])
     uint32_t* ilist;
     uint32_t* icount;
+    // iface[ts] = 1 when sand particle ts (index among the sand, plist[nw + ts])
+    // has water in its 9 column ranges (written by the density pass, read by
+    // the sand momentum pass to skip an empty water walk)
+    uint8_t* iface;
 };
 // Launch shape of a neighbour pass: workgroup size and the per-lane LDS
 // candidate-list capacity (the latter only for the passes that use one).

@@ -418,6 +418,7 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
         }
         const bool listed = valid && other != 0u;
         list_interface(f, listed, true, nw + t, nw, (uint32_t)n);
+        if (valid) f.iface[t] = listed ? 1 : 0;
         if (valid && !listed) write_no_interface(f, i, true, make_float2(k.rho_s, 0.f));
         return;
     }
@@ -839,13 +840,17 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
         }
         fsum += xyz(q.xfj) * (w * vol_i);
     };
-    for_phase_ranges(k, f, cb, cx, cy, cz, true, nw, [&](uint32_t t0, uint32_t t1) {
-        filter_push<kSmL>(L, f.pph, t0 + sub, t1, [&](uint32_t, const float4& pj) {
-            const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            return r2 < k.h2;
-        }, fetch_water, water);
-    });
+    // a grain with no water in its 9 column ranges (the density pass flagged
+    // the others) has nothing to walk
+    if (ldo(f.iface, t)) {
+        for_phase_ranges(k, f, cb, cx, cy, cz, true, nw, [&](uint32_t t0, uint32_t t1) {
+            filter_push<kSmL>(L, f.pph, t0 + sub, t1, [&](uint32_t, const float4& pj) {
+                const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                return r2 < k.h2;
+            }, fetch_water, water);
+        });
+    }
     nl_flush2(L, fetch_water, water);
     if (k.blabel1) {  // ComputeSFUniRadiusDemBoundaryForces: boundary label 1 only
         for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {

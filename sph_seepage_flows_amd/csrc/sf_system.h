@@ -45,6 +45,7 @@ struct sf_system {
     DevBuf<uint32_t> keys, kA, kB, vA, vB, hist, cell, bcell, wpre, wpart, plist;
     DevBuf<float4> pph;  // positions in phase order (Fields::pph)
     DevBuf<uint32_t> ilist, icount;  // interface particles (Fields::ilist)
+    DevBuf<uint8_t> iface;            // sand with water in its columns (Fields::iface)
     DevBuf<uint8_t> wflag;
     DevBuf<uint2> ct;
     // incremental sort (sf_set_sort_strategy): sorted-key ping-pong, permutation, work arrays
@@ -118,6 +119,7 @@ struct sf_system {
         f.pph = pph.p;
         f.ilist = ilist.p;
         f.icount = icount.p;
+        f.iface = iface.p;
         return f;
     }
     uint32_t local_cells() const { return K.C; }
