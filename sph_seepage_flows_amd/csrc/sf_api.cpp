@@ -410,7 +410,7 @@ int sf_create(const sf_params* p, const sf_boundary_params* b, const sf_solver_o
         (rc = h->xf.alloc(c)) || (rc = h->acc.alloc(c)))
         return bail(rc);
     if ((rc = h->cell.alloc((size_t)h->C + 1)) || (rc = h->bcell.alloc((size_t)h->local_cells() + 1)) ||
-        (rc = h->ct.alloc((size_t)h->C + 1)))
+        (rc = h->bnear.alloc((size_t)h->local_cells())) || (rc = h->ct.alloc((size_t)h->C + 1)))
         return bail(rc);
     if ((rc = h->cfl_part.alloc(cfl_partials(0))) || (rc = h->cfl_out.alloc(1)) ||
         (rc = h->frame_dev.alloc(1)))
@@ -425,6 +425,7 @@ int sf_create(const sf_params* p, const sf_boundary_params* b, const sf_solver_o
     }
     if ((rc = zero(h, h->rp)) || (rc = zero(h, h->es)) || (rc = zero(h, h->flow)) || (rc = zero(h, h->adh)) ||
         (rc = zero(h, h->xf)) || (rc = zero(h, h->acc)) || (rc = zero(h, h->cell)) || (rc = zero(h, h->bcell)) ||
+        (rc = zero(h, h->bnear)) ||
         (rc = zero(h, h->ct)) || (rc = zero(h, h->wt)) || (rc = zero(h, h->wpre)) || (rc = zero(h, h->plist)) ||
         (rc = zero(h, h->keys)) || (rc = zero(h, h->bpv)))
         return bail(rc);
@@ -548,6 +549,7 @@ int sf_upload_boundary(sf_handle h, size_t nb, const float* pos3, const uint64_t
     launch_cell_start(sk, (int)nb, h->local_cells(), nullptr, h->bcell.p, nullptr, nullptr, PhasePos{}, h->stream);
     launch_boundary_gather(bperm, tmp.p, h->bpv.p, (int)nb, h->stream);
     launch_boundary_volume(h->K, h->bpv.p, h->bcell.p, (int)nb, h->stream);
+    launch_boundary_near(h->K, h->bcell.p, h->bnear.p, h->stream);
     SF_HIP(hipGetLastError());
     SF_HIP(hipStreamSynchronize(h->stream));
     h->nb = nb;

@@ -115,6 +115,9 @@ void launch_gather_wprefix(const GatherArgs& a, int n, uint32_t* wpart, uint32_t
                            hipStream_t s);
 void launch_boundary_gather(const uint32_t* perm, const float4* bp_in, float4* bp_out, int nb, hipStream_t s);
 void launch_boundary_volume(const SFConst& k, float4* bpv, const uint32_t* bcs, int nb, hipStream_t s);
+// near[c] for every local cell c: whether any of its 9 neighbour column
+// ranges of the boundary cellStart bcs is non-empty.
+void launch_boundary_near(const SFConst& k, const uint32_t* bcs, uint8_t* near, hipStream_t s);
 void launch_nbr_count(const SFConst& k, const float4* pm, const uint16_t* scene, const uint32_t* cs, uint32_t* cnt,
                       uint32_t* cand, int n, hipStream_t s);
 void launch_hash_boundary(const SFConst& k, const float4* bp, uint32_t* keys, int nb, hipStream_t s);
@@ -146,6 +149,9 @@ struct Fields {
     const uint2* ct;  // (cellStart[c], water prefix at cellStart[c]), c in [0, C]
     const float4* bpv;
     const uint32_t* bcell;
+    // bnear[c] = 1 when a boundary particle lies in cell c's 27-cell
+    // neighbourhood (local cell keys; set at boundary upload)
+    const uint8_t* bnear;
     float2* rp;
     float2* es;
     float2* wt;  // water: (rho / max(eps, voidage), p / max(eps, (rho / max(eps, voidage))^2))

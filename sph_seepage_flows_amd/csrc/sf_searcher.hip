@@ -343,6 +343,23 @@ void launch_boundary_volume(const SFConst& k, float4* bpv, const uint32_t* bcs, 
     k_boundary_volume<<<(nb + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, bpv, bcs, nb);
 }
 
+// Cells whose 27-cell neighbourhood holds boundary particles: the passes'
+// boundary loops skip the other cells' 9 empty column ranges.
+__global__ __launch_bounds__(kBlock) void k_boundary_near(SFConst k, const uint32_t* __restrict__ bcs,
+                                                          uint8_t* __restrict__ near) {
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= k.C) return;
+    const int cz = (int)(c % (uint32_t)k.gz), cy = (int)((c / (uint32_t)k.gz) % (uint32_t)k.gy);
+    const int cx = (int)(c / ((uint32_t)k.gz * (uint32_t)k.gy));
+    bool any = false;
+    for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) { any = any || bcs[c1] > bcs[c0]; });
+    near[c] = any ? 1 : 0;
+}
+void launch_boundary_near(const SFConst& k, const uint32_t* bcs, uint8_t* near, hipStream_t s) {
+    if (k.C == 0) return;
+    k_boundary_near<<<(k.C + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, bcs, near);
+}
+
 // ---------------------------------------------------------------------------
 // wpre[i] = number of water particles among sorted slots [0, i), i in [0, n]:
 // lets a neighbour loop see how many water / sand particles a contiguous

@@ -468,7 +468,7 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
             for (; j < b.y; j += kDenL) add(j, ldo(f.pph, j));
         });
     }
-    if (water) {
+    if (water && ldo(f.bnear, cell_key(k, cx, cy, cz))) {  // boundary particles within the 27 cells
         const uint32_t sub = threadIdx.x % kDenL;
         for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
             const uint32_t be = ldo(f.bcell, c1);
@@ -852,7 +852,7 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
         });
     }
     nl_flush2(L, fetch_water, water);
-    if (k.blabel1) {  // ComputeSFUniRadiusDemBoundaryForces: boundary label 1 only
+    if (k.blabel1 && ldo(f.bnear, cell_key(k, cx, cy, cz))) {  // ComputeSFUniRadiusDemBoundaryForces: label 1 only
         for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
             const uint32_t be = ldo(f.bcell, c1);
             for (uint32_t j = ldo(f.bcell, c0) + sub; j < be; j += kSmL) {
@@ -983,24 +983,26 @@ __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int 
     column_filter<kWmL, kWmStage>(k, f, stage, active, cb, cx, cy, cz, L, in_range, fetch, pair);
     nl_flush2(L, fetch, pair);
     if (!active) return;
-    for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
-        const uint32_t be = ldo(f.bcell, c1);
-        for (uint32_t j = ldo(f.bcell, c0) + sub; j < be; j += kWmL) {
-            const float4 b = ldo(f.bpv, j);
-            const v3 d = mk3(pi.x - b.x, pi.y - b.y, pi.z - b.z);
-            const float r2 = dot(d, d);
-            if (r2 >= k.h2) continue;
-            const float rs = frsq(r2);
-            const float r = r2 * rs;
-            const float hr = k.h - r;
-            const float gs = r >= kEps ? k.g_coef * hr * hr * rs : 0.f;
-            const float vb = fabsf(b.w);
-            float s = vb * bpress;  // ComputeBoundaryPressure (densityi = rho0, :944)
-            const float dd = dot(vi, d);
-            if (dd < 0.f) s += -vb * k.rho0 * (bvisc * (dd * frcp(r2 + kEps)));  // ComputeBoundaryViscosity
-            a += d * (s * gs);
-        }
-    });
+    if (ldo(f.bnear, cell_key(k, cx, cy, cz))) {  // boundary particles within the 27 cells
+        for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
+            const uint32_t be = ldo(f.bcell, c1);
+            for (uint32_t j = ldo(f.bcell, c0) + sub; j < be; j += kWmL) {
+                const float4 b = ldo(f.bpv, j);
+                const v3 d = mk3(pi.x - b.x, pi.y - b.y, pi.z - b.z);
+                const float r2 = dot(d, d);
+                if (r2 >= k.h2) continue;
+                const float rs = frsq(r2);
+                const float r = r2 * rs;
+                const float hr = k.h - r;
+                const float gs = r >= kEps ? k.g_coef * hr * hr * rs : 0.f;
+                const float vb = fabsf(b.w);
+                float s = vb * bpress;  // ComputeBoundaryPressure (densityi = rho0, :944)
+                const float dd = dot(vi, d);
+                if (dd < 0.f) s += -vb * k.rho0 * (bvisc * (dd * frcp(r2 + kEps)));  // ComputeBoundaryViscosity
+                a += d * (s * gs);
+            }
+        });
+    }
 #pragma unroll
     for (int o = 1; o < kWmL; o <<= 1) {
         a.x += __shfl_xor(a.x, o);

@@ -46,6 +46,7 @@ struct sf_system {
     DevBuf<float4> pph;  // positions in phase order (Fields::pph)
     DevBuf<uint32_t> ilist, icount;  // interface particles (Fields::ilist)
     DevBuf<uint8_t> iface;            // sand with water in its columns (Fields::iface)
+    DevBuf<uint8_t> bnear;            // cells with boundary particles near (Fields::bnear)
     DevBuf<uint8_t> wflag;
     DevBuf<uint2> ct;
     // incremental sort (sf_set_sort_strategy): sorted-key ping-pong, permutation, work arrays
@@ -104,6 +105,7 @@ struct sf_system {
         f.ct = ct.p;
         f.bpv = bpv.p;
         f.bcell = bcell.p;
+        f.bnear = bnear.p;
         f.rp = rp.p;
         f.es = es.p;
         f.wt = wt.p;
