@@ -206,7 +206,9 @@ int build_neighbors(sf_system* h, hipEvent_t* ev) {
     launch_gather_wprefix(g, n, h->wpart.p, h->wpre.p, h->plist.p, s);
     SF_MARK(ev, SF_PASS_CELL_START);
     launch_cell_start(h->sorted_keys, n, h->C, h->wpre.p, h->cell.p, h->ct.p, h->in_count.p,
-                      PhasePos{h->plist.p, g.pm_out, h->pph.p}, s);
+                      PhasePos{h->plist.p, g.pm_out, h->pph.p},
+                      // the incremental sort wrote stay_pre and in_start only for n > 0
+                      inc && n > 0 ? IncStart{h->stay_pre.p, h->in_start.p} : IncStart{nullptr, nullptr}, s);
     h->cur ^= 1;
     h->cur_cs ^= 1;
     if (h->K.multi) h->cur_aux ^= 1;
@@ -546,7 +548,8 @@ int sf_upload_boundary(sf_handle h, size_t nb, const float* pos3, const uint64_t
     SortBuffers sb{h->kA.p, h->vA.p, h->kB.p, h->vB.p, h->hist.p};
     const uint32_t *sk = nullptr, *bperm = nullptr;
     if ((rc = radix_sort(h->keys.p, (int)nb, h->plan, sb, &sk, &bperm, h->stream))) return rc;
-    launch_cell_start(sk, (int)nb, h->local_cells(), nullptr, h->bcell.p, nullptr, nullptr, PhasePos{}, h->stream);
+    launch_cell_start(sk, (int)nb, h->local_cells(), nullptr, h->bcell.p, nullptr, nullptr, PhasePos{},
+                      IncStart{nullptr, nullptr}, h->stream);
     launch_boundary_gather(bperm, tmp.p, h->bpv.p, (int)nb, h->stream);
     launch_boundary_volume(h->K, h->bpv.p, h->bcell.p, (int)nb, h->stream);
     launch_boundary_near(h->K, h->bcell.p, h->bnear.p, h->stream);

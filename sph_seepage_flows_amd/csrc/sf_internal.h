@@ -95,10 +95,19 @@ struct PhasePos {
     const float4* pm;
     float4* pph;
 };
+// After an incremental sort: cellStart[c] is then the stayers before c's old
+// start plus the newcomers of the cells before c.
+struct IncStart {
+    const uint32_t* stay_pre;  // n + 1
+    const uint32_t* in_start;  // C + 1
+};
 // cellStart[c] for c in [0, C] plus the cell table ct[c] = (cellStart[c], wpre[cellStart[c]]);
-// zero_c (may be null) is cleared over [0, C) (the incremental sort's newcomer counters)
+// zero_c (may be null) is cleared over [0, C) (the incremental sort's newcomer counters).
+// With inc.stay_pre set, cell_start holds the previous sort's cellStart on
+// entry and each entry is updated in place from it; otherwise each cell's
+// start is found by binary search over sorted_keys.
 void launch_cell_start(const uint32_t* sorted_keys, int n, uint32_t C, const uint32_t* wpre, uint32_t* cell_start,
-                       uint2* ct, uint32_t* zero_c, PhasePos pp, hipStream_t s);
+                       uint2* ct, uint32_t* zero_c, PhasePos pp, IncStart inc, hipStream_t s);
 struct GatherArgs {
     const uint32_t* perm;
     const float4 *pm_in, *vr_in, *cs_in, *cda_in;

@@ -238,28 +238,35 @@ int radix_sort(const uint32_t* keys, int n, const RadixPlan& plan, const SortBuf
 __global__ __launch_bounds__(kBlock) void k_cell_start(const uint32_t* __restrict__ sk, int n, uint32_t C,
                                                        const uint32_t* __restrict__ wpre, uint32_t* __restrict__ cs,
                                                        uint2* __restrict__ ct, uint32_t* __restrict__ zero_c,
-                                                       PhasePos pp) {
+                                                       PhasePos pp, IncStart inc) {
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     // pph[t] = pm[plist[t]]: coalesced stores, loads that walk each phase's
     // slots in increasing order
     if (pp.pph && c < (uint32_t)n) pp.pph[c] = ldo(pp.pm, ldo(pp.plist, c));
     if (c > C) return;
     if (zero_c && c < C) zero_c[c] = 0u;  // the next incremental sort's newcomer counters
-    uint32_t lo = 0, hi = (uint32_t)n;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (sk[mid] < c) lo = mid + 1;
-        else hi = mid;
+    uint32_t lo = 0;
+    if (inc.stay_pre) {
+        // particles with a new key below c: the stayers (key unchanged) of the
+        // old cells below c and the newcomers of the cells below c
+        lo = inc.stay_pre[cs[c]] + inc.in_start[c];
+    } else {
+        uint32_t hi = (uint32_t)n;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (sk[mid] < c) lo = mid + 1;
+            else hi = mid;
+        }
     }
     cs[c] = lo;
     if (ct) ct[c] = make_uint2(lo, wpre ? wpre[lo] : 0u);
 }
 void launch_cell_start(const uint32_t* sorted_keys, int n, uint32_t C, const uint32_t* wpre, uint32_t* cell_start,
-                       uint2* ct, uint32_t* zero_c, PhasePos pp, hipStream_t s) {
+                       uint2* ct, uint32_t* zero_c, PhasePos pp, IncStart inc, hipStream_t s) {
     uint32_t total = C + 1;
     if (pp.pph && (uint32_t)n > total) total = (uint32_t)n;
     k_cell_start<<<(total + kBlock - 1) / kBlock, kBlock, 0, s>>>(sorted_keys, n, C, wpre, cell_start, ct, zero_c,
-                                                                  pp);
+                                                                  pp, inc);
 }
 
 // ---------------------------------------------------------------------------
