@@ -21,6 +21,7 @@ struct TapeStep {
     const uint2* ct;     // cell table of the step
     const float4* bpv;   // boundary particles (static)
     const uint32_t* bcell;
+    const uint8_t* bnear;  // cells with boundary particles in their 27-cell neighbourhood
 };
 
 // Adjoint accumulators, same order as the step's TapeStep.

@@ -115,6 +115,7 @@ template <int L, class F>
 SF_DEV void boundary_neighbours(const SFConst& k, const TapeStep& t, float4 p, int sub, F&& fn) {
     int cx, cy, cz;
     cell_of(k, p.x, p.y, p.z, cx, cy, cz);
+    if (!t.bnear[cell_key(k, cx, cy, cz)]) return;  // no boundary particle in the 27 cells
     for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
         const uint32_t e = t.bcell[c1];
         for (uint32_t j = t.bcell[c0] + (uint32_t)sub; j < e; j += (uint32_t)L) fn(j);

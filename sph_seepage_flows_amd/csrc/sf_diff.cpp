@@ -295,7 +295,7 @@ int sf_diff_backward(sf_handle h, const float* pos_bar3, const float* vel_bar3, 
         for (int q = s1 - 1; q >= s0; --q) {
             const StepTapeBufs& T = d->tape[q - s0];
             TapeStep t{T.pm.p, T.vr.p, T.rp.p, T.es.p, T.wt.p, T.flow.p, T.adh.p, T.xf.p, T.acc.p, T.den4.p,
-                       T.ct.p, h->bpv.p, h->bcell.p};
+                       T.ct.p, h->bpv.p, h->bcell.p, h->bnear.p};
             AdjFields a{d->gx[cur].p, d->gv[cur].p, d->gacc.p, d->grp.p, d->ges.p, d->gwt.p, d->gflow.p,
                         d->gadh.p, d->gxf.p, d->o4.p, d->o2.p, d->gth.p};
             launch_adj_step(h->K, t, a, q == d->steps - 1 ? d->sat_seed.p : nullptr, d->dt, n, s);
