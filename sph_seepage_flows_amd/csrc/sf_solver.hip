@@ -967,7 +967,7 @@ __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int 
             a += d * (s * gs);
         } else {  // sand j: ComputeSFWaterSeepageTerm
             const float mj = -pj.w;
-            const float rho_j = ldo(f.rp, j).x;
+            const float rho_j = k.rho_s;  // the density pass gives every grain rho_sand
             const float w = k.w_coef * sq3(k.h2 - r2);
             a += d * (-mj * p_i * frcp(fmaxf(kEps, rt_i * rho_j)) * gs);
             a += xyz(ldo(f.xf, j)) * (-w * inv_rt_i);
