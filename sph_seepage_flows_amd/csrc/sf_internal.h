@@ -191,6 +191,13 @@ struct Fields {
     // has water in its 9 column ranges (written by the density pass, read by
     // the sand momentum pass to skip an empty water walk)
     uint8_t* iface;
+    // Listed water particles (those with sand in their 9 column ranges) by
+    // water phase index t: wq[t] = (rho, p, voidage, 0) from the avg-flow
+    // pass and xq[t] = avgAdhesion from the voidage/adhesion pass.  Every
+    // water neighbour of a grain within h is listed, so the sand momentum
+    // pass reads these without the slot indirection plist[t].
+    float4* wq;
+    float4* xq;
 };
 // Launch shape of a neighbour pass: workgroup size and the per-lane LDS
 // candidate-list capacity (the latter only for the passes that use one).

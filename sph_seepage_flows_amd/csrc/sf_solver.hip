@@ -539,6 +539,7 @@ void launch_interface(const SFConst& k, const Fields& f, int n, hipStream_t s) {
 template <int NB, int LP>
 struct PhaseSlot {
     int i;         // the particle's slot
+    uint32_t t;    // its phase index (plist[t] = i)
     uint32_t sub;  // its lane among LP
     bool sand;
 };
@@ -553,7 +554,8 @@ SF_DEV bool phase_slot(const Fields& f, uint32_t nw, PhaseSlot<NB, LP>& ps) {
     const uint32_t q = pb.blk * PB + threadIdx.x / LP;
     ps.sub = threadIdx.x % LP;
     if (q >= (ps.sand ? cs : cw)) return false;
-    ps.i = (int)ldo(f.plist, ldo(f.ilist, (ps.sand ? nw : 0u) + q));
+    ps.t = ldo(f.ilist, (ps.sand ? nw : 0u) + q);
+    ps.i = (int)ldo(f.plist, ps.t);
     return true;
 }
 // enough blocks for every particle listed
@@ -626,6 +628,7 @@ __global__ __launch_bounds__(NB) void k_avg_flow(SFConst k, Fields f, int n) {
         const float2 rpi = ldo(f.rp, (uint32_t)i);
         const float rt = rpi.x / fmaxf(kEps, eps_i);
         f.wt[i] = make_float2(rt, rpi.y / fmaxf(kEps, rt * rt));
+        f.wq[ps.t] = make_float4(rpi.x, rpi.y, eps_i, 0.f);
     }
 }
 void launch_avg_flow(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
@@ -713,7 +716,9 @@ __global__ __launch_bounds__(NB) void k_voidage_adhesion(SFConst k, Fields f, in
     F = mk3(lane_sum<LP>(F.x), lane_sum<LP>(F.y), lane_sum<LP>(F.z));
     if (sub != 0) return;
     f.adh[i] = f4(F, 0.f);
-    f.xf[i] = vs != 0.f ? f4(F * (1.f / vs), 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 avg = vs != 0.f ? f4(F * (1.f / vs), 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+    f.xf[i] = avg;
+    f.xq[ps.t] = avg;
     if (f.den4) reinterpret_cast<float*>(f.den4)[4 * i + 1] = vs;
 }
 void launch_voidage_adhesion(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
@@ -819,9 +824,11 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
         float2 rpj;
         float eps_j;
     };
+    // a water neighbour within h has this grain in its columns, so it is
+    // listed and its fields are in wq / xq at its phase index u
     auto fetch_water = [&](uint32_t u) {
-        const uint32_t j = ldo(f.plist, u);
-        return WaterNbr{ldo(f.pph, u), ldo(f.xf, j), ldo(f.rp, j), ldo(f.es, j).x};
+        const float4 w = ldo(f.wq, u);
+        return WaterNbr{ldo(f.pph, u), ldo(f.xq, u), make_float2(w.x, w.y), w.z};
     };
     auto water = [&](uint32_t, const WaterNbr& q) {
         const float4 pj = q.pj;
