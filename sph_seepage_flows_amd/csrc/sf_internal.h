@@ -191,11 +191,16 @@ struct Fields {
     // has water in its 9 column ranges (written by the density pass, read by
     // the sand momentum pass to skip an empty water walk)
     uint8_t* iface;
-    // Listed water particles (those with sand in their 9 column ranges) by
-    // water phase index t: wq[t] = (rho, p, voidage, 0) from the avg-flow
-    // pass and xq[t] = avgAdhesion from the voidage/adhesion pass.  Every
-    // water neighbour of a grain within h is listed, so the sand momentum
-    // pass reads these without the slot indirection plist[t].
+    // Fields of the listed particles (those with the other phase in their 9
+    // column ranges) by phase index t, written only for those:
+    //   vq[t] = the velocity (density pass);
+    //   water: wq[t] = (rho, p) (density pass), voidage in wq[t].z (avg
+    //   flow), xq[t] = avgAdhesion (voidage/adhesion pass);
+    //   sand: saturation in wq[t].w (avg flow).
+    // Every other-phase neighbour within h of a particle is listed, so the
+    // interface and sand momentum passes read these without the slot
+    // indirection plist[t].
+    float4* vq;
     float4* wq;
     float4* xq;
 };

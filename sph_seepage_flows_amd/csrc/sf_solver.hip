@@ -419,6 +419,7 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
         const bool listed = valid && other != 0u;
         list_interface(f, listed, true, nw + t, nw, (uint32_t)n);
         if (valid) f.iface[t] = listed ? 1 : 0;
+        if (listed) f.vq[nw + t] = ldo(f.vr, (uint32_t)i);
         if (valid && !listed) write_no_interface(f, i, true, make_float2(k.rho_s, 0.f));
         return;
     }
@@ -491,7 +492,12 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
     const float rho = pi.w * k.w0 + k.w_coef * (sf + k.rho0 * sb);
     const float2 rpi = make_float2(rho, eos(k, rho));
     f.rp[i] = rpi;
-    if (!listed) write_no_interface(f, i, false, rpi);
+    if (!listed) {
+        write_no_interface(f, i, false, rpi);
+    } else {
+        f.vq[t] = ldo(f.vr, (uint32_t)i);
+        f.wq[t] = make_float4(rpi.x, rpi.y, 0.f, 0.f);  // voidage: avg flow
+    }
 }
 void launch_density(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
     if (n <= 0) return;
@@ -597,10 +603,8 @@ __global__ __launch_bounds__(NB) void k_avg_flow(SFConst k, Fields f, int n) {
         float rho;
     };
     // sand i: water neighbours (rho_j needed); water i: sand neighbours
-    auto fetch = [&](uint32_t u) {
-        const uint32_t j = ldo(f.plist, u);
-        return Nbr{ldo(f.pph, u), ldo(f.vr, j), sand_i ? ldo(f.rp, j).x : 0.f};
-    };
+    // neighbours within h are listed: their fields by phase index
+    auto fetch = [&](uint32_t u) { return Nbr{ldo(f.pph, u), ldo(f.vq, u), sand_i ? ldo(f.wq, u).x : 0.f}; };
     auto pair = [&](uint32_t, const Nbr& q) {
         const float4 pj = q.pj;
         const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -624,11 +628,12 @@ __global__ __launch_bounds__(NB) void k_avg_flow(SFConst k, Fields f, int n) {
     if (f.den4) reinterpret_cast<float*>(f.den4)[4 * i] = vsum;
     const float eps_i = 1.f - sandv;
     f.es[i] = make_float2(eps_i, sand_i ? vsum : 0.f);
+    if (sand_i) reinterpret_cast<float*>(f.wq)[4 * ps.t + 3] = vsum;  // saturation, for P3
     if (!sand_i) {
         const float2 rpi = ldo(f.rp, (uint32_t)i);
         const float rt = rpi.x / fmaxf(kEps, eps_i);
         f.wt[i] = make_float2(rt, rpi.y / fmaxf(kEps, rt * rt));
-        f.wq[ps.t] = make_float4(rpi.x, rpi.y, eps_i, 0.f);
+        reinterpret_cast<float*>(f.wq)[4 * ps.t + 2] = eps_i;
     }
 }
 void launch_avg_flow(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
@@ -665,9 +670,11 @@ __global__ __launch_bounds__(NB) void k_voidage_adhesion(SFConst k, Fields f, in
         float4 pj;
         float2 x;  // sand i: (rho_j, voidage_j) of a water j; water i: saturation of a sand j in y
     };
+    // neighbours within h are listed: water (rho, voidage) or sand saturation
+    // (in y) by phase index
     auto fetch = [&](uint32_t u) {
-        const uint32_t j = ldo(f.plist, u);
-        return Nbr{ldo(f.pph, u), ps.sand ? make_float2(ldo(f.rp, j).x, ldo(f.es, j).x) : ldo(f.es, j)};
+        const float4 w = ldo(f.wq, u);
+        return Nbr{ldo(f.pph, u), ps.sand ? make_float2(w.x, w.z) : make_float2(0.f, w.w)};
     };
     if (ps.sand) {
         float num = 0.f, den = 0.f;

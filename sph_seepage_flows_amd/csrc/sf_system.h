@@ -47,7 +47,7 @@ struct sf_system {
     DevBuf<uint32_t> ilist, icount;  // interface particles (Fields::ilist)
     DevBuf<uint8_t> iface;            // sand with water in its columns (Fields::iface)
     DevBuf<uint8_t> bnear;            // cells with boundary particles near (Fields::bnear)
-    DevBuf<float4> wq, xq;            // listed water's fields by phase index (Fields::wq)
+    DevBuf<float4> vq, wq, xq;        // listed particles' fields by phase index (Fields::wq)
     DevBuf<uint8_t> wflag;
     DevBuf<uint2> ct;
     // incremental sort (sf_set_sort_strategy): sorted-key ping-pong, permutation, work arrays
@@ -123,6 +123,7 @@ struct sf_system {
         f.ilist = ilist.p;
         f.icount = icount.p;
         f.iface = iface.p;
+        f.vq = vq.p;
         f.wq = wq.p;
         f.xq = xq.p;
         return f;
