@@ -576,7 +576,8 @@ inline int phase_slot_grid(int n, int lanes, int nb) { return phase_grid(n * lan
 // Water particles also store wt = (rho / max(eps, voidage), p / max(eps, that^2)),
 // the per-particle factors the water momentum pass needs from every neighbour.
 // Candidates are filtered four at a time into the lane's list; the flush
-// fetches each kept neighbour's fields (slot j = plist[u]) ahead of its use.
+// fetches each kept neighbour's fields (by phase index u: every neighbour
+// within h is listed, see Fields::vq) ahead of its use.
 template <bool B, int NB, int CAP, int LP>
 __global__ __launch_bounds__(NB) void k_avg_flow(SFConst k, Fields f, int n) {
     SF_NLIST(L);
