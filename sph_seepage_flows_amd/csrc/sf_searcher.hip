@@ -533,8 +533,10 @@ void launch_gather_wprefix(const GatherArgs& a, int n, uint32_t* wpart, uint32_t
 //   newStart(c) + #(stayers of c before it) + #(newcomers of c before it),
 // where stayers of c are the particles of c's old block whose key is still c
 // (a prefix over "key unchanged" flags counts them) and newcomers are the
-// few particles whose key changed to c (collected per cell and ordered by
-// index).  newStart(c) = stayers in cells < c + newcomers in cells < c.
+// few particles whose key changed to c (collected per cell, in the order the
+// atomics leave; a newcomer's rank is the count of smaller indices in its
+// cell's list, which that order does not change).
+// newStart(c) = stayers in cells < c + newcomers in cells < c.
 // The result is the stable sort's, element for element.
 __global__ __launch_bounds__(kBlock) void k_inc_flag(const uint32_t* __restrict__ key, const uint32_t* __restrict__ old,
                                                      int n, uint32_t* __restrict__ in_count,
@@ -551,22 +553,6 @@ __global__ __launch_bounds__(kBlock) void k_inc_fill(const uint32_t* __restrict_
     if (i >= n) return;
     const uint32_t c = key[i];
     if (c != old[i]) in_list[in_start[c] + slot[i]] = (uint32_t)i;
-}
-// newcomers of each cell in increasing index (insertion sort; lists are short)
-__global__ __launch_bounds__(kBlock) void k_inc_order(const uint32_t* __restrict__ in_start, uint32_t C,
-                                                      uint32_t* __restrict__ in_list) {
-    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= C) return;
-    const uint32_t s = in_start[c], e = in_start[c + 1];
-    for (uint32_t a = s + 1; a < e; ++a) {
-        const uint32_t v = in_list[a];
-        uint32_t b = a;
-        while (b > s && in_list[b - 1] > v) {
-            in_list[b] = in_list[b - 1];
-            --b;
-        }
-        in_list[b] = v;
-    }
 }
 __global__ __launch_bounds__(kBlock) void k_inc_place(const uint32_t* __restrict__ key, const uint32_t* __restrict__ old,
                                                       int n, const uint32_t* __restrict__ old_start,
@@ -603,7 +589,6 @@ void launch_incremental_sort(const uint32_t* key, const uint32_t* old_key, const
     if (!flags_done) k_inc_flag<<<gn, kBlock, 0, s>>>(key, old_key, n, b.in_count, b.slot);
     launch_vprefix2(StayVal{key, old_key}, n, b.stay_pre, ArrayVal{b.in_count}, (int)C, b.in_start, b.part, s);
     k_inc_fill<<<gn, kBlock, 0, s>>>(key, old_key, n, b.in_start, b.slot, b.in_list);
-    k_inc_order<<<(int)((C + kBlock - 1) / kBlock), kBlock, 0, s>>>(b.in_start, C, b.in_list);
     k_inc_place<<<gn, kBlock, 0, s>>>(key, old_key, n, old_start, b.stay_pre, b.in_start, b.in_list, perm,
                                        sorted_keys);
 }
