@@ -505,15 +505,46 @@ struct FlagVal {
     }
 };
 
+constexpr int kDirectTiles = 4096;  // tiles up to which each block sums its own tile prefix
 // Water prefix of the gathered slots (k_gather left the tile counts in
 // wpart) and the slots grouped by phase (water first, each group in slot
 // order) so that a pass over one phase can give every XCD an equal
 // contiguous share of it.
+// With scanned = 0, wpart holds the raw per-tile counts and each block sums
+// the counts before its tile, and all of them, itself (a few thousand tiles
+// at most), instead of waiting for a one-block scan launch.
 __global__ __launch_bounds__(kBlock) void k_wprefix_plist(const uint8_t* __restrict__ wflag, int n,
-                                                          const uint32_t* __restrict__ wpart, uint32_t* __restrict__ wpre,
-                                                          uint32_t* __restrict__ plist) {
-    const uint32_t nwater = wpart[gridDim.x];
-    tile_prefix(FlagVal{wflag}, n, (int)blockIdx.x, wpart[blockIdx.x], wpre, [&](int i, uint32_t pre, uint32_t water) {
+                                                          const uint32_t* __restrict__ wpart, int scanned,
+                                                          uint32_t* __restrict__ wpre, uint32_t* __restrict__ plist) {
+    uint32_t nwater, part;
+    if (scanned) {
+        nwater = wpart[gridDim.x];
+        part = wpart[blockIdx.x];
+    } else {
+        __shared__ uint32_t red[2][kBlock / 64];
+        uint32_t all = 0, before = 0;
+        for (uint32_t q = threadIdx.x; q < gridDim.x; q += kBlock) {
+            const uint32_t c = wpart[q];
+            all += c;
+            before += q < blockIdx.x ? c : 0u;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            all += __shfl_down(all, o, 64);
+            before += __shfl_down(before, o, 64);
+        }
+        if ((threadIdx.x & 63u) == 0) {
+            red[0][threadIdx.x >> 6] = all;
+            red[1][threadIdx.x >> 6] = before;
+        }
+        __syncthreads();
+        nwater = 0;
+        part = 0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            nwater += red[0][w];
+            part += red[1][w];
+        }
+    }
+    tile_prefix(FlagVal{wflag}, n, (int)blockIdx.x, part, wpre, [&](int i, uint32_t pre, uint32_t water) {
         plist[water ? pre : nwater + (uint32_t)i - pre] = (uint32_t)i;
     });
 }
@@ -521,8 +552,9 @@ void launch_gather_wprefix(const GatherArgs& a, int n, uint32_t* wpart, uint32_t
                            hipStream_t s) {
     const int nb = wprefix_blocks(n + 1);
     k_gather<<<nb, kBlock, 0, s>>>(a, n, wpart);
-    k_rs_scan2<<<1, 1024, 0, s>>>(wpart, nb, 0, wpart + nb);
-    k_wprefix_plist<<<nb, kBlock, 0, s>>>(a.wflag, n, wpart, wpre, plist);
+    const int scanned = nb > kDirectTiles ? 1 : 0;
+    if (scanned) k_rs_scan2<<<1, 1024, 0, s>>>(wpart, nb, 0, wpart + nb);
+    k_wprefix_plist<<<nb, kBlock, 0, s>>>(a.wflag, n, wpart, scanned, wpre, plist);
 }
 
 // ---------------------------------------------------------------------------
