@@ -400,6 +400,36 @@ SF_DEV void tile_sum(V v, int n, int tile, uint32_t* __restrict__ part_out) {
 // tile's slot prefixes from its scanned partial (thread t owns kWpItems
 // consecutive slots); slot n (one past the end) receives the total.
 // emit(i, prefix, value) is called for every slot i < n.
+// Up to kDirectTiles tiles, each prefix block sums the tile counts it needs
+// itself instead of waiting for a one-block scan launch: the sum of cnt[q]
+// over q in [lo, b) (before) and over [lo, hi) (all), by the whole block.
+constexpr int kDirectTiles = 4096;
+SF_DEV void block_sums(const uint32_t* __restrict__ cnt, uint32_t lo, uint32_t b, uint32_t hi, uint32_t& before,
+                       uint32_t& all) {
+    __shared__ uint32_t red[2][kBlock / 64];
+    uint32_t a = 0, p = 0;
+    for (uint32_t q = lo + threadIdx.x; q < hi; q += kBlock) {
+        const uint32_t c = cnt[q];
+        a += c;
+        p += q < b ? c : 0u;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_down(a, o, 64);
+        p += __shfl_down(p, o, 64);
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        red[0][threadIdx.x >> 6] = a;
+        red[1][threadIdx.x >> 6] = p;
+    }
+    __syncthreads();
+    all = 0;
+    before = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        all += red[0][w];
+        before += red[1][w];
+    }
+}
+
 template <class V, class Emit>
 SF_DEV void tile_prefix(V v, int n, int tile, uint32_t part, uint32_t* __restrict__ out, Emit&& emit) {
     __shared__ uint32_t wsum[kBlock / 64];
@@ -447,22 +477,34 @@ __global__ __launch_bounds__(kBlock) void k_vsum2(V1 v1, int n1, int nb1, V2 v2,
     else tile_sum(v2, n2, b - nb1, part + b);
 }
 template <class V1, class V2>
+// scanned = 0: part holds the raw block sums, and each block sums those
+// before it within its own prefix (the block holding slot n thereby ends on
+// the total, as the scan would give it).
 __global__ __launch_bounds__(kBlock) void k_vprefix2(V1 v1, int n1, int nb1, uint32_t* __restrict__ out1, V2 v2,
                                                      int n2, uint32_t* __restrict__ out2,
-                                                     const uint32_t* __restrict__ part) {
+                                                     const uint32_t* __restrict__ part, int scanned) {
     const int b = blockIdx.x;
-    if (b < nb1) tile_prefix(v1, n1, b, part[b], out1, NoEmit{});
-    else tile_prefix(v2, n2, b - nb1, part[b], out2, NoEmit{});
+    uint32_t pb = 0;
+    if (scanned) {
+        pb = part[b];
+    } else {
+        uint32_t all;
+        if (b < nb1) block_sums(part, 0u, (uint32_t)b, (uint32_t)nb1, pb, all);
+        else block_sums(part, (uint32_t)nb1, (uint32_t)b, gridDim.x, pb, all);
+    }
+    if (b < nb1) tile_prefix(v1, n1, b, pb, out1, NoEmit{});
+    else tile_prefix(v2, n2, b - nb1, pb, out2, NoEmit{});
 }
 int vprefix2_partials(int n1, int n2) { return wprefix_blocks(n1 + 1) + wprefix_blocks(n2 + 1); }
 template <class V1, class V2>
 void launch_vprefix2(V1 v1, int n1, uint32_t* out1, V2 v2, int n2, uint32_t* out2, uint32_t* part, hipStream_t s) {
     // blocks cover slots [0, n]; the block holding slot n counts nothing and
-    // receives the total from the scan
+    // receives the total from the prefix of the blocks before it
     const int nb1 = wprefix_blocks(n1 + 1), nb2 = wprefix_blocks(n2 + 1);
     k_vsum2<<<nb1 + nb2, kBlock, 0, s>>>(v1, n1, nb1, v2, n2, part);
-    k_rs_scan2<<<1, 1024, 0, s>>>(part, nb1, nb2, nullptr);
-    k_vprefix2<<<nb1 + nb2, kBlock, 0, s>>>(v1, n1, nb1, out1, v2, n2, out2, part);
+    const int scanned = nb1 + nb2 > kDirectTiles ? 1 : 0;
+    if (scanned) k_rs_scan2<<<1, 1024, 0, s>>>(part, nb1, nb2, nullptr);
+    k_vprefix2<<<nb1 + nb2, kBlock, 0, s>>>(v1, n1, nb1, out1, v2, n2, out2, part, scanned);
 }
 // Per-slot values of the prefixes; load8(i, f) reads slots i .. i + 7 (i a
 // multiple of 8, all below n) with vector loads.
@@ -505,7 +547,6 @@ struct FlagVal {
     }
 };
 
-constexpr int kDirectTiles = 4096;  // tiles up to which each block sums its own tile prefix
 // Water prefix of the gathered slots (k_gather left the tile counts in
 // wpart) and the slots grouped by phase (water first, each group in slot
 // order) so that a pass over one phase can give every XCD an equal
@@ -521,28 +562,7 @@ __global__ __launch_bounds__(kBlock) void k_wprefix_plist(const uint8_t* __restr
         nwater = wpart[gridDim.x];
         part = wpart[blockIdx.x];
     } else {
-        __shared__ uint32_t red[2][kBlock / 64];
-        uint32_t all = 0, before = 0;
-        for (uint32_t q = threadIdx.x; q < gridDim.x; q += kBlock) {
-            const uint32_t c = wpart[q];
-            all += c;
-            before += q < blockIdx.x ? c : 0u;
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            all += __shfl_down(all, o, 64);
-            before += __shfl_down(before, o, 64);
-        }
-        if ((threadIdx.x & 63u) == 0) {
-            red[0][threadIdx.x >> 6] = all;
-            red[1][threadIdx.x >> 6] = before;
-        }
-        __syncthreads();
-        nwater = 0;
-        part = 0;
-        for (int w = 0; w < kBlock / 64; ++w) {
-            nwater += red[0][w];
-            part += red[1][w];
-        }
+        block_sums(wpart, 0u, blockIdx.x, gridDim.x, part, nwater);
     }
     tile_prefix(FlagVal{wflag}, n, (int)blockIdx.x, part, wpre, [&](int i, uint32_t pre, uint32_t water) {
         plist[water ? pre : nwater + (uint32_t)i - pre] = (uint32_t)i;
