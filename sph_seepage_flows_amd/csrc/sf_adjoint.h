@@ -60,4 +60,19 @@ void launch_gather_by_id1(const uint32_t* id, const float* src, float* dst, int 
 // vr[k].xyz = vel3[id[k]]
 void launch_set_vel_by_id(const uint32_t* id, const float* vel3, float4* vr, int n, hipStream_t s);
 
+
+// Up to four device-to-device copies in one launch (the taped step's
+// copies, which as separate copies cost a launch each).  Sizes are in bytes,
+// multiples of 4; pointers 16-byte aligned.
+struct CopyJob {
+    const void* src;
+    void* dst;
+    size_t bytes;
+};
+struct CopyJobs {
+    CopyJob j[4];
+    int count;
+};
+void launch_copy_jobs(const CopyJobs& c, hipStream_t s);
+
 }  // namespace sf

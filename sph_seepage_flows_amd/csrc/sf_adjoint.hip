@@ -800,4 +800,27 @@ void launch_set_vel_by_id(const uint32_t* id, const float* vel3, float4* vr, int
     if (n > 0) k_set_vel_by_id<<<blocks(n), kAB, 0, s>>>(id, vel3, vr, n);
 }
 
+
+__global__ __launch_bounds__(256) void k_copy_jobs(CopyJobs c) {
+    const uint32_t stride = gridDim.x * 256u, t0 = blockIdx.x * 256u + threadIdx.x;
+    for (int q = 0; q < c.count; ++q) {
+        const uint32_t n16 = (uint32_t)(c.j[q].bytes / 16u), n4 = (uint32_t)(c.j[q].bytes / 4u);
+        const uint4* s16 = static_cast<const uint4*>(c.j[q].src);
+        uint4* d16 = static_cast<uint4*>(c.j[q].dst);
+        for (uint32_t w = t0; w < n16; w += stride) d16[w] = s16[w];
+        const uint32_t* s4 = static_cast<const uint32_t*>(c.j[q].src);
+        uint32_t* d4 = static_cast<uint32_t*>(c.j[q].dst);
+        for (uint32_t w = 4u * n16 + t0; w < n4; w += stride) d4[w] = s4[w];
+    }
+}
+void launch_copy_jobs(const CopyJobs& c, hipStream_t s) {
+    size_t most = 0;
+    for (int q = 0; q < c.count; ++q) most = c.j[q].bytes > most ? c.j[q].bytes : most;
+    if (most == 0) return;
+    size_t g = (most / 16u + 255u) / 256u;
+    if (g < 1) g = 1;
+    if (g > 1024) g = 1024;
+    k_copy_jobs<<<(int)g, 256, 0, s>>>(c);
+}
+
 }  // namespace sf

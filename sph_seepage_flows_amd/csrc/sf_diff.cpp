@@ -94,11 +94,12 @@ int diff_step(sf_system* h, float dt, StepTapeBufs* tape) {
         return advect(h, dt, nullptr);
     }
     StepTapeBufs& T = *tape;
-    if ((rc = copy(T.pm.p, h->pm[h->cur].p, n * sizeof(float4), s)) ||
-        (rc = copy(T.vr.p, h->vr[h->cur].p, n * sizeof(float4), s)) ||
-        (rc = copy(T.perm.p, h->perm, n * sizeof(uint32_t), s)) ||
-        (rc = copy(T.ct.p, h->ct.p, ((size_t)h->C + 1) * sizeof(uint2), s)))
-        return rc;
+    CopyJobs before{{{h->pm[h->cur].p, T.pm.p, n * sizeof(float4)},
+                     {h->vr[h->cur].p, T.vr.p, n * sizeof(float4)},
+                     {h->perm, T.perm.p, n * sizeof(uint32_t)},
+                     {h->ct.p, T.ct.p, ((size_t)h->C + 1) * sizeof(uint2)}},
+                    4};
+    launch_copy_jobs(before, s);
     Fields f = h->fields();
     f.rp = T.rp.p;
     f.es = T.es.p;
@@ -108,10 +109,14 @@ int diff_step(sf_system* h, float dt, StepTapeBufs* tape) {
     f.xf = T.xf.p;
     f.den4 = T.den4.p;
     if ((rc = solve_with(h, f))) return rc;
-    if ((rc = copy(T.acc.p, h->acc.p, n * sizeof(float4), s))) return rc;
     // the handle's own transient fields keep the last step's values for downloads
-    if ((rc = copy(h->rp.p, T.rp.p, n * sizeof(float2), s)) || (rc = copy(h->es.p, T.es.p, n * sizeof(float2), s)))
-        return rc;
+    CopyJobs after{{{h->acc.p, T.acc.p, n * sizeof(float4)},
+                    {T.rp.p, h->rp.p, n * sizeof(float2)},
+                    {T.es.p, h->es.p, n * sizeof(float2)},
+                    {nullptr, nullptr, 0}},
+                   3};
+    launch_copy_jobs(after, s);
+    SF_HIP(hipGetLastError());
     return advect(h, dt, nullptr);
 }
 
