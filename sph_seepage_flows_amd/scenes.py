@@ -181,7 +181,11 @@ def mixed(seed: int = 7, sf_type: int = 0, n_water: int = 1440, n_sand: int = 11
     if sf_type == SF_TYPE_MULTI_SF:
         u = uniform(seed, 11, sc.n, 1)[:, 0]
         sc.cda0asat[:, 0] = np.float32(0.2) + u  # per-particle drag coefficient
-    # nonzero saturation history so the colour / maxSat branch is exercised both ways
+    # nonzero saturation history U(0, 0.5) on the grains, so the colour / maxSat
+    # update runs both ways (grains wetter and drier than their history)
+    ms = np.zeros(sc.n, np.float32)
+    ms[n_water:] = uniform(seed, 12, n_sand, 1)[:, 0] * np.float32(0.5)
+    sc.max_sat = ms
     return sc
 
 

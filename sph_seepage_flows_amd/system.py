@@ -268,6 +268,9 @@ class BatchSystem(SFSystem):
         mass, rad, cda, amc = cat("mass"), cat("radius"), cat("cda0asat"), cat("amcamcp")
         check(lib().sf_upload_particles(s._h, len(pos), ptr(pos), ptr(vel), ptr(col), ptr(lab), ptr(mass),
                                         ptr(rad), ptr(cda), ptr(amc)))
+        if any(getattr(sc, "max_sat", None) is not None for sc in scenes):
+            s.set_field("max_saturation", np.concatenate(
+                [sc.max_sat if sc.max_sat is not None else np.zeros(sc.n, np.float32) for sc in scenes]))
         if scenes[0].bpos is not None:
             s.upload_boundary(scenes[0].bpos, scenes[0].blabel)
         return s
