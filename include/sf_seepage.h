@@ -24,7 +24,15 @@
  *   sf_update_frame      <- the frame loop of Update()  seepage_flows/src/seepageflow/main.cpp:235-259
  *   sf_download / sf_device_buffer <- CudaSFParticles::Get*Ptr()  cuda_sf_particles.cuh:170-190,
  *                           CudaGNBaseSearcher::GetCellStart()  cuda_neighbor_searcher.cuh:44-50
+ *   sf_particle_view     <- the same getters in the reference's array layout: CudaSFParticles::Get*Ptr()
+ *                           (cuda_sf_particles.cuh:170-190, CudaParticles::GetPosPtr cuda_particles.cuh:57)
+ *                           and GetCellStart() (cuda_neighbor_searcher.cuh:44-50)
+ *   sf_update_solver     <- CudaWCSphSFSolver::UpdateSolver  cuda_wcsph_sf_solver.cuh:36-43,
+ *                           cuda_wcsph_sf_solver.cpp:34-179 (and CudaSphSFSolver::UpdateSolver,
+ *                           cuda_sph_sf_solver.cpp:14-152) on the current neighbour build
  *   sf_export_bgeo       <- ExportBgeoFileCUDA  seepage_flows/include/kiri_utils.h:364-448
+ *   sf_export_bgeo_device <- ExportBgeoFileCUDA(folder, name, pos, col, radius, label, n)
+ *                           seepage_flows/include/kiri_utils.h:364-371 (device arrays in)
  *   sf_write_bgeo        <- Partio::writeBGEO (uncompressed V5)  extlib/include/partio/io/BGEO.cpp:311-476
  *   sf_build_world_boundary <- CudaBoundaryEmitter::BuildWorldBoundary  cuda_boundary_emitter.cpp:12-80
  *   sf_read_bgeo         <- ReadBgeoFileForGPU  seepage_flows/include/kiri_utils.h:252-308
@@ -33,6 +41,7 @@
  *                        <- CudaVolumeEmitter::BuildSeepageflowShape(Multi)Volume  cuda_volume_emitter.cpp:109-175
  *   sf_bgeo_to_vtk       <- Partio2VTK  seepage_flows/include/kiri_utils.h:54-250
  *   sf_grid_size         <- make_int3((highest - lowest) / kernel_radius)  main.cpp:104, cuda_neighbor_searcher.cu:26
+ *   sf_grid_build        <- CudaGNBaseSearcher::BuildGNSearcher  cuda_neighbor_searcher.cu:36-51 on caller arrays
  *   sf_set_emitter / sf_get_emitter <- CudaEmitter + CudaSphEmitterParams (see their declarations)
  *   sf_batch_*, sf_step_graph, sf_*_launch_config, sf_autotune, sf_set_field, sf_diff_*,
  *   sf_set_sort_strategy, sf_set_interface_lanes, sf_step_timed:
@@ -251,6 +260,12 @@ int sf_build_neighbors(sf_handle h);
 int sf_step(sf_handle h, float dt);
 int sf_step_n(sf_handle h, float dt, int n_steps);
 int sf_update_system(sf_handle h, float render_interval, float* ms_out);
+/* The solver part of sf_update_system on the neighbour structures the last
+ * sf_build_neighbors built: the passes, the sub-step count (CFL for the WCSPH
+ * solver) and the advect, without the build and without emission (the
+ * reference emits in CudaSFSystem::OnUpdateSolver, after UpdateSolver).
+ * SF_E_STATE unless sf_build_neighbors ran since the last step. */
+int sf_update_solver(sf_handle h, float render_interval, float* ms_out);
 int sf_num_sub_steps(sf_handle h, size_t* out);
 /* One frame of the reference driver's loop, with the results of calling
  * sf_update_system from the host the way main.cpp:235-259 does:
@@ -303,8 +318,49 @@ int sf_set_field(sf_handle h, int field, const void* src, size_t src_bytes);
  * writes through it between steps are seen (otherwise the step's advect
  * already computed the next build's cell keys). */
 int sf_device_buffer(sf_handle h, int field, void** dptr, size_t* stride_bytes);
+/* A device array of the field in the reference's layout, as
+ * CudaSFParticles::Get*Ptr() returns it: packed float3 (12-byte elements) for
+ * SF_FIELD_POS, VEL, ACC, COL, AVG_FLOW_VEL, AVG_DRAG_FORCE, ADHESION_FORCE,
+ * AVG_ADHESION_FORCE, CDA0ASAT; float for MASS, RADIUS, DENSITY, PRESSURE,
+ * VOIDAGE, SATURATION, MAX_SATURATION; uint64 (size_t) for LABEL; float2 for
+ * AMCAMCP; uint64 for SORTED_KEYS (GetGridIdxArray(), the keys after the
+ * last sort), CELL_START and B_CELL_START (numCells + 1 entries, as
+ * GetCellStart()).  The array has room for max_num particles, belongs to the
+ * handle and keeps its address until sf_destroy.  The handle keeps every view
+ * it handed out current: each call that changes the state (upload, set_field,
+ * build, step, update, frame, differentiable window) refreshes it before
+ * returning, in the current slot order (the particles are re-sorted by cell
+ * every step, as the reference's arrays are).  Values written into a view of
+ * a persistent field (POS, VEL, COL, LABEL, MASS, RADIUS, MAX_SATURATION,
+ * CDA0ASAT, AMCAMCP) between calls are taken into the state at the start of
+ * the next build or step; a label other than 0 reads as sand, masses must
+ * stay > 0.  Writes into the other views are overwritten by the next step,
+ * which recomputes those fields (in the reference too).  sf_download and
+ * sf_export_bgeo read the state, not the views.  Handing out the position,
+ * mass or label view makes every later build hash the positions again. */
+int sf_particle_view(sf_handle h, int field, void** dptr);
 /* Writes folder/name.bgeo from the current state (creates folder). */
 int sf_export_bgeo(sf_handle h, const char* folder, const char* name);
+/* Writes folder/name.bgeo (creates folder) from n particles in device memory
+ * of the current device, in the reference's layout: packed float3 positions
+ * and colours, float radii, uint64 labels (sf_particle_view arrays). */
+int sf_export_bgeo_device(const char* folder, const char* name, const float* pos3, const float* col3,
+                          const float* radius, const uint64_t* label, size_t n);
+
+/* ---- the searcher on caller arrays ----------------------------------------
+ * CudaGNBaseSearcher::BuildGNSearcher (cuda_neighbor_searcher.cu:36-51) for
+ * any particle set the caller holds on the current device: pos3 = n packed
+ * float3 positions (device memory).  Writes sorted_keys[n] (the cell keys in
+ * sorted order, GetGridIdxArray()), perm[n] (the stable sort's permutation,
+ * perm[new] = old; the reference permutes the particle arrays by it) and
+ * cell_start[numCells + 1] (GetCellStart(); [k + 1] is cellEnd[k]), with
+ * numCells = grid_size.x * y * z, keys and cells exactly as the handle's
+ * searcher computes them.  The positions are not moved.  Work is ordered on
+ * `stream` (a hipStream_t; NULL = the null stream) and finished on return;
+ * the call allocates and frees its own work arrays (about 8 bytes per
+ * particle).  n < 2^27, numCells < 2^31. */
+int sf_grid_build(const float* pos3, size_t n, sf_float3 lowest, float cell_size, sf_int3 grid_size,
+                  uint32_t* sorted_keys, uint32_t* perm, uint32_t* cell_start, void* stream);
 
 /* ---- inflow emitter ------------------------------------------------------
  * sf_emitter_params mirrors CudaSphEmitterParams (cuda_sph_params.h:49-70);

@@ -19,7 +19,7 @@ BUILD = PKG / "_build"
 LIB = LIBDIR / "libsf_seepage.so"
 INCLUDE = PKG.parent / "include"
 
-SOURCES = ["sf_searcher.hip", "sf_solver.hip", "sf_adjoint.hip", "sf_api.cpp", "sf_diff.cpp", "sf_emitter.cpp",
+SOURCES = ["sf_searcher.hip", "sf_solver.hip", "sf_adjoint.hip", "sf_views.hip", "sf_grid.hip", "sf_api.cpp", "sf_diff.cpp", "sf_emitter.cpp",
            "sf_host.cpp", "sf_scene.cpp"]
 ARCH = os.environ.get("SF_OFFLOAD_ARCH", "gfx950")
 

@@ -17,11 +17,23 @@
 //     (.../solver/seepageflow/cuda_wcsph_sf_solver.cuh:19-68)
 //   CudaSFSystem(...), UpdateSystem(renderInterval), GetNumOfSubTimeSteps()
 //     (.../system/cuda_sf_system.cuh:15-50, cuda_base_system.cuh:17-44)
-//   ExportBgeoFileCUDA(folder, name, ...)             (seepage_flows/include/kiri_utils.h:364-448)
+//   CudaGNBaseSearcher::BuildGNSearcher / GetCellStart / GetGridIdxArray
+//     -> sf_build_neighbors, sf_particle_view (.../searcher/cuda_neighbor_searcher.cuh:29-50)
+//   CudaWCSphSFSolver::UpdateSolver(particles, boundaries, cellStart, bCellStart,
+//     renderInterval, params, bparams) -> sf_update_solver (.../solver/seepageflow/cuda_wcsph_sf_solver.cuh:36-43)
+//   CudaSFParticles::Get*Ptr()                        -> sf_particle_view
+//     (.../particle/cuda_sf_particles.cuh:170-190, cuda_particles.cuh:57): device arrays
+//     in the reference's layout (float3 / float / size_t / float2), kept current by
+//     the library after every step; writes into persistent fields are taken in at
+//     the next step
+//   ExportBgeoFileCUDA(folder, name, pos, col, radius, label, n)
+//     -> sf_export_bgeo_device (seepage_flows/include/kiri_utils.h:364-448)
 //
-// Differences a caller sees: Get*Ptr() return device pointers to this
-// library's float4 storage (stride 16 bytes, see sf_device_buffer); the bgeo
-// export takes the particle object instead of four device pointers.
+// Differences a caller sees: a particle object has device arrays only once a
+// CudaSFSystem holds it (the reference copies to the device in the particle
+// constructor; nothing in the reference's driver reads them before); the
+// searcher and solver classes act on the system's handle, which owns the grid
+// and the boundary particles.
 //   CudaEmitter / CUDA_SPH_EMITTER_PARAMS             -> sf_set_emitter
 //     (emitter/cuda_emitter.cpp:12-97, system/cuda_sf_system.cu:36-58,86-115)
 //   ReadBgeoFileForGPU / ReadMultiBgeoFilesForGPU     -> sf_read_bgeo
@@ -253,6 +265,22 @@ typedef SharedPtr<CudaVolumeEmitter> CudaVolumeEmitterPtr;
 
 class CudaSFSystem;
 
+// Device array handed out by the library (the reference's owning CudaArray,
+// cuda_array.cuh:19-61, offers the same Data() / Length()).  The library owns
+// the memory; it stays valid while the system lives.
+template <class T>
+class CudaArray {
+  public:
+    CudaArray() = default;
+    CudaArray(T* p, size_t len) : mData(p), mLen(len) {}
+    T* Data() const { return mData; }
+    size_t Length() const { return mLen; }
+
+  private:
+    T* mData = nullptr;
+    size_t mLen = 0;
+};
+
 class CudaSFParticles {
   public:
     CudaSFParticles(size_t maxNum, const Vec_Float3& p, const Vec_Float3& col, const Vec_SizeT& label,
@@ -270,13 +298,30 @@ class CudaSFParticles {
         return n;
     }
     size_t MaxSize() const { return mMax; }
-    // device pointers into the library's storage (float4 elements, stride 16)
-    float4* GetPosPtr() const { return (float4*)ptr(SF_FIELD_POS); }
-    float4* GetVelPtr() const { return (float4*)ptr(SF_FIELD_VEL); }
-    float4* GetAccPtr() const { return (float4*)ptr(SF_FIELD_ACC); }
-    float4* GetColPtr() const { return (float4*)ptr(SF_FIELD_COL); }
-    float2* GetDensityPressurePtr() const { return (float2*)ptr(SF_FIELD_DENSITY); }
-    float2* GetVoidageSaturationPtr() const { return (float2*)ptr(SF_FIELD_VOIDAGE); }
+    // CudaParticles / CudaSFParticles getters (cuda_particles.cuh:57,
+    // cuda_sf_particles.cuh:170-190): device arrays in the reference's layout
+    // in the current (cell-sorted) order, null until a CudaSFSystem holds the
+    // particles.  The library refreshes them after every step and takes what
+    // the caller writes into pos, vel, col, label, mass, radius,
+    // maxSaturation, cdA0Asat and amcAmcp in at the start of the next step.
+    float3* GetPosPtr() const { return (float3*)view(SF_FIELD_POS); }
+    float3* GetVelPtr() const { return (float3*)view(SF_FIELD_VEL); }
+    float3* GetAccPtr() const { return (float3*)view(SF_FIELD_ACC); }
+    float3* GetColPtr() const { return (float3*)view(SF_FIELD_COL); }
+    size_t* GetLabelPtr() const { return (size_t*)view(SF_FIELD_LABEL); }
+    float* GetRadiusPtr() const { return (float*)view(SF_FIELD_RADIUS); }
+    float* GetPressurePtr() const { return (float*)view(SF_FIELD_PRESSURE); }
+    float* GetDensityPtr() const { return (float*)view(SF_FIELD_DENSITY); }
+    float* GetMassPtr() const { return (float*)view(SF_FIELD_MASS); }
+    float* GetVoidagePtr() const { return (float*)view(SF_FIELD_VOIDAGE); }
+    float* GetSaturationPtr() const { return (float*)view(SF_FIELD_SATURATION); }
+    float* GetMaxSaturationPtr() const { return (float*)view(SF_FIELD_MAX_SATURATION); }
+    float3* GetAvgFlowVelPtr() const { return (float3*)view(SF_FIELD_AVG_FLOW_VEL); }
+    float3* GetAvgDragForcePtr() const { return (float3*)view(SF_FIELD_AVG_DRAG_FORCE); }
+    float3* GetAdhesionForcePtr() const { return (float3*)view(SF_FIELD_ADHESION_FORCE); }
+    float3* GetAvgAdhesionForcePtr() const { return (float3*)view(SF_FIELD_AVG_ADHESION_FORCE); }
+    float3* GetCdA0AsatPtr() const { return (float3*)view(SF_FIELD_CDA0ASAT); }
+    float2* GetAmcAmcpPtr() const { return (float2*)view(SF_FIELD_AMCAMCP); }
 
     // host copies of any field (packed float3 / float / size_t as the reference stores them)
     Vec_Float3 DownloadFloat3(int field) const {
@@ -298,11 +343,10 @@ class CudaSFParticles {
 
   private:
     friend class CudaSFSystem;
-    void* ptr(int field) const {
+    void* view(int field) const {
         void* p = nullptr;
-        size_t s = 0;
         if (!mHandle) return nullptr;
-        sf_check(sf_device_buffer(mHandle, field, &p, &s), "sf_device_buffer");
+        sf_check(sf_particle_view(mHandle, field, &p), "sf_particle_view");
         return p;
     }
     void Attach(sf_handle h, bool multi) {
@@ -351,36 +395,83 @@ class CudaBoundaryParticles {
 };
 typedef SharedPtr<CudaBoundaryParticles> CudaBoundaryParticlesPtr;
 
-// Searchers are configuration here: the grid they describe is built inside
-// every step of the system (cuda_neighbor_searcher.cu:19-34).
+// Uniform-grid searchers (cuda_neighbor_searcher.cuh:22-98).  The grid and
+// its arrays live in the system's handle: the particle searcher builds them
+// with BuildGNSearcher (sf_build_neighbors: hash, stable sort of the particle
+// state by cell, cellStart); the boundary searcher's were built when the
+// system took the boundary particles (CudaBaseSystem ctor, cuda_base_system.cu:26).
+// GetCellStart() / GetGridIdxArray() are the handle's size_t arrays of the
+// last build (numCells + 1 entries / one key per particle).
 class CudaGNBaseSearcher {
   public:
     CudaGNBaseSearcher(float3 lo, float3 hi, size_t num, float cellSize)
         : mLowestPoint(lo), mHighestPoint(hi), mCellSize(cellSize), mGridSize(GridSize(lo, hi, cellSize)),
           mMaxNumOfParticles(num) {}
+    virtual ~CudaGNBaseSearcher() = default;
+    CudaGNBaseSearcher(const CudaGNBaseSearcher&) = delete;
+    CudaGNBaseSearcher& operator=(const CudaGNBaseSearcher&) = delete;
     float3 GetLowestPoint() const { return mLowestPoint; }
     float3 GetHighestPoint() const { return mHighestPoint; }
     float GetCellSize() const { return mCellSize; }
     int3 GetGridSize() const { return mGridSize; }
+    size_t* GetCellStartPtr() const { return GetCellStart().Data(); }
+    const CudaArray<size_t>& GetCellStart() const {
+        mCellStart = CudaArray<size_t>(index_view(cell_start_field()), num_cells() + 1);
+        return mCellStart;
+    }
 
   protected:
+    friend class CudaSFSystem;
+    virtual int cell_start_field() const = 0;
+    size_t num_cells() const { return (size_t)mGridSize.x * (size_t)mGridSize.y * (size_t)mGridSize.z; }
+    size_t* index_view(int field) const {
+        if (!mHandle) return nullptr;
+        void* p = nullptr;
+        sf_check(sf_particle_view(mHandle, field, &p), "sf_particle_view");
+        return (size_t*)p;
+    }
     float3 mLowestPoint, mHighestPoint;
     float mCellSize;
     int3 mGridSize;
     size_t mMaxNumOfParticles;
+    sf_handle mHandle = nullptr;  // set by the CudaSFSystem that uses this searcher
+    mutable CudaArray<size_t> mCellStart, mGridIdx;
 };
 class CudaGNSearcher final : public CudaGNBaseSearcher {
   public:
     CudaGNSearcher(float3 lo, float3 hi, size_t num, float cellSize, SearcherParticleType type)
         : CudaGNBaseSearcher(lo, hi, num, cellSize), mType(type) {}
     SearcherParticleType GetSearcherType() const { return mType; }
+    // BuildGNSearcher (cuda_neighbor_searcher.cu:36-51): hash, stable sort by
+    // cell of the particle state, cellStart.  Failures are printed, as the
+    // reference's KIRI_CUCALL does.
+    void BuildGNSearcher(const SharedPtr<CudaSFParticles>& particles) {
+        sf_handle h = particles ? particles->Handle() : nullptr;
+        if (!h) {
+            std::printf("BuildGNSearcher: the particles are not held by a CudaSFSystem\n");
+            return;
+        }
+        mHandle = h;
+        if (sf_build_neighbors(h) != SF_OK) std::printf("BuildGNSearcher failed: %s\n", sf_last_error());
+    }
+    size_t* GetGridIdxArrayPtr() const { return GetGridIdxArray().Data(); }
+    const CudaArray<size_t>& GetGridIdxArray() const {
+        size_t n = 0;
+        if (mHandle) sf_size(mHandle, &n, nullptr, nullptr);
+        mGridIdx = CudaArray<size_t>(index_view(SF_FIELD_SORTED_KEYS), n);
+        return mGridIdx;
+    }
 
   private:
+    int cell_start_field() const override { return SF_FIELD_CELL_START; }
     SearcherParticleType mType;
 };
 class CudaGNBoundarySearcher final : public CudaGNBaseSearcher {
   public:
     using CudaGNBaseSearcher::CudaGNBaseSearcher;
+
+  private:
+    int cell_start_field() const override { return SF_FIELD_B_CELL_START; }
 };
 typedef SharedPtr<CudaGNSearcher> CudaGNSearcherPtr;
 typedef SharedPtr<CudaGNBoundarySearcher> CudaGNBoundarySearcherPtr;
@@ -395,10 +486,40 @@ class CudaWCSphSFSolver {
     void SetSpeedOfSound(float c) { mOpt.speed_of_sound = c > 1e-6f ? c : 1e-6f; }        // cuda_wcsph_sf_solver.cpp:19-22
     void SetTimeStepLimitScale(float s) { mOpt.time_step_limit_scale = s > 0.f ? s : 0.f; } // :29-32
     const sf_solver_options& Options() const { return mOpt; }
+    // CudaBaseSolver::GetNumOfSubTimeSteps (cuda_base_solver.cuh:30)
+    size_t GetNumOfSubTimeSteps() const { return mNumOfSubTimeSteps; }
+
+    // UpdateSolver (cuda_wcsph_sf_solver.cuh:36-43, cuda_wcsph_sf_solver.cpp:34-179):
+    // the solver passes, the CFL sub-step count and the advect on the
+    // neighbour structures of the last BuildGNSearcher.  The cell starts are
+    // the handle's own (the arguments are the searchers' GetCellStart(), as
+    // in the reference's call); params are used for this step; bparams must
+    // describe the system's grid.  Failures are printed.
+    void UpdateSolver(SharedPtr<CudaSFParticles>& particles, SharedPtr<CudaBoundaryParticles>& /*boundaries*/,
+                      const CudaArray<size_t>& /*cellStart*/, const CudaArray<size_t>& /*bCellStart*/,
+                      float renderInterval, CudaSeepageflowParams params, CudaBoundaryParams bparams) {
+        sf_handle h = particles ? particles->Handle() : nullptr;
+        if (!h) {
+            std::printf("UpdateSolver: the particles are not held by a CudaSFSystem\n");
+            return;
+        }
+        sf_int3 g{};
+        if (sf_grid_size(bparams.lowest_point, bparams.highest_point, bparams.kernel_radius, &g) != SF_OK ||
+            g.x != bparams.grid_size.x || g.y != bparams.grid_size.y || g.z != bparams.grid_size.z) {
+            std::printf("UpdateSolver: boundary params do not describe a grid\n");
+            return;
+        }
+        if (sf_set_params(h, &params) != SF_OK || sf_update_solver(h, renderInterval, nullptr) != SF_OK) {
+            std::printf("UpdateSolver failed: %s\n", sf_last_error());
+            return;
+        }
+        sf_num_sub_steps(h, &mNumOfSubTimeSteps);
+    }
 
   private:
     size_t mNum;
     sf_solver_options mOpt;
+    size_t mNumOfSubTimeSteps = 1;
 };
 typedef SharedPtr<CudaWCSphSFSolver> CudaSphSFSolverPtr;
 
@@ -505,6 +626,8 @@ class CudaSFSystem {
             if (i < mBoundaries->mLabel.size()) bl[i] = mBoundaries->mLabel[i];
         }
         sf_check(sf_upload_boundary(mHandle, mBoundaries->mPos.size(), bp.data(), bl.data()), "sf_upload_boundary");
+        if (mSearcher) mSearcher->mHandle = mHandle;
+        if (mBoundarySearcher) mBoundarySearcher->mHandle = mHandle;
         if (mEmitter && CUDA_SPH_EMITTER_PARAMS.enable) {  // cuda_sf_system.cu:36-58
             const float r = CUDA_SEEPAGEFLOW_PARAMS.sph_particle_radius;
             switch (CUDA_SPH_EMITTER_PARAMS.emit_type) {
@@ -560,6 +683,10 @@ class CudaSFSystem {
     size_t NumOfParticles() const { return mParticles->Size(); }
     size_t MaxNumOfParticles() const { return mParticles->MaxSize(); }
     CudaSFParticlesPtr GetSFParticles() const { return mParticles; }
+    CudaSphSFSolverPtr GetSolver() const { return mSolver; }
+    CudaGNSearcherPtr GetSearcher() const { return mSearcher; }
+    CudaGNBoundarySearcherPtr GetBoundarySearcher() const { return mBoundarySearcher; }
+    CudaBoundaryParticlesPtr GetBoundaries() const { return mBoundaries; }
     sf_handle Handle() const { return mHandle; }
 
   private:
@@ -582,7 +709,16 @@ inline std::string UInt2Str4Digit(unsigned input) {
 }
 
 // ExportBgeoFileCUDA (kiri_utils.h:364-448): folder/file_name.bgeo with
-// position, Cd, pscale and label of the particles' current state.
+// position, Cd, pscale and label of n particles held in device arrays of the
+// reference's layout (the particles' Get*Ptr() views).
+inline void ExportBgeoFileCUDA(String folder, String file_name, const float3* pos, const float3* col,
+                               const float* radius, const size_t* label, const size_t particlesNum) {
+    if (sf_export_bgeo_device(folder.c_str(), file_name.c_str(), (const float*)pos, (const float*)col, radius,
+                              (const uint64_t*)label, particlesNum) != SF_OK)
+        std::printf("ExportBgeoFileCUDA failed: %s\n", sf_last_error());
+}
+
+// The same file straight from the particle state (no views needed).
 inline void ExportBgeoFileCUDA(const std::string& folder, const std::string& file_name,
                                const CudaSFParticlesPtr& particles) {
     if (sf_export_bgeo(particles->Handle(), folder.c_str(), file_name.c_str()) != SF_OK)

@@ -14,7 +14,7 @@
 // be compared bit for bit with the Python host.
 //
 //   seepage_main [--frames N] [--out FOLDER] [--render-interval S] [--fixed] [--export-initial] [--emit]
-//                [--resources DIR] [--shape FOLDER/NAME[:CD]]... [--flip-yz] [--no-emit] [--device-frame]
+//                [--resources DIR] [--shape FOLDER/NAME[:CD]]... [--flip-yz] [--no-emit] [--device-frame] [--solver-api]
 //
 // --emit switches on main.cpp's inflow emitter (main.cpp:85-95: a circle of
 // radius 0.17 at the box centre + (size/10, size/5, size/2.3), velocity
@@ -23,6 +23,10 @@
 // main.cpp:137-138); --no-emit keeps the shape scene's emitter off.
 // --device-frame runs each frame with CudaSFSystem::UpdateFrame (the same
 // loop inside the library) instead of calling UpdateSystem per sub-step.
+// --solver-api runs each sub-step as CudaSFSystem::OnUpdateSolver does
+// (cuda_sf_system.cu:66-84): searcher->BuildGNSearcher(particles), then
+// solver->UpdateSolver(particles, boundaries, cellStart, bCellStart, ...)
+// (no emitter).
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -279,6 +283,7 @@ bool parse_shape(const char* arg, ShapeSpec* out) {
 int main(int argc, char** argv) {
     int frames = 2;
     bool export_initial = false, emit = false, no_emit = false, flip_yz = false, device_frame = false;
+    bool solver_api = false;
     std::vector<ShapeSpec> shapes;
     float render_interval = 1.f / 30.f;
     std::string out = "export/bgeo/seepage_small_block";
@@ -292,6 +297,7 @@ int main(int argc, char** argv) {
         else if (!std::strcmp(argv[i], "--no-emit")) no_emit = true;
         else if (!std::strcmp(argv[i], "--flip-yz")) flip_yz = true;
         else if (!std::strcmp(argv[i], "--device-frame")) device_frame = true;
+        else if (!std::strcmp(argv[i], "--solver-api")) solver_api = true;
         else if (!std::strcmp(argv[i], "--resources") && i + 1 < argc) DbPbrPath() = argv[++i];
         else if (!std::strcmp(argv[i], "--shape") && i + 1 < argc) {
             ShapeSpec sp;
@@ -328,7 +334,17 @@ int main(int argc, char** argv) {
         } else if (system->GetAdaptiveSubTimeStep()) {
             float remaining_time = render_interval;
             while (remaining_time > 1e-6f) {
-                system->UpdateSystem(remaining_time);
+                if (solver_api) {
+                    auto particles = system->GetSFParticles();
+                    auto boundaries = system->GetBoundaries();
+                    auto searcher = system->GetSearcher();
+                    searcher->BuildGNSearcher(particles);
+                    system->GetSolver()->UpdateSolver(particles, boundaries, searcher->GetCellStart(),
+                                                      system->GetBoundarySearcher()->GetCellStart(), remaining_time,
+                                                      CUDA_SEEPAGEFLOW_PARAMS, CUDA_BOUNDARY_PARAMS);
+                } else {
+                    system->UpdateSystem(remaining_time);
+                }
                 remaining_time -= remaining_time / static_cast<float>(system->GetNumOfSubTimeSteps());
                 ++sub;
             }
@@ -339,9 +355,19 @@ int main(int argc, char** argv) {
         const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         total += sec;
         std::printf("Simulation Frame=%d, sub-steps=%zu, Time Per Frame=%.4f s\n", frame, sub, sec);
+        // the export call of main.cpp:264-275, with this loop's frame number
         if (CUDA_SEEPAGEFLOW_APP_PARAMS.bgeo_export)
-            ExportBgeoFileCUDA(CUDA_SEEPAGEFLOW_APP_PARAMS.bgeo_export_folder, UInt2Str4Digit(frame),
-                               system->GetSFParticles());
+        {
+            auto particles = system->GetSFParticles();
+            ExportBgeoFileCUDA(
+                CUDA_SEEPAGEFLOW_APP_PARAMS.bgeo_export_folder,
+                UInt2Str4Digit(frame),
+                particles->GetPosPtr(),
+                particles->GetColPtr(),
+                particles->GetRadiusPtr(),
+                particles->GetLabelPtr(),
+                particles->Size());
+        }
     }
     std::printf("Average Per Frame=%.4f s\n", total / frames);
     return 0;

@@ -214,6 +214,7 @@ int build_neighbors(sf_system* h, hipEvent_t* ev) {
     h->cur_cs ^= 1;
     if (h->K.multi) h->cur_aux ^= 1;
     h->built = true;
+    h->build_pending = true;
     SF_HIP(hipGetLastError());
     return SF_OK;
 }
@@ -259,6 +260,7 @@ int advect(sf_system* h, float dt, const CflOut* dt_dev, hipEvent_t* ev) {
     a.scene = h->nscenes ? h->scene.p : nullptr;
     launch_advect(h->K, a, (int)h->n, s);
     h->keys_fresh = fuse;
+    h->build_pending = false;
     h->cur ^= 1;
     SF_MARK(ev, SF_PASS_COUNT);
     SF_HIP(hipGetLastError());
@@ -284,12 +286,103 @@ int wcsph_substep_tail(sf_system* h, float render_interval, FrameDev* frame) {
     return advect(h, 0.f, h->cfl_out.p);
 }
 
+// CudaWCSphSFSolver::UpdateSolver / CudaSphSFSolver::UpdateSolver on the
+// current neighbour build: the passes, the sub-step count and the advect;
+// records ev1 after the advect and leaves h->nsub set (reads the WCSPH count
+// back from the device).
+int update_solver(sf_system* h, float render_interval) {
+    int rc;
+    if (h->P.solver_type == SF_WCSPH_SOLVER) {
+        if ((rc = wcsph_substep_tail(h, render_interval, nullptr))) return rc;
+        CflOut o;
+        SF_HIP(hipMemcpyAsync(&o, h->cfl_out.p, sizeof(CflOut), hipMemcpyDeviceToHost, h->stream));
+        SF_HIP(hipEventRecord(h->ev1, h->stream));
+        SF_HIP(hipEventSynchronize(h->ev1));
+        h->nsub = o.nsub;
+    } else {
+        // CudaSphSFSolver::UpdateSolver: nSub = renderInterval / params.dt, advect with params.dt
+        const float q = render_interval / h->P.dt;
+        h->nsub = q >= 1.f ? (size_t)q : 1;
+        if ((rc = solve(h))) return rc;
+        if ((rc = advect(h, h->P.dt, nullptr))) return rc;
+        SF_HIP(hipEventRecord(h->ev1, h->stream));
+        SF_HIP(hipEventSynchronize(h->ev1));
+    }
+    return SF_OK;
+}
+
 int check_handle(sf_system* h) {
     if (!h) return set_error(SF_E_INVALID, "null handle");
     return SF_OK;
 }
 int set_device(sf_system* h) {
     SF_HIP(hipSetDevice(h->device));
+    return SF_OK;
+}
+
+// bit of each field that has a reference-layout view
+constexpr uint32_t kViewable = (1u << 18) - 1u;  // SF_FIELD_POS .. SF_FIELD_AMCAMCP
+constexpr uint32_t kPersistentViews = (1u << SF_FIELD_POS) | (1u << SF_FIELD_VEL) | (1u << SF_FIELD_COL) |
+                                      (1u << SF_FIELD_LABEL) | (1u << SF_FIELD_MASS) | (1u << SF_FIELD_RADIUS) |
+                                      (1u << SF_FIELD_MAX_SATURATION) | (1u << SF_FIELD_CDA0ASAT) |
+                                      (1u << SF_FIELD_AMCAMCP);
+constexpr uint32_t kIndexViews =
+    (1u << SF_FIELD_SORTED_KEYS) | (1u << SF_FIELD_CELL_START) | (1u << SF_FIELD_B_CELL_START);
+
+static ViewArgs view_args(sf_system* h, uint32_t bits) {
+    auto v = [&](int f) -> void* { return (bits >> f) & 1u ? (void*)h->views[f].p : nullptr; };
+    ViewArgs a;
+    a.pm = h->pm[h->cur].p;
+    a.vr = h->vr[h->cur].p;
+    a.cs = h->cs[h->cur_cs].p;
+    a.cda = h->cda[h->cur_aux].p;
+    a.amc = h->amc[h->cur_aux].p;
+    a.acc = h->acc.p;
+    a.rp = h->rp.p;
+    a.es = h->es.p;
+    a.flow = h->flow.p;
+    a.xf = h->xf.p;
+    a.adh = h->adh.p;
+    a.pos = (float*)v(SF_FIELD_POS);
+    a.vel = (float*)v(SF_FIELD_VEL);
+    a.accv = (float*)v(SF_FIELD_ACC);
+    a.col = (float*)v(SF_FIELD_COL);
+    a.label = (uint64_t*)v(SF_FIELD_LABEL);
+    a.mass = (float*)v(SF_FIELD_MASS);
+    a.radius = (float*)v(SF_FIELD_RADIUS);
+    a.density = (float*)v(SF_FIELD_DENSITY);
+    a.pressure = (float*)v(SF_FIELD_PRESSURE);
+    a.voidage = (float*)v(SF_FIELD_VOIDAGE);
+    a.saturation = (float*)v(SF_FIELD_SATURATION);
+    a.maxsat = (float*)v(SF_FIELD_MAX_SATURATION);
+    a.avgflow = (float*)v(SF_FIELD_AVG_FLOW_VEL);
+    a.avgdrag = (float*)v(SF_FIELD_AVG_DRAG_FORCE);
+    a.adhesion = (float*)v(SF_FIELD_ADHESION_FORCE);
+    a.avgadh = (float*)v(SF_FIELD_AVG_ADHESION_FORCE);
+    a.cda3 = (float*)v(SF_FIELD_CDA0ASAT);
+    a.amc2 = (float2*)v(SF_FIELD_AMCAMCP);
+    return a;
+}
+
+int views_in(sf_system* h) {
+    const uint32_t bits = h->view_bits & kPersistentViews;
+    if (!bits || !h->uploaded) return SF_OK;
+    launch_views_in(view_args(h, bits), (int)h->n, h->stream);
+    SF_HIP(hipGetLastError());
+    return SF_OK;
+}
+
+int views_out(sf_system* h) {
+    if (!h->view_bits || !h->uploaded) return SF_OK;
+    if (h->view_bits & kViewable) launch_views_out(view_args(h, h->view_bits & kViewable), (int)h->n, h->stream);
+    if (((h->view_bits >> SF_FIELD_SORTED_KEYS) & 1u) && h->built && h->sorted_keys)
+        launch_widen_u32(h->sorted_keys, (uint64_t*)h->views[SF_FIELD_SORTED_KEYS].p, (int)h->n, h->stream);
+    if ((h->view_bits >> SF_FIELD_CELL_START) & 1u)
+        launch_widen_u32(h->cell.p, (uint64_t*)h->views[SF_FIELD_CELL_START].p, (int)h->C + 1, h->stream);
+    if ((h->view_bits >> SF_FIELD_B_CELL_START) & 1u)
+        launch_widen_u32(h->bcell.p, (uint64_t*)h->views[SF_FIELD_B_CELL_START].p, (int)h->local_cells() + 1,
+                         h->stream);
+    SF_HIP(hipGetLastError());
     return SF_OK;
 }
 
@@ -522,7 +615,10 @@ int sf_upload_particles(sf_handle h, size_t n, const float* pos3, const float* v
     h->n = n;
     h->uploaded = true;
     h->built = false;
+    h->build_pending = false;
     h->drop_graph();
+    if ((rc = views_out(h))) return rc;
+    SF_HIP(hipStreamSynchronize(h->stream));
     return SF_OK;
 }
 
@@ -560,15 +656,19 @@ int sf_upload_boundary(sf_handle h, size_t nb, const float* pos3, const uint64_t
     h->blabel1 = l1;
     h->K = make_const(h->P, h->B, h->O, h->blabel1);
     h->built = false;  // the particle searcher buffers were reused
+    h->build_pending = false;
     h->drop_graph();
-    return upload_scene_table(h);
+    if ((rc = upload_scene_table(h))) return rc;
+    if ((rc = views_out(h))) return rc;
+    SF_HIP(hipStreamSynchronize(h->stream));
+    return SF_OK;
 }
 
 int sf_build_neighbors(sf_handle h) {
     int rc;
     if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
     if (!h->uploaded) return set_error(SF_E_STATE, "no particles uploaded");
-    if ((rc = build_neighbors(h))) return rc;
+    if ((rc = views_in(h)) || (rc = build_neighbors(h)) || (rc = views_out(h))) return rc;
     SF_HIP(hipStreamSynchronize(h->stream));
     return SF_OK;
 }
@@ -581,8 +681,10 @@ int sf_step_n(sf_handle h, float dt, int n_steps) {
     if (!h->uploaded) return set_error(SF_E_STATE, "no particles uploaded");
     if (!std::isfinite(dt) || !(dt > 0.f)) return set_error(SF_E_INVALID, "dt must be finite and > 0");
     if (n_steps < 0) return set_error(SF_E_INVALID, "n_steps must be >= 0");
+    if ((rc = views_in(h))) return rc;
     for (int q = 0; q < n_steps; ++q)
         if ((rc = one_step(h, dt))) return rc;
+    if ((rc = views_out(h))) return rc;
     SF_HIP(hipStreamSynchronize(h->stream));
     return SF_OK;
 }
@@ -601,11 +703,13 @@ int sf_step_profiled(sf_handle h, float dt, int n_steps, float* pass_ms, float* 
         h->ev_pool.push_back(e);
     }
     hipStream_t s = h->stream;
+    if ((rc = views_in(h))) return rc;
     SF_HIP(hipEventRecord(h->ev0, s));
     for (int q = 0; q < n_steps; ++q)
         if ((rc = one_step(h, dt, pass_ms ? h->ev_pool.data() + per * q : nullptr))) return rc;
     SF_HIP(hipEventRecord(h->ev1, s));
-    SF_HIP(hipEventSynchronize(h->ev1));
+    if ((rc = views_out(h))) return rc;
+    SF_HIP(hipStreamSynchronize(s));
     if (total_ms) SF_HIP(hipEventElapsedTime(total_ms, h->ev0, h->ev1));
     if (pass_ms) {
         for (int k = 0; k < SF_PASS_COUNT; ++k) pass_ms[k] = 0.f;
@@ -637,6 +741,7 @@ int sf_step_timed(sf_handle h, float dt, int n_steps, int pass, float* pass_ms, 
     }
     hipStream_t s = h->stream;
     std::vector<hipEvent_t> marks(SF_PASS_COUNT + 1, nullptr);
+    if ((rc = views_in(h))) return rc;
     SF_HIP(hipEventRecord(h->ev0, s));
     for (int q = 0; q < n_steps; ++q) {
         marks[pass] = h->ev_pool[2 * q];
@@ -644,7 +749,8 @@ int sf_step_timed(sf_handle h, float dt, int n_steps, int pass, float* pass_ms, 
         if ((rc = one_step(h, dt, marks.data()))) return rc;
     }
     SF_HIP(hipEventRecord(h->ev1, s));
-    SF_HIP(hipEventSynchronize(h->ev1));
+    if ((rc = views_out(h))) return rc;
+    SF_HIP(hipStreamSynchronize(s));
     if (total_ms) SF_HIP(hipEventElapsedTime(total_ms, h->ev0, h->ev1));
     if (pass_ms) {
         float sum = 0.f;
@@ -665,26 +771,35 @@ int sf_update_system(sf_handle h, float render_interval, float* ms_out) {
     if (!std::isfinite(render_interval) || !(render_interval > 0.f))
         return set_error(SF_E_INVALID, "render interval must be finite and > 0");
     if (h->nscenes) return set_error(SF_E_STATE, "batched handles step with a fixed dt (sf_step, sf_step_graph)");
+    if ((rc = views_in(h))) return rc;
     SF_HIP(hipEventRecord(h->ev0, h->stream));
-    if ((rc = build_neighbors(h))) return rc;
-    if (h->P.solver_type == SF_WCSPH_SOLVER) {
-        if ((rc = wcsph_substep_tail(h, render_interval, nullptr))) return rc;
-        CflOut o;
-        SF_HIP(hipMemcpyAsync(&o, h->cfl_out.p, sizeof(CflOut), hipMemcpyDeviceToHost, h->stream));
-        SF_HIP(hipEventRecord(h->ev1, h->stream));
-        SF_HIP(hipEventSynchronize(h->ev1));
-        h->nsub = o.nsub;
-    } else {
-        // CudaSphSFSolver::UpdateSolver: nSub = renderInterval / params.dt, advect with params.dt
-        const float q = render_interval / h->P.dt;
-        h->nsub = q >= 1.f ? (size_t)q : 1;
-        if ((rc = solve(h))) return rc;
-        if ((rc = advect(h, h->P.dt, nullptr))) return rc;
-        SF_HIP(hipEventRecord(h->ev1, h->stream));
-        SF_HIP(hipEventSynchronize(h->ev1));
-    }
+    if ((rc = build_neighbors(h)) || (rc = update_solver(h, render_interval))) return rc;
     // CudaSFSystem::OnUpdateSolver emits after the solver step (cuda_sf_system.cu:86-115)
     if ((rc = emit_after_update(h, render_interval))) return rc;
+    if ((rc = views_out(h))) return rc;
+    SF_HIP(hipStreamSynchronize(h->stream));
+    if (ms_out) {
+        float ms = 0.f;
+        SF_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        *ms_out = ms;
+    }
+    return SF_OK;
+}
+
+int sf_update_solver(sf_handle h, float render_interval, float* ms_out) {
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!h->uploaded) return set_error(SF_E_STATE, "no particles uploaded");
+    if (!std::isfinite(render_interval) || !(render_interval > 0.f))
+        return set_error(SF_E_INVALID, "render interval must be finite and > 0");
+    if (h->nscenes) return set_error(SF_E_STATE, "batched handles step with a fixed dt (sf_step, sf_step_graph)");
+    if (!h->build_pending)
+        return set_error(SF_E_STATE, "sf_update_solver needs a neighbour build (sf_build_neighbors) since the last step");
+    if ((rc = views_in(h))) return rc;
+    SF_HIP(hipEventRecord(h->ev0, h->stream));
+    if ((rc = update_solver(h, render_interval))) return rc;
+    if ((rc = views_out(h))) return rc;
+    SF_HIP(hipStreamSynchronize(h->stream));
     if (ms_out) {
         float ms = 0.f;
         SF_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
@@ -814,13 +929,15 @@ int sf_update_frame(sf_handle h, float render_interval, int adaptive, size_t* su
     if (!std::isfinite(render_interval) || !(render_interval > 0.f))
         return set_error(SF_E_INVALID, "render interval must be finite and > 0");
     if (h->nscenes) return set_error(SF_E_STATE, "batched handles step with a fixed dt (sf_step, sf_step_graph)");
+    if ((rc = views_in(h))) return rc;
     SF_HIP(hipEventRecord(h->ev0, h->stream));
     size_t steps = 0;
     rc = h->P.solver_type == SF_WCSPH_SOLVER ? frame_wcsph(h, render_interval, adaptive != 0, &steps)
                                              : frame_sph(h, render_interval, adaptive != 0, &steps);
     if (rc) return rc;
     SF_HIP(hipEventRecord(h->ev1, h->stream));
-    SF_HIP(hipEventSynchronize(h->ev1));
+    if ((rc = views_out(h))) return rc;
+    SF_HIP(hipStreamSynchronize(h->stream));
     if (sub_steps) *sub_steps = steps;
     if (ms_out) {
         float ms = 0.f;
@@ -981,6 +1098,7 @@ int sf_set_field(sf_handle h, int field, const void* src, size_t src_bytes) {
             }
         }
         SF_HIP(hipMemcpyAsync(dev, v.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+        if ((r = views_out(h))) return r;
         SF_HIP(hipStreamSynchronize(h->stream));
         return SF_OK;
     };
@@ -992,6 +1110,7 @@ int sf_set_field(sf_handle h, int field, const void* src, size_t src_bytes) {
     case SF_FIELD_AMCAMCP: {
         if (src_bytes < n * 8) return set_error(SF_E_INVALID, "source buffer too small");
         SF_HIP(hipMemcpyAsync(h->amc[h->cur_aux].p, src, n * 8, hipMemcpyHostToDevice, h->stream));
+        if ((rc = views_out(h))) return rc;
         SF_HIP(hipStreamSynchronize(h->stream));
         return SF_OK;
     }
@@ -1034,6 +1153,57 @@ int sf_device_buffer(sf_handle h, int field, void** dptr, size_t* stride) {
     *dptr = p;
     *stride = s;
     return SF_OK;
+}
+
+int sf_particle_view(sf_handle h, int field, void** dptr) {
+    int rc;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    if (!dptr) return set_error(SF_E_INVALID, "null output");
+    if (field < 0 || field >= SF_FIELD_COUNT_ || !(((kViewable | kIndexViews) >> field) & 1u))
+        return set_error(SF_E_NOTFOUND, "field has no reference-layout view");
+    if (!((h->view_bits >> field) & 1u)) {
+        size_t bytes;
+        switch (field) {
+        case SF_FIELD_LABEL: case SF_FIELD_SORTED_KEYS: bytes = 8 * h->cap; break;
+        case SF_FIELD_AMCAMCP: bytes = 8 * h->cap; break;
+        case SF_FIELD_MASS: case SF_FIELD_RADIUS: case SF_FIELD_DENSITY: case SF_FIELD_PRESSURE:
+        case SF_FIELD_VOIDAGE: case SF_FIELD_SATURATION: case SF_FIELD_MAX_SATURATION: bytes = 4 * h->cap; break;
+        case SF_FIELD_CELL_START: bytes = 8 * ((size_t)h->C + 1); break;
+        case SF_FIELD_B_CELL_START: bytes = 8 * ((size_t)h->local_cells() + 1); break;
+        default: bytes = 12 * h->cap; break;  // float3
+        }
+        if ((rc = h->views[field].alloc(bytes))) return rc;
+        SF_HIP(hipMemsetAsync(h->views[field].p, 0, h->views[field].n, h->stream));
+        h->view_bits |= 1u << field;
+        if (field == SF_FIELD_POS || field == SF_FIELD_MASS || field == SF_FIELD_LABEL) {
+            // the caller may now write positions: every later build hashes them
+            h->pm_shared = true;
+            h->keys_fresh = false;
+        }
+        h->drop_graph();
+        if ((rc = views_out(h))) return rc;
+        SF_HIP(hipStreamSynchronize(h->stream));
+    }
+    *dptr = h->views[field].p;
+    return SF_OK;
+}
+
+int sf_export_bgeo_device(const char* folder, const char* name, const float* pos3, const float* col3,
+                          const float* radius, const uint64_t* label, size_t n) {
+    if (!folder || !name) return set_error(SF_E_INVALID, "null folder or name");
+    if (n > 0 && (!pos3 || !col3 || !radius || !label)) return set_error(SF_E_INVALID, "null device array");
+    std::vector<float> pos(3 * n), col(3 * n), rad(n);
+    std::vector<uint64_t> lab(n);
+    if (n) {
+        SF_HIP(hipMemcpy(pos.data(), pos3, 12 * n, hipMemcpyDeviceToHost));
+        SF_HIP(hipMemcpy(col.data(), col3, 12 * n, hipMemcpyDeviceToHost));
+        SF_HIP(hipMemcpy(rad.data(), radius, 4 * n, hipMemcpyDeviceToHost));
+        SF_HIP(hipMemcpy(lab.data(), label, 8 * n, hipMemcpyDeviceToHost));
+    }
+    int rc;
+    if ((rc = make_dirs(folder))) return rc;
+    std::string path = std::string(folder) + "/" + name + ".bgeo";
+    return sf_write_bgeo(path.c_str(), n, pos.data(), col.data(), rad.data(), lab.data());
 }
 
 int sf_export_bgeo(sf_handle h, const char* folder, const char* name) {
@@ -1126,6 +1296,7 @@ int sf_step_graph(sf_handle h, float dt, int n_steps) {
     if (!h->uploaded) return set_error(SF_E_STATE, "no particles uploaded");
     if (!std::isfinite(dt) || !(dt > 0.f)) return set_error(SF_E_INVALID, "dt must be finite and > 0");
     if (n_steps < 0) return set_error(SF_E_INVALID, "n_steps must be >= 0");
+    if ((rc = views_in(h))) return rc;
     while (n_steps >= 2 && !h->keys_cycle()) {  // at most two plain steps first: see keys_cycle
         if ((rc = one_step(h, dt))) return rc;
         --n_steps;
@@ -1158,6 +1329,7 @@ int sf_step_graph(sf_handle h, float dt, int n_steps) {
     for (int q = 0; q + 1 < n_steps; q += 2) SF_HIP(hipGraphLaunch(h->gexec, h->stream));
     if (n_steps & 1)
         if ((rc = one_step(h, dt))) return rc;
+    if ((rc = views_out(h))) return rc;
     SF_HIP(hipStreamSynchronize(h->stream));
     return SF_OK;
 }

@@ -204,6 +204,7 @@ int sf_diff_forward(sf_handle h, float dt, int n_steps, int checkpoint_every) {
     if ((rc = alloc_tape(h, checkpoint_every))) return rc;
     const int last0 = (nck - 1) * checkpoint_every;
     d->last_taped = false;
+    if ((rc = views_in(h))) return rc;
     launch_iota(d->id.p, (int)h->n, h->stream);
     for (int q = 0; q < n_steps; ++q) {
         if (q % checkpoint_every == 0 && (rc = save_state(h, d->ck[q / checkpoint_every]))) return rc;
@@ -211,7 +212,7 @@ int sf_diff_forward(sf_handle h, float dt, int n_steps, int checkpoint_every) {
     }
     d->last_taped = true;
     if ((rc = save_state(h, d->fin)) || (rc = copy(d->fin_rp.p, h->rp.p, h->n * sizeof(float2), h->stream)) ||
-        (rc = copy(d->fin_es.p, h->es.p, h->n * sizeof(float2), h->stream)))
+        (rc = copy(d->fin_es.p, h->es.p, h->n * sizeof(float2), h->stream)) || (rc = views_out(h)))
         return rc;
     SF_HIP(hipGetLastError());
     SF_HIP(hipStreamSynchronize(h->stream));
@@ -230,6 +231,7 @@ int sf_diff_reset(sf_handle h, const float* vel3, int on_device) {
         if ((rc = to_device(h, vel3, 3 * h->n, on_device, d->tmpf.p, &dv))) return rc;
         launch_set_vel_by_id(d->ck[0].id.p, dv, h->vr[h->cur].p, (int)h->n, h->stream);
     }
+    if ((rc = views_out(h))) return rc;
     SF_HIP(hipGetLastError());
     SF_HIP(hipStreamSynchronize(h->stream));
     return SF_OK;
@@ -339,7 +341,7 @@ int sf_diff_backward(sf_handle h, const float* pos_bar3, const float* vel_bar3, 
     d->last_taped = false;  // the tape slots now hold the first segment
     // leave the handle at the end of the window, as after sf_diff_forward
     if ((rc = load_state(h, d->fin)) || (rc = copy(h->rp.p, d->fin_rp.p, (size_t)n * sizeof(float2), s)) ||
-        (rc = copy(h->es.p, d->fin_es.p, (size_t)n * sizeof(float2), s)))
+        (rc = copy(h->es.p, d->fin_es.p, (size_t)n * sizeof(float2), s)) || (rc = views_out(h)))
         return rc;
     SF_HIP(hipGetLastError());
     SF_HIP(hipStreamSynchronize(s));

@@ -274,3 +274,48 @@ struct AdvectArgs {
 void launch_advect(const SFConst& k, const AdvectArgs& a, int n, hipStream_t s);
 
 }  // namespace sf
+
+namespace sf {
+// ---- reference-layout views (sf_views.hip) ----
+// The library's storage (current buffers) and the caller-visible arrays in
+// the layout of CudaSFParticles::Get*Ptr(): packed float3, float, uint64
+// labels, float2.  A view pointer is null when that view was not handed out.
+struct ViewArgs {
+    float4* pm;  // xyz position, w = +mass (water) / -mass (sand)
+    float4* vr;  // xyz velocity, w radius
+    float4* cs;  // xyz colour, w maxSaturation
+    float4* cda;
+    float2* amc;
+    const float4* acc;
+    const float2* rp;    // density, pressure
+    const float2* es;    // voidage, saturation
+    const float4* flow;  // avg flow velocity
+    const float4* xf;    // avgDrag (sand slots) / avgAdhesion (water slots)
+    const float4* adh;   // adhesion (water slots)
+    float* pos;
+    float* vel;
+    float* accv;
+    float* col;
+    uint64_t* label;
+    float* mass;
+    float* radius;
+    float* density;
+    float* pressure;
+    float* voidage;
+    float* saturation;
+    float* maxsat;
+    float* avgflow;
+    float* avgdrag;
+    float* adhesion;
+    float* avgadh;
+    float* cda3;
+    float2* amc2;
+};
+// storage -> every view that is set
+void launch_views_out(const ViewArgs& a, int n, hipStream_t s);
+// persistent views that are set (pos, vel, col, label, mass, radius,
+// maxsat, cda3, amc2) -> storage
+void launch_views_in(const ViewArgs& a, int n, hipStream_t s);
+// out[i] = in[i] widened to 64 bits (cell starts as the reference's size_t)
+void launch_widen_u32(const uint32_t* in, uint64_t* out, int n, hipStream_t s);
+}  // namespace sf

@@ -90,6 +90,13 @@ struct sf_system {
     // differentiable window (sf_diff_*), null until first used
     DiffState* diff = nullptr;
     EmitterState emitter;
+    // reference-layout views handed out (sf_particle_view), by field id;
+    // view_bits has bit f set for each
+    DevBuf<uint8_t> views[SF_FIELD_COUNT_];
+    uint32_t view_bits = 0;
+    // the last neighbour build has not been followed by an advect yet
+    // (sf_update_solver runs on that build)
+    bool build_pending = false;
     // diagnostics
     DevBuf<uint32_t> nbr_cnt, nbr_cand;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -156,6 +163,7 @@ int solve(sf_system* h, hipEvent_t* ev = nullptr);
 int solve_with(sf_system* h, const Fields& f, hipEvent_t* ev = nullptr);
 int advect(sf_system* h, float dt, const CflOut* dt_dev, hipEvent_t* ev = nullptr);
 int wcsph_substep_tail(sf_system* h, float render_interval, FrameDev* frame);
+int update_solver(sf_system* h, float render_interval);
 int one_step(sf_system* h, float dt, hipEvent_t* ev = nullptr);
 int check_handle(sf_system* h);
 // Inflow emitter after a sf_update_system step (sf_emitter.cpp);
@@ -169,5 +177,10 @@ int emitter_wcsph_check(sf_system* h);
 // WCSPH emission test is certainly still false (at most cap)
 size_t emitter_quiet_steps(const sf_system* h, double step_max, size_t cap);
 int set_device(sf_system* h);
+// Views of the reference layout: views_in takes what callers wrote into the
+// persistent fields' views into the state (start of a stepping call),
+// views_out refreshes every view from the state (end of a call that changed it).
+int views_in(sf_system* h);
+int views_out(sf_system* h);
 int fetch4(sf_system* h, const float4* src, size_t n, std::vector<float4>& out);
 }  // namespace sf

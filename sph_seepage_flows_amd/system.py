@@ -92,6 +92,13 @@ class SFSystem:
         check(lib().sf_update_frame(self._h, float(render_interval), int(adaptive), C.byref(k), C.byref(ms)))
         return k.value, ms.value
 
+    def UpdateSolver(self, render_interval: float) -> float:
+        """CudaWCSphSFSolver::UpdateSolver on the last BuildGNSearcher's
+        structures (sf_update_solver): passes, sub-step count, advect; no emission."""
+        ms = C.c_float(0.0)
+        check(lib().sf_update_solver(self._h, float(render_interval), C.byref(ms)))
+        return ms.value
+
     def GetNumOfSubTimeSteps(self) -> int:
         k = C.c_size_t(0)
         check(lib().sf_num_sub_steps(self._h, C.byref(k)))
@@ -215,9 +222,25 @@ class SFSystem:
         check(lib().sf_device_buffer(self._h, fid, C.byref(p), C.byref(s)))
         return p.value or 0, s.value
 
+    def view(self, name: str) -> int:
+        """Device address of the field's reference-layout array (sf_particle_view):
+        what CudaSFParticles::Get*Ptr() / GetCellStart() return."""
+        fid, _ = FIELDS[name]
+        p = C.c_void_p()
+        check(lib().sf_particle_view(self._h, fid, C.byref(p)))
+        return p.value or 0
+
     def export_bgeo(self, folder: str, name: str) -> str:
         check(lib().sf_export_bgeo(self._h, folder.encode(), name.encode()))
         return f"{folder}/{name}.bgeo"
+
+
+def export_bgeo_device(folder: str, name: str, pos_dev: int, col_dev: int, radius_dev: int, label_dev: int,
+                       n: int) -> str:
+    """ExportBgeoFileCUDA(folder, name, pos, col, radius, label, n) over device
+    arrays in the reference's layout (sf_export_bgeo_device)."""
+    check(lib().sf_export_bgeo_device(folder.encode(), name.encode(), pos_dev, col_dev, radius_dev, label_dev, int(n)))
+    return f"{folder}/{name}.bgeo"
 
 
 def write_bgeo(path: str, pos, col, radius, label) -> None:

@@ -150,3 +150,22 @@ def test_cpp_driver_device_frame_loop_is_identical(tmp_path):
         outs.append(out)
     for f in ("0001.bgeo", "0002.bgeo"):
         assert (outs[0] / f).read_bytes() == (outs[1] / f).read_bytes(), f
+
+
+def test_cpp_driver_solver_api_is_identical(tmp_path):
+    """--solver-api steps with the reference's OnUpdateSolver sequence
+    (searcher->BuildGNSearcher(particles); solver->UpdateSolver(particles,
+    boundaries, searcher->GetCellStart(), bsearcher->GetCellStart(), ...),
+    cuda_sf_system.cu:66-84) and writes the same frames, byte for byte, as
+    the UpdateSystem loop; the frames go out through main.cpp's own export
+    call on the particles' Get*Ptr() arrays (main.cpp:264-275)."""
+    outs = []
+    for extra in ([], ["--solver-api"]):
+        out = tmp_path / ("api" if extra else "sys")
+        r = subprocess.run([str(DRIVER), "--frames", "2", "--out", str(out), *extra], capture_output=True, text=True,
+                           timeout=300)
+        print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+        assert r.returncode == 0 and "failed" not in r.stdout
+        outs.append(out)
+    for f in ("0001.bgeo", "0002.bgeo"):
+        assert (outs[0] / f).read_bytes() == (outs[1] / f).read_bytes(), f

@@ -68,12 +68,12 @@ def test_mixed_200_steps(oracle_mod):
 
 
 def test_drop_in_1m_neighbour_structures_every_50_steps(oracle_mod):
-    """1M-particle scene: after every 50 GPU steps, the oracle rebuilds keys,
+    """1M-particle scene, 500 steps: after every 50 GPU steps, the oracle rebuilds keys,
     permutation, cellStart and neighbour counts from the GPU's own positions;
     the GPU's next neighbour build must match them bit for bit."""
     big = scenes.drop_in_1m()
     g = SFSystem.from_scene(big)
-    steps = int(os.environ.get("SF_MULTISTEP_1M", "150"))
+    steps = int(os.environ.get("SF_MULTISTEP_1M", "500"))
     for k in range(50, steps + 1, 50):
         g.step(1e-4, 50)
         pos = g.get("pos")
