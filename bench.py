@@ -32,21 +32,35 @@ PROFILE_STEPS = 10  # untimed steps with an event between every two passes
 HBM_PEAK_GBS = 8000.0
 
 
-def pass_bytes(n_w: int, n_s: int, nb: int, cells: int, passes: int, multi: bool) -> dict:
+def pass_bytes(n_w: int, n_s: int, nb: int, cells: int, passes: int, multi: bool, listed=(0, 0)) -> dict:
     """Algorithmic HBM bytes per launch of each pass: every array element the
     pass must touch, read or written once (neighbour re-reads are served
     on-chip and not counted).  Layout: pm/vr/cs/flow/adh/xf/acc float4 (16 B),
-    rp/es float2 (8 B), keys/perm/cellStart u32 (4 B), boundary float4."""
+    rp/es/wt float2 (8 B), keys/perm/cellStart/lists u32 (4 B), flags u8,
+    boundary float4.  listed = (water, sand) particles with the other phase in
+    their 27 cells: the interface passes run over those only, and the density
+    pass writes the empty sums' results of the rest that a later pass reads
+    (water wt, sand voidage / saturation) plus one flag byte each."""
     n = n_w + n_s
     aux = 24 if multi else 0
+    lw, ls = listed
     return {
         "hash": n * (16 + 4),
         "sort": n * (4 + 16) * passes - n * 4,          # upsweep key read + key/value read+write per pass
         "gather": n * (4 + 2 * (48 + aux)) + n * 16 * 2 + 4 * (n + 1),  # + water-prefix scan (2 reads of pm.w)
         "cell_start": 4 * (cells + 1) * 3 + 4 * n,                          # cellStart + (start, prefix) table
-        "density": n * (16 + 8) + nb * 16 + 8 * (cells + 1) + 4 * (cells + 1),
-        "avg_flow": n * (16 + 16 + 8 + 16 + 8) + n_w * 8 + 8 * (cells + 1),
-        "voidage_adhesion": n * (16 + 8 + 8) + n_s * 4 + n_w * 32 + n_s * 16 + 8 * (cells + 1),
+        # plist, own position (water: pm, sand: pph), water candidates' pph, rho/p;
+        # unlisted: wt or es + flag; listed: velocity, (rho, p), list entry
+        "density": n_w * (4 + 16 + 16 + 8 + 1) + n_s * (4 + 16 + 8 + 1) + (n_w - lw) * 8 + (n_s - ls) * 8
+                   + (lw + ls) * (16 + 4) + lw * 16 + 8 * (cells + 1) + 4 * (cells + 1) + cells + nb * 16,
+        # listed only: list + slot, own position, phase-index position / velocity
+        # (/ rho for water), flow, es, water wt and voidage, sand saturation
+        "avg_flow": lw * (4 + 4 + 16 + 16 + 16 + 16 + 8 + 16 + 8 + 8 + 4) + ls * (4 + 4 + 16 + 16 + 16 + 16 + 8 + 4)
+                    + 8 * (cells + 1),
+        # listed only: list + slot, own position, phase-index position / (rho, voidage, sat);
+        # sand voidage + adhesion 0, water adhesion + avgAdhesion (slot and phase index)
+        "voidage_adhesion": lw * (4 + 4 + 16 + 16 + 16 + 48) + ls * (4 + 4 + 16 + 16 + 16 + 4 + 16)
+                            + 8 * (cells + 1),
         "sand_momentum": n * (16 + 16 + 8 + 8) + n_w * 16 + n_s * (16 + 16 + 16 + 16 + 16) + 8 * (cells + 1),
         "water_momentum": n * (16 + 16 + 8) + n_s * 16 + n_w * (8 + 16 + 16) + nb * 16 + 8 * (cells + 1) + 4 * (cells + 1),
         "advect": n * (16 * 3 + 16 * 2) + n_s * 16,
@@ -59,20 +73,75 @@ def radix_passes(cells: int) -> int:
     return (bits + 7) // 8
 
 
-def cpu_baseline(scene, steps: int, dt: float) -> dict:
+def interface_counts(pos, lab, b) -> tuple[int, int]:
+    """(water, sand) particles with a particle of the other phase in their 27
+    grid cells: the interface passes' particles (the density pass's test),
+    counted on the host from FP32 cell coordinates as the grid hash computes
+    them (b: the sf_boundary_params)."""
+    import numpy as np
+
+    g = np.array([b.grid_size.x, b.grid_size.y, b.grid_size.z], np.int64)
+    lo = np.array([b.lowest_point.x, b.lowest_point.y, b.lowest_point.z], np.float32)
+    q = (np.asarray(pos, np.float32) - lo) / np.float32(b.kernel_radius)
+    c = np.where(q >= 0, np.where(q >= g.astype(np.float32), g - 1, np.floor(np.nan_to_num(q)).astype(np.int64)), 0)
+    key = (c[:, 0] * g[1] + c[:, 1]) * g[2] + c[:, 2]
+    lab = np.asarray(lab)
+    C = int(g.prod())
+    cnt = [np.bincount(key[lab == ph], minlength=C) for ph in (0, 1)]
+    other = np.zeros(len(key), np.int64)
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                x, y, z = c[:, 0] + dx, c[:, 1] + dy, c[:, 2] + dz
+                ok = (x >= 0) & (x < g[0]) & (y >= 0) & (y < g[1]) & (z >= 0) & (z < g[2])
+                k = np.where(ok, (x * g[1] + y) * g[2] + z, 0)
+                other += np.where(ok, np.where(lab == 0, cnt[1][k], cnt[0][k]), 0)
+    listed = other > 0
+    return int((listed & (lab == 0)).sum()), int((listed & (lab == 1)).sum())
+
+
+def cpu_baseline(scene, steps: int, dt: float, small_steps: int, one_thread_steps: int) -> dict:
+    """The FP32 C++ oracle (OpenMP over particles) on the host cores this
+    process may use (the GPU's CPU share, OMP_NUM_THREADS when set), on the
+    headline scene; plus a 1-thread sample of the same scene and the small
+    block scene (24,576 particles) for `small_steps` steps."""
     import oracle
 
+    from sph_seepage_flows_amd import scenes
+
     oracle.build()
-    cores = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1)
-    cores = oracle.threads(cores)
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    want = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or avail
+    cores = oracle.threads(min(want, avail))
     o = oracle.OracleSystem(scene, 32)
     o.step(dt, 1)  # warm-up (first-touch allocation)
     t = time.perf_counter()
     o.step(dt, steps)
     sec = time.perf_counter() - t
-    return {"value": scene.n * steps / sec, "unit": "particle-steps/s", "cores": cores, "kind": "port",
-            "sample": f"FP32 C++ oracle (OpenMP), {steps} steps of the same {scene.n}-particle scene after 1 "
-                      f"warm-up step; {sec:.1f} s"}
+    out = {"value": scene.n * steps / sec, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+           "sample": f"FP32 C++ oracle (OpenMP, {cores} threads), {steps} steps of the same {scene.n}-particle "
+                     f"scene after 1 warm-up step; {sec:.1f} s",
+           "host_cpus_reported": os.cpu_count(), "cpus_in_affinity": avail}
+    if one_thread_steps:
+        oracle.threads(1)
+        t = time.perf_counter()
+        o.step(dt, one_thread_steps)
+        sec1 = time.perf_counter() - t
+        out["one_thread"] = {"value": scene.n * one_thread_steps / sec1, "cores": 1,
+                             "sample": f"{one_thread_steps} further steps of the same scene; {sec1:.1f} s"}
+        oracle.threads(cores)
+    o.close()
+    if small_steps:
+        sb = scenes.small_block()
+        o = oracle.OracleSystem(sb, 32)
+        t = time.perf_counter()
+        o.step(dt, small_steps)
+        secs = time.perf_counter() - t
+        o.close()
+        out["small_block"] = {"value": sb.n * small_steps / secs, "cores": cores,
+                              "sample": f"{sb.name}: {sb.n} particles, {small_steps} steps from the initial "
+                                        f"state; {secs:.1f} s"}
+    return out
 
 
 def ensemble_bench(dt: float, steps_batch: int, steps_plain: int) -> dict:
@@ -253,6 +322,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-small-steps", type=int, default=2000,
+                    help="steps of the small-block scene in the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-one-thread-steps", type=int, default=2,
+                    help="1-thread steps of the headline scene in the CPU baseline (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="drop_in_1m", choices=["drop_in_1m", "small_block", "dam_4m"])
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
@@ -322,14 +395,23 @@ def main() -> int:
         nbr_pairs = {"particles": int(mine.sum()), "pairs_within_h_per_launch": pairs,
                      "candidates_per_launch": cands, "pairs_per_s": pairs / (dom_ms * 1e-3),
                      "candidates_per_s": cands / (dom_ms * 1e-3)}
+        listed = interface_counts(g.get("pos"), lab, g._b)
+        stiff = g._p.sph_stiff
         g.close()
+        # the configuration's own run: 500 steps from the initial state, one
+        # event pair around all of them
+        g5 = sf.SFSystem.from_scene(scene, device=0)
+        t500, _ = g5.step_profiled(dt, 500, per_pass=False)
+        g5.close()
+        run500 = {"steps": 500, "ms_per_step_mean": t500 / 500, "particle_steps_per_s": n * 500 / (t500 / 1e3)}
+        print(f"[bench] 500 steps from the initial state: {t500 / 500:.4f} ms/step", file=sys.stderr, flush=True)
         ms_per_step = total_ms / args.steps
         print(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step (events), wall {wall * 1e3 / args.steps:.3f} "
               f"ms/step; {dom} {dom_ms:.4f} ms; passes of {PROFILE_STEPS} profiled steps "
               f"{json.dumps({k: round(v, 4) for k, v in pass_ms.items()})}", file=sys.stderr, flush=True)
         n_s = int(np.count_nonzero(scene.label == 1))
         n_w = n - n_s
-        pb = pass_bytes(n_w, n_s, nb, cells, radix_passes(cells), scene.physics.sf_type == 1)
+        pb = pass_bytes(n_w, n_s, nb, cells, radix_passes(cells), scene.physics.sf_type == 1, listed)
         ach = pb[dom] / (dom_ms * 1e-3) / 1e9
         step_bytes = sum(pb.values())
         traffic = None
@@ -355,8 +437,10 @@ def main() -> int:
             "value": n * args.steps / (total_ms / 1e3),
             "ms_per_step": ms_per_step,
             "config": {"workload": f"{scene.name}: {n_w} water + {n_s} sand, {nb} boundary particles, "
-                                   f"{cells} grid cells, dt=1e-4, FP32", "n_particles": n,
-                       "parallelism": "single-gpu"},
+                                   f"{cells} grid cells, dt=1e-4, FP32, Tait stiffness B = {stiff:.1f} Pa (the "
+                                   f"reference driver's 0.001 c^2 rho0 / 7 with c = 100, main.cpp:48,53; "
+                                   f"c0 = 40 m/s would give c0^2 rho0 / 7 = 228571 Pa)", "n_particles": n,
+                       "tait_stiffness_pa": stiff, "parallelism": "single-gpu"},
             "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "alg_bytes_per_launch": pb[dom],
                          "mean_launch_ms": dom_ms, "valu": valu, "neighbour_pairs": nbr_pairs},
@@ -364,6 +448,8 @@ def main() -> int:
                               "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
                               "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
             "pass_ms_profiled": pass_ms,
+            "interface_particles": {"water": listed[0], "sand": listed[1]},
+            "run_500_steps": run500,
         })
     if "mixed" in parts:
         print("[bench] mixed state (6 frames first)...", file=sys.stderr, flush=True)
@@ -383,7 +469,7 @@ def main() -> int:
         print(f"[bench] frame loop: {json.dumps(out['frame_loop'])}", file=sys.stderr, flush=True)
     if rank == 0 and "step" in parts and not args.no_cpu_baseline:
         print(f"[bench] cpu baseline ({args.cpu_steps} oracle steps)...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(scene, args.cpu_steps, dt)
+        out["cpu_baseline"] = cpu_baseline(scene, args.cpu_steps, dt, args.cpu_small_steps, args.cpu_one_thread_steps)
     if rank == 0:
         print(json.dumps(out), flush=True)
     return 0

@@ -129,7 +129,7 @@ int ensure_sort_capacity(sf_system* h, size_t m) {
     if ((rc = h->hist.alloc(256 * nblocks))) return rc;
     if ((rc = h->wpre.alloc(c + 1)) || (rc = h->wpart.alloc((size_t)wprefix_blocks((int)c + 1) + 1)) ||
         (rc = h->plist.alloc(c)) || (rc = h->wflag.alloc(c)) || (rc = h->pph.alloc(c)) || (rc = h->ilist.alloc(c)) ||
-        (rc = h->icount.alloc(2)) || (rc = h->iface.alloc(c)) || (rc = h->vq.alloc(c)) || (rc = h->wq.alloc(c)) ||
+        (rc = h->icount.alloc(2)) || (rc = h->iface.alloc(c)) || (rc = h->wl.alloc(c)) || (rc = h->vq.alloc(c)) || (rc = h->wq.alloc(c)) ||
         (rc = h->xq.alloc(c)))
         return rc;
     // the interface counters start at zero (each step's water momentum pass
@@ -224,6 +224,8 @@ int solve(sf_system* h, hipEvent_t* ev) { return solve_with(h, h->fields(), ev);
 int solve_with(sf_system* h, const Fields& f, hipEvent_t* ev) {
     const int n = (int)h->n;
     hipStream_t s = h->stream;
+    h->empty_pending = !(f.full_empty || f.den4);
+    h->n_built = h->n;
     SF_MARK(ev, SF_PASS_DENSITY);
     launch_density(h->K, f, n, h->cfg[0], s);
     SF_MARK(ev, SF_PASS_AVG_FLOW);
@@ -364,6 +366,25 @@ static ViewArgs view_args(sf_system* h, uint32_t bits) {
     return a;
 }
 
+int drop_fresh_keys(sf_system* h) {
+    if (!h->keys_fresh) return SF_OK;
+    // the last advect hashed the next build's keys and took the newcomers'
+    // ranks in in_count; the next build hashes and ranks again from zero
+    SF_HIP(hipMemsetAsync(h->in_count.p, 0, h->in_count.n * sizeof(uint32_t), h->stream));
+    h->keys_fresh = false;
+    return SF_OK;
+}
+
+int fill_empty(sf_system* h) {
+    if (!h->empty_pending) return SF_OK;
+    Fields f = h->fields();
+    f.nwater = h->wpre.p + h->n_built;
+    launch_fill_empty(f, (int)h->n_built, h->stream);
+    SF_HIP(hipGetLastError());
+    h->empty_pending = false;
+    return SF_OK;
+}
+
 int views_in(sf_system* h) {
     const uint32_t bits = h->view_bits & kPersistentViews;
     if (!bits || !h->uploaded) return SF_OK;
@@ -372,14 +393,23 @@ int views_in(sf_system* h) {
     return SF_OK;
 }
 
-int views_out(sf_system* h) {
-    if (!h->view_bits || !h->uploaded) return SF_OK;
-    if (h->view_bits & kViewable) launch_views_out(view_args(h, h->view_bits & kViewable), (int)h->n, h->stream);
-    if (((h->view_bits >> SF_FIELD_SORTED_KEYS) & 1u) && h->built && h->sorted_keys)
+int views_out(sf_system* h) { return views_out_bits(h, h->view_bits); }
+
+int views_out_bits(sf_system* h, uint32_t bits) {
+    if (!bits || !h->uploaded) return SF_OK;
+    constexpr uint32_t kEmptySumViews = (1u << SF_FIELD_VOIDAGE) | (1u << SF_FIELD_SATURATION) |
+                                        (1u << SF_FIELD_AVG_FLOW_VEL) | (1u << SF_FIELD_ADHESION_FORCE) |
+                                        (1u << SF_FIELD_AVG_ADHESION_FORCE);
+    if (bits & kEmptySumViews) {
+        int rc = fill_empty(h);
+        if (rc) return rc;
+    }
+    if (bits & kViewable) launch_views_out(view_args(h, bits & kViewable), (int)h->n, h->stream);
+    if (((bits >> SF_FIELD_SORTED_KEYS) & 1u) && h->built && h->sorted_keys)
         launch_widen_u32(h->sorted_keys, (uint64_t*)h->views[SF_FIELD_SORTED_KEYS].p, (int)h->n, h->stream);
-    if ((h->view_bits >> SF_FIELD_CELL_START) & 1u)
+    if ((bits >> SF_FIELD_CELL_START) & 1u)
         launch_widen_u32(h->cell.p, (uint64_t*)h->views[SF_FIELD_CELL_START].p, (int)h->C + 1, h->stream);
-    if ((h->view_bits >> SF_FIELD_B_CELL_START) & 1u)
+    if ((bits >> SF_FIELD_B_CELL_START) & 1u)
         launch_widen_u32(h->bcell.p, (uint64_t*)h->views[SF_FIELD_B_CELL_START].p, (int)h->local_cells() + 1,
                          h->stream);
     SF_HIP(hipGetLastError());
@@ -1025,6 +1055,13 @@ int sf_download(sf_handle h, int field, void* dst, size_t dst_bytes) {
         return SF_OK;
     };
     switch (field) {
+    case SF_FIELD_VOIDAGE: case SF_FIELD_SATURATION: case SF_FIELD_AVG_FLOW_VEL: case SF_FIELD_ADHESION_FORCE:
+    case SF_FIELD_AVG_ADHESION_FORCE:
+        if ((rc = fill_empty(h))) return rc;
+        break;
+    default: break;
+    }
+    switch (field) {
     case SF_FIELD_POS: return put3(h->pm[h->cur].p, n, -1);
     case SF_FIELD_VEL: return put3(h->vr[h->cur].p, n, -1);
     case SF_FIELD_ACC: return put3(h->acc.p, n, -1);
@@ -1128,7 +1165,7 @@ int sf_device_buffer(sf_handle h, int field, void** dptr, size_t* stride) {
     case SF_FIELD_POS: case SF_FIELD_MASS: case SF_FIELD_LABEL:
         // the caller may now write positions: every later build hashes them
         h->pm_shared = true;
-        h->keys_fresh = false;
+        if ((rc = set_device(h)) || (rc = drop_fresh_keys(h))) return rc;
         h->drop_graph();
         p = h->pm[h->cur].p;
         break;
@@ -1149,6 +1186,19 @@ int sf_device_buffer(sf_handle h, int field, void** dptr, size_t* stride) {
     case SF_FIELD_B_POS: case SF_FIELD_B_VOLUME: case SF_FIELD_B_LABEL: p = h->bpv.p; break;
     case SF_FIELD_B_CELL_START: p = h->bcell.p; s = 4; break;
     default: return set_error(SF_E_NOTFOUND, "field has no device buffer");
+    }
+    switch (field) {
+    case SF_FIELD_VOIDAGE: case SF_FIELD_SATURATION: case SF_FIELD_AVG_FLOW_VEL:
+    case SF_FIELD_AVG_DRAG_FORCE: case SF_FIELD_AVG_ADHESION_FORCE: case SF_FIELD_ADHESION_FORCE: {
+        // the caller reads these buffers directly: every later step writes
+        // the empty sums' constants too
+        h->full_empty = true;
+        h->drop_graph();
+        if ((rc = set_device(h)) || (rc = fill_empty(h))) return rc;
+        SF_HIP(hipStreamSynchronize(h->stream));
+        break;
+    }
+    default: break;
     }
     *dptr = p;
     *stride = s;
@@ -1178,10 +1228,11 @@ int sf_particle_view(sf_handle h, int field, void** dptr) {
         if (field == SF_FIELD_POS || field == SF_FIELD_MASS || field == SF_FIELD_LABEL) {
             // the caller may now write positions: every later build hashes them
             h->pm_shared = true;
-            h->keys_fresh = false;
+            if ((rc = drop_fresh_keys(h))) return rc;
         }
         h->drop_graph();
-        if ((rc = views_out(h))) return rc;
+        // fill this view only: the others may hold writes for the next step
+        if ((rc = views_out_bits(h, 1u << field))) return rc;
         SF_HIP(hipStreamSynchronize(h->stream));
     }
     *dptr = h->views[field].p;
@@ -1327,6 +1378,10 @@ int sf_step_graph(sf_handle h, float dt, int n_steps) {
             return set_error(SF_E_STATE, "two steps did not restore the buffer parity");
     }
     for (int q = 0; q + 1 < n_steps; q += 2) SF_HIP(hipGraphLaunch(h->gexec, h->stream));
+    if (n_steps >= 2) {  // the replayed solves, as solve_with records them
+        h->empty_pending = !h->full_empty;
+        h->n_built = h->n;
+    }
     if (n_steps & 1)
         if ((rc = one_step(h, dt))) return rc;
     if ((rc = views_out(h))) return rc;

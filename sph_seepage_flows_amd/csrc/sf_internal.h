@@ -191,6 +191,15 @@ struct Fields {
     // has water in its 9 column ranges (written by the density pass, read by
     // the sand momentum pass to skip an empty water walk)
     uint8_t* iface;
+    // wl[t] = 1 when water particle t (phase index) is listed (density pass).
+    // Unlisted particles' avg flow, adhesion, water voidage / saturation and
+    // avgAdhesion are the empty sums' constants; the density pass writes them
+    // only when full_empty is set (or den4, the differentiable window),
+    // otherwise they are filled in on request (launch_fill_empty).  The
+    // fields a later pass reads (sand voidage / saturation, water wt) are
+    // always written.
+    uint8_t* wl;
+    int full_empty;
     // Fields of the listed particles (those with the other phase in their 9
     // column ranges) by phase index t, written only for those:
     //   vq[t] = the velocity (density pass);
@@ -223,6 +232,10 @@ void launch_density(const SFConst& k, const Fields& f, int n, PassCfg c, hipStre
 void launch_avg_flow(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);
 void launch_zero_counts(const Fields& f, hipStream_t s);
 void launch_interface(const SFConst& k, const Fields& f, int n, hipStream_t s);
+// The empty sums' constants of every unlisted particle's avg flow, adhesion,
+// water voidage / saturation and avgAdhesion (what full_empty writes), over
+// the n particles of the last neighbour build.
+void launch_fill_empty(const Fields& f, int n, hipStream_t s);
 void launch_voidage_adhesion(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);
 void launch_sand_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);
 void launch_water_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);

@@ -350,17 +350,25 @@ void dispatch(const Fields& f, PassCfg c, L&& l) {
 // candidate order.
 
 // What P2 then P3 leave for particle i with empty sums; rpi = (rho, p).
-SF_DEV void write_no_interface(const Fields& f, int i, bool sand, float2 rpi) {
+// Later passes read the sand's voidage / saturation (drag, capillary bridges)
+// and the water's wt; the rest is output only and written here when the
+// fields are read directly (full_empty) or taped (den4), else on request.
+SF_DEV void write_empty_outputs(const Fields& f, int i, bool sand) {
     f.flow[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     f.adh[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!sand) {
+        f.es[i] = make_float2(1.f, 0.f);
+        f.xf[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+SF_DEV void write_no_interface(const Fields& f, int i, bool sand, float2 rpi) {
     if (sand) {
         f.es[i] = make_float2(0.f, 0.f);
     } else {
-        f.es[i] = make_float2(1.f, 0.f);
-        f.xf[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         const float rt = rpi.x / fmaxf(kEps, 1.f);
         f.wt[i] = make_float2(rt, rpi.y / fmaxf(kEps, rt * rt));
     }
+    if (f.full_empty || f.den4) write_empty_outputs(f, i, sand);
     if (f.den4) {
         reinterpret_cast<float*>(f.den4)[4 * i] = 0.f;
         reinterpret_cast<float*>(f.den4)[4 * i + 1] = 0.f;
@@ -489,6 +497,7 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
     const bool listed = water && lead && other != 0u;
     list_interface(f, listed, false, t, nw, (uint32_t)n);
     if (!water || !lead) return;
+    f.wl[t] = listed ? 1 : 0;
     const float rho = pi.w * k.w0 + k.w_coef * (sf + k.rho0 * sb);
     const float2 rpi = make_float2(rho, eos(k, rho));
     f.rp[i] = rpi;
@@ -534,7 +543,20 @@ __global__ __launch_bounds__(kBlock) void k_interface(SFConst k, Fields f, int n
     }
     const bool listed = valid && other != 0u;
     list_interface(f, listed, sand_i, t, nw, (uint32_t)n);
+    if (valid && !sand_i) f.wl[t] = listed ? 1 : 0;
+    if (valid && sand_i) f.iface[t - nw] = listed ? 1 : 0;
     if (valid && !listed) write_no_interface(f, i, sand_i, ldo(f.rp, (uint32_t)i));
+}
+__global__ __launch_bounds__(kBlock) void k_fill_empty(Fields f, int n) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= (uint32_t)n) return;
+    const uint32_t nw = *f.nwater;
+    const bool sand = t >= nw;
+    if (sand ? ldo(f.iface, t - nw) : ldo(f.wl, t)) return;
+    write_empty_outputs(f, (int)ldo(f.plist, t), sand);
+}
+void launch_fill_empty(const Fields& f, int n, hipStream_t s) {
+    if (n > 0) k_fill_empty<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(f, n);
 }
 void launch_zero_counts(const Fields& f, hipStream_t s) { k_zero_counts<<<1, 64, 0, s>>>(f.icount); }
 void launch_interface(const SFConst& k, const Fields& f, int n, hipStream_t s) {
@@ -1025,7 +1047,8 @@ __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int 
         a.z += __shfl_xor(a.z, o);
     }
     if (sub != 0) return;
-    const v3 ad = xyz(ldo(f.adh, (uint32_t)i));
+    // adhesion: P3 wrote it for the listed water; the rest has none
+    const v3 ad = ldo(f.wl, t) ? xyz(ldo(f.adh, (uint32_t)i)) : mk3(0.f, 0.f, 0.f);
     const v3 acc = mk3(k.gvx, k.gvy, k.gvz) + a - ad * (1.f / mi);
     f.acc[i] = f4(acc, 0.f);
 }

@@ -46,6 +46,13 @@ struct sf_system {
     DevBuf<float4> pph;  // positions in phase order (Fields::pph)
     DevBuf<uint32_t> ilist, icount;  // interface particles (Fields::ilist)
     DevBuf<uint8_t> iface;            // sand with water in its columns (Fields::iface)
+    DevBuf<uint8_t> wl;               // listed water (Fields::wl)
+    // the density pass writes every empty-sum field (sf_device_buffer handed
+    // one of those buffers out); otherwise empty_pending marks a solve whose
+    // unlisted particles still need them before a read (n_built particles)
+    bool full_empty = false;
+    bool empty_pending = false;
+    size_t n_built = 0;
     DevBuf<uint8_t> bnear;            // cells with boundary particles near (Fields::bnear)
     DevBuf<float4> vq, wq, xq;        // listed particles' fields by phase index (Fields::wq)
     DevBuf<uint8_t> wflag;
@@ -130,6 +137,8 @@ struct sf_system {
         f.ilist = ilist.p;
         f.icount = icount.p;
         f.iface = iface.p;
+        f.wl = wl.p;
+        f.full_empty = full_empty ? 1 : 0;
         f.vq = vq.p;
         f.wq = wq.p;
         f.xq = xq.p;
@@ -182,5 +191,13 @@ int set_device(sf_system* h);
 // views_out refreshes every view from the state (end of a call that changed it).
 int views_in(sf_system* h);
 int views_out(sf_system* h);
+// Before anything outside a step may move particles: forgets the next
+// build's keys and newcomer ranks that the last advect took (the build then
+// hashes and ranks them again).
+int drop_fresh_keys(sf_system* h);
+int views_out_bits(sf_system* h, uint32_t bits);  // the views in `bits` only
+// Writes the empty sums' fields of the unlisted particles of the last solve
+// when they are still pending (before the host reads those fields).
+int fill_empty(sf_system* h);
 int fetch4(sf_system* h, const float4* src, size_t n, std::vector<float4>& out);
 }  // namespace sf

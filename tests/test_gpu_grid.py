@@ -4,13 +4,13 @@ permutation and cellStart are compared bit for bit with the oracle's FP32
 hash restatement, numpy's stable argsort and a lower-bound count, for grids
 whose radix sort takes one, two and three passes (the work pair and the
 caller's pair swap roles with the pass count), and on the edge cases."""
-import ctypes as C
-
 import numpy as np
 import pytest
 
 from sph_seepage_flows_amd import _abi, scenes
 from sph_seepage_flows_amd._abi import check, lib, sf_float3, sf_int3
+
+from _hip import DeviceArray
 
 pytestmark = pytest.mark.gpu
 
@@ -25,22 +25,17 @@ def grid_build_ref(oracle_mod, pos, lo, h, g):
 
 
 def run_grid(pos, lo, h, g, fn="sf_grid_build"):
-    import torch
-
+    """one grid build through the C-ABI on device arrays of the library's runtime"""
     n = len(pos)
     C_ = int(g[0]) * int(g[1]) * int(g[2])
-    dev = torch.device("cuda:0")
-    p = torch.from_numpy(np.ascontiguousarray(pos, np.float32).reshape(-1)).to(dev)
-    sk = torch.full((max(n, 1),), 0x7F7F7F7F, dtype=torch.int32, device=dev)
-    pm = torch.full((max(n, 1),), 0x7F7F7F7F, dtype=torch.int32, device=dev)
-    cs = torch.full((C_ + 1,), 0x7F7F7F7F, dtype=torch.int32, device=dev)
-    torch.cuda.synchronize()
-    check(getattr(lib(), fn)(p.data_ptr() if n else None, n, sf_float3.of(lo), float(np.float32(h)),
-                             sf_int3(*[int(v) for v in g]), sk.data_ptr() if n else None,
-                             pm.data_ptr() if n else None, cs.data_ptr(), None))
-    torch.cuda.synchronize()
-    u = lambda t: t.cpu().numpy().view(np.uint32)
-    return u(sk)[:n], u(pm)[:n], u(cs)
+    p = DeviceArray.of(np.ascontiguousarray(pos, np.float32).reshape(-1, 3))
+    sk = DeviceArray.of(np.full(max(n, 1), 0x7F7F7F7F, np.uint32))
+    pm = DeviceArray.of(np.full(max(n, 1), 0x7F7F7F7F, np.uint32))
+    cs = DeviceArray.of(np.full(C_ + 1, 0x7F7F7F7F, np.uint32))
+    check(getattr(lib(), fn)(p.ptr if n else None, n, sf_float3.of(lo), float(np.float32(h)),
+                             sf_int3(*[int(v) for v in g]), sk.ptr if n else None, pm.ptr if n else None, cs.ptr,
+                             None))
+    return sk.get(n, np.uint32), pm.get(n, np.uint32), cs.get(C_ + 1, np.uint32)
 
 
 @pytest.mark.parametrize("name", ["small_block", "drop_in_1m", "mixed"])
@@ -78,12 +73,10 @@ def test_grid_build_edge_cases(oracle_mod):
 
 
 def test_grid_build_errors():
-    import torch
-
-    cs = torch.zeros(16, dtype=torch.int32, device="cuda:0")
+    cs = DeviceArray(16 * 4)
     L = lib()
     for g, h in (((0, 2, 2), 0.1), ((2, 2, 2), 0.0), ((2, 2, 2), float("nan"))):
-        rc = L.sf_grid_build(None, 0, sf_float3.of((0, 0, 0)), h, sf_int3(*g), None, None, cs.data_ptr(), None)
+        rc = L.sf_grid_build(None, 0, sf_float3.of((0, 0, 0)), h, sf_int3(*g), None, None, cs.ptr, None)
         assert rc == _abi.SF_E_INVALID
-    rc = L.sf_grid_build(None, 5, sf_float3.of((0, 0, 0)), 0.1, sf_int3(2, 2, 2), None, None, cs.data_ptr(), None)
+    rc = L.sf_grid_build(None, 5, sf_float3.of((0, 0, 0)), 0.1, sf_int3(2, 2, 2), None, None, cs.ptr, None)
     assert rc == _abi.SF_E_INVALID
