@@ -43,7 +43,7 @@
  *   sf_grid_size         <- make_int3((highest - lowest) / kernel_radius)  main.cpp:104, cuda_neighbor_searcher.cu:26
  *   sf_grid_build        <- CudaGNBaseSearcher::BuildGNSearcher  cuda_neighbor_searcher.cu:36-51 on caller arrays
  *   sf_set_emitter / sf_get_emitter <- CudaEmitter + CudaSphEmitterParams (see their declarations)
- *   sf_batch_*, sf_step_graph, sf_*_launch_config, sf_autotune, sf_set_field, sf_diff_*,
+ *   sf_grid_build_noalloc, sf_batch_*, sf_step_graph, sf_*_launch_config, sf_autotune, sf_set_field, sf_diff_*,
  *   sf_set_sort_strategy, sf_set_interface_lanes, sf_step_timed:
  *                           no counterpart (see their declarations)
  *
@@ -361,6 +361,17 @@ int sf_export_bgeo_device(const char* folder, const char* name, const float* pos
  * particle).  n < 2^27, numCells < 2^31. */
 int sf_grid_build(const float* pos3, size_t n, sf_float3 lowest, float cell_size, sf_int3 grid_size,
                   uint32_t* sorted_keys, uint32_t* perm, uint32_t* cell_start, void* stream);
+/* No counterpart in the reference.  sf_grid_build for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same outputs, bit for bit, with the work arrays in
+ * a library-owned device array of 8.25 MiB per GPU (held until the process
+ * ends) instead of an allocation.  At most 1048576 particles (SF_E_CAPACITY
+ * above).  The call allocates and frees nothing, enqueues its kernels on
+ * `stream` and returns without waiting for them.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (other threads,
+ * other streams without ordering) race and must not be issued. */
+int sf_grid_build_noalloc(const float* pos3, size_t n, sf_float3 lowest, float cell_size, sf_int3 grid_size,
+                          uint32_t* sorted_keys, uint32_t* perm, uint32_t* cell_start, void* stream);
 
 /* ---- inflow emitter ------------------------------------------------------
  * sf_emitter_params mirrors CudaSphEmitterParams (cuda_sph_params.h:49-70);

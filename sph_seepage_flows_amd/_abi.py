@@ -159,6 +159,7 @@ _SIG = {
     "sf_export_bgeo": (C.c_int, [_VP, C.c_char_p, C.c_char_p]),
     "sf_particle_view": (C.c_int, [_VP, C.c_int, C.POINTER(_VP)]),
     "sf_grid_build": (C.c_int, [_VP, C.c_size_t, sf_float3, C.c_float, sf_int3, _VP, _VP, _VP, _VP]),
+    "sf_grid_build_noalloc": (C.c_int, [_VP, C.c_size_t, sf_float3, C.c_float, sf_int3, _VP, _VP, _VP, _VP]),
     "sf_update_solver": (C.c_int, [_VP, C.c_float, C.POINTER(C.c_float)]),
     "sf_export_bgeo_device": (C.c_int, [C.c_char_p, C.c_char_p, _VP, _VP, _VP, _VP, C.c_size_t]),
     "sf_batch_create": (C.c_int, [C.POINTER(sf_params), C.POINTER(sf_boundary_params), C.POINTER(sf_solver_options),

@@ -117,3 +117,57 @@ int sf_grid_build(const float* pos3, size_t n, sf_float3 lowest, float cell_size
 }
 
 }  // extern "C"
+
+// ---- the same build without allocation -------------------------------------
+namespace sf {
+namespace {
+// Largest particle count of sf_grid_build_noalloc and its work area:
+// 2 * 2^20 words of key / value ping-pong plus the radix histogram of 2^20 / kRsTile tiles
+// (8.25 MiB per GPU, held until the process ends).
+constexpr size_t kNoAllocMaxParticles = (size_t)1 << 20;
+constexpr size_t kScratchWords = 2 * kNoAllocMaxParticles + 256 * (kNoAllocMaxParticles / kRsTile);
+static_assert(kNoAllocMaxParticles % kRsTile == 0, "whole radix tiles");
+
+__device__ __align__(256) uint32_t scratch[kScratchWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_dev[kMaxDevices] = {};  // the array's address on each device
+
+// The array's device address on the current device, looked up once per device
+// (each GPU holds its own copy at its own address; host code never passes the
+// array's name itself to a kernel).
+int scratch_on_current_device(uint32_t** out) {
+    int dev = -1;
+    SF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_error(SF_E_INVALID, "device id out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_error(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+        g_scratch_dev[dev] = (uint32_t*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return SF_OK;
+}
+}  // namespace
+}  // namespace sf
+
+extern "C" {
+
+int sf_grid_build_noalloc(const float* pos3, size_t n, sf_float3 lowest, float cell_size, sf_int3 grid_size,
+                          uint32_t* sorted_keys, uint32_t* perm, uint32_t* cell_start, void* stream) {
+    int rc;
+    if ((rc = grid_validate(pos3, n, cell_size, grid_size, sorted_keys, perm, cell_start))) return rc;
+    if (n > kNoAllocMaxParticles)
+        return set_error(SF_E_CAPACITY, "sf_grid_build_noalloc takes at most 1048576 particles (use sf_grid_build)");
+    uint32_t* work = nullptr;
+    if ((rc = scratch_on_current_device(&work))) return rc;
+    static_assert(kScratchWords >= 2 * kNoAllocMaxParticles + 256 * ((kNoAllocMaxParticles + kRsTile - 1) / kRsTile),
+                  "work area of grid_build_with");
+    return grid_build_with(work, pos3, n, lowest, cell_size, grid_size, sorted_keys, perm, cell_start,
+                           (hipStream_t)stream);
+}
+
+}  // extern "C"
