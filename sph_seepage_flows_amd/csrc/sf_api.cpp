@@ -379,7 +379,7 @@ int fill_empty(sf_system* h) {
     if (!h->empty_pending) return SF_OK;
     Fields f = h->fields();
     f.nwater = h->wpre.p + h->n_built;
-    launch_fill_empty(f, (int)h->n_built, h->stream);
+    launch_fill_empty(h->K, f, (int)h->n_built, h->stream);
     SF_HIP(hipGetLastError());
     h->empty_pending = false;
     return SF_OK;
@@ -399,7 +399,8 @@ int views_out_bits(sf_system* h, uint32_t bits) {
     if (!bits || !h->uploaded) return SF_OK;
     constexpr uint32_t kEmptySumViews = (1u << SF_FIELD_VOIDAGE) | (1u << SF_FIELD_SATURATION) |
                                         (1u << SF_FIELD_AVG_FLOW_VEL) | (1u << SF_FIELD_ADHESION_FORCE) |
-                                        (1u << SF_FIELD_AVG_ADHESION_FORCE);
+                                        (1u << SF_FIELD_AVG_ADHESION_FORCE) | (1u << SF_FIELD_DENSITY) |
+                                        (1u << SF_FIELD_PRESSURE);
     if (bits & kEmptySumViews) {
         int rc = fill_empty(h);
         if (rc) return rc;
@@ -1056,7 +1057,7 @@ int sf_download(sf_handle h, int field, void* dst, size_t dst_bytes) {
     };
     switch (field) {
     case SF_FIELD_VOIDAGE: case SF_FIELD_SATURATION: case SF_FIELD_AVG_FLOW_VEL: case SF_FIELD_ADHESION_FORCE:
-    case SF_FIELD_AVG_ADHESION_FORCE:
+    case SF_FIELD_AVG_ADHESION_FORCE: case SF_FIELD_DENSITY: case SF_FIELD_PRESSURE:
         if ((rc = fill_empty(h))) return rc;
         break;
     default: break;
@@ -1188,8 +1189,9 @@ int sf_device_buffer(sf_handle h, int field, void** dptr, size_t* stride) {
     default: return set_error(SF_E_NOTFOUND, "field has no device buffer");
     }
     switch (field) {
-    case SF_FIELD_VOIDAGE: case SF_FIELD_SATURATION: case SF_FIELD_AVG_FLOW_VEL:
-    case SF_FIELD_AVG_DRAG_FORCE: case SF_FIELD_AVG_ADHESION_FORCE: case SF_FIELD_ADHESION_FORCE: {
+    case SF_FIELD_VOIDAGE: case SF_FIELD_SATURATION: case SF_FIELD_AVG_FLOW_VEL: case SF_FIELD_DENSITY:
+    case SF_FIELD_PRESSURE: case SF_FIELD_AVG_DRAG_FORCE: case SF_FIELD_AVG_ADHESION_FORCE:
+    case SF_FIELD_ADHESION_FORCE: {
         // the caller reads these buffers directly: every later step writes
         // the empty sums' constants too
         h->full_empty = true;

@@ -233,9 +233,10 @@ void launch_avg_flow(const SFConst& k, const Fields& f, int n, PassCfg c, hipStr
 void launch_zero_counts(const Fields& f, hipStream_t s);
 void launch_interface(const SFConst& k, const Fields& f, int n, hipStream_t s);
 // The empty sums' constants of every unlisted particle's avg flow, adhesion,
-// water voidage / saturation and avgAdhesion (what full_empty writes), over
-// the n particles of the last neighbour build.
-void launch_fill_empty(const Fields& f, int n, hipStream_t s);
+// water voidage / saturation and avgAdhesion, and every grain's (rho_sand, 0)
+// density / pressure (what full_empty writes), over the n particles of the
+// last neighbour build.
+void launch_fill_empty(const SFConst& k, const Fields& f, int n, hipStream_t s);
 void launch_voidage_adhesion(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);
 void launch_sand_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);
 void launch_water_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s);

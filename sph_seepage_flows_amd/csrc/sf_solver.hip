@@ -415,7 +415,10 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
         uint32_t other = 0;
         if (valid) {
             i = (int)ldo(f.plist, nw + t);
-            f.rp[i] = make_float2(k.rho_s, 0.f);
+            // every grain has rho_sand and p = 0; the passes use the constant,
+            // so the field is written here only when it is read directly or
+            // taped, else on request (launch_fill_empty)
+            if (f.full_empty || f.den4) f.rp[i] = make_float2(k.rho_s, 0.f);
             const float4 p = ldo(f.pph, nw + t);
             const uint32_t cb = scene_base<B>(k, f, i);
             int cx, cy, cz;
@@ -547,16 +550,18 @@ __global__ __launch_bounds__(kBlock) void k_interface(SFConst k, Fields f, int n
     if (valid && sand_i) f.iface[t - nw] = listed ? 1 : 0;
     if (valid && !listed) write_no_interface(f, i, sand_i, ldo(f.rp, (uint32_t)i));
 }
-__global__ __launch_bounds__(kBlock) void k_fill_empty(Fields f, int n) {
+__global__ __launch_bounds__(kBlock) void k_fill_empty(SFConst k, Fields f, int n) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= (uint32_t)n) return;
     const uint32_t nw = *f.nwater;
     const bool sand = t >= nw;
+    const int i = (int)ldo(f.plist, t);
+    if (sand) f.rp[i] = make_float2(k.rho_s, 0.f);
     if (sand ? ldo(f.iface, t - nw) : ldo(f.wl, t)) return;
-    write_empty_outputs(f, (int)ldo(f.plist, t), sand);
+    write_empty_outputs(f, i, sand);
 }
-void launch_fill_empty(const Fields& f, int n, hipStream_t s) {
-    if (n > 0) k_fill_empty<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(f, n);
+void launch_fill_empty(const SFConst& k, const Fields& f, int n, hipStream_t s) {
+    if (n > 0) k_fill_empty<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(k, f, n);
 }
 void launch_zero_counts(const Fields& f, hipStream_t s) { k_zero_counts<<<1, 64, 0, s>>>(f.icount); }
 void launch_interface(const SFConst& k, const Fields& f, int n, hipStream_t s) {
@@ -795,7 +800,7 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
     const float mi = -pi.w;
     const v3 xi = xyz(pi);
     const v3 vi = xyz(ldo(f.vr, (uint32_t)i));
-    const float rho_i = ldo(f.rp, (uint32_t)i).x;
+    const float rho_i = k.rho_s;  // the density pass's value for every grain
     const float2 esi = ldo(f.es, (uint32_t)i);
     const float sat_i = esi.y;
     const float twoR = 2.f * k.R;
