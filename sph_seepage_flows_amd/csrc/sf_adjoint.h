@@ -35,9 +35,11 @@ struct AdjFields {
     float4* gflow;  // average flow velocity (sand)
     float4* gadh;   // adhesion force (water)
     float4* gxf;    // avgAdhesion (water) / avgDrag (sand)
-    float4* o4;     // per-pass adjoints of raw sums: P4 (f, vw), P3 water (F, vs), P2 sand (av, vsum)
-    float2* o2;     // P3 sand (num, den), P2 water (sandv, -)
+    float4* o4;     // per-pass adjoints of raw sums: P4 (f, vw), P2 sand (av, vsum)
+    float2* o2;     // P2 water (sandv, -)
     float4* gth;    // per-slot parameter adjoints (cd, c0, cmc, tan(friction angle)), summed over steps
+    float4* o4b;    // P3 water (F, vs): apart from o4, which P4's pairs still read while P3's are formed
+    float2* o2b;    // P3 sand (num, den)
 };
 
 // The reverse of one step (advect, P5, P4, P3, P2, P1) into a; sat_seed (may

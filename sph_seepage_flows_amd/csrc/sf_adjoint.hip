@@ -19,6 +19,7 @@
 // took.
 #include "sf_adjoint.h"
 #include "sf_dual.h"
+#include "sf_nlist.h"
 
 namespace sf {
 
@@ -28,6 +29,19 @@ namespace {
 // of particles) still spread over every CU, and the pair kernels are latency
 // bound (dependent loads of neighbour data), not LDS or occupancy bound.
 constexpr int kAB = 64;
+
+// The pair kernels filter their candidates into per-lane LDS lists
+// (sf_nlist.h) and run the pair bodies over the lists, so a wave executes a
+// body only for kept pairs instead of whenever some lane keeps one.  A lane
+// still takes its candidates in slot order, so its sums keep their order.
+constexpr int kAdjCap = 16;
+#define SF_ADJ_LIST(name)                  \
+    constexpr int NB = kAB, CAP = kAdjCap; \
+    SF_NLIST(name)
+// The loads most pair bodies start with, issued a pair ahead by the flush.
+struct AdjPre {
+    float4 pj, vj, wj;  // position, velocity and one per-kernel adjoint field of j
+};
 
 template <int K>
 SF_DEV void contract3(const G3<Dn<K>>& o, float3 ob, float* g) {
@@ -124,6 +138,16 @@ SF_DEV void boundary_neighbours(const SFConst& k, const TapeStep& t, float4 p, i
 
 SF_DEV float sgn1(float v) { return (float)((0.f < v) - (v < 0.f)); }
 
+// The per-particle steps between two pair passes run as the tail of the
+// earlier pair kernel, in the thread that finishes particle i: they read and
+// write only particle i's adjoints, and no pair kernel reads, for a
+// neighbour, a field that the tail of the same kernel writes (P3's adjoints
+// go to o4b / o2b while P4's pairs still read o4).
+SF_DEV void adj_sand_local(const SFConst& k, const TapeStep& t, const AdjFields& a, int i);
+SF_DEV void adj_voidage_local(const SFConst& k, const TapeStep& t, const AdjFields& a, int i);
+SF_DEV void adj_flow_local(const SFConst& k, const TapeStep& t, const AdjFields& a, int i);
+SF_DEV void adj_density_local(const SFConst& k, const TapeStep& t, const AdjFields& a, int i);
+
 // ---- A0: advect (symplectic Euler, sand damping, water box clamp) ---------
 __global__ __launch_bounds__(kAB) void k_adj_advect(SFConst k, TapeStep t, AdjFields a, const float* sat_seed,
                                                        float dt, int n) {
@@ -206,9 +230,10 @@ __global__ __launch_bounds__(kAB) void k_adj_water(SFConst k, TapeStep t, AdjFie
     const float p_k = t.rp[i].y;
     const float4 xfk = t.xf[i];
     using D8 = Dn<8>;
-    neighbours<L>(k, t, pk, sub, water ? kNeedBoth : kNeedWater, [&](uint32_t j) {
-        if ((int)j == i) return;
-        const float4 pj = t.pm[j];
+    SF_ADJ_LIST(Lst);
+    auto fetch = [&](uint32_t j) { return AdjPre{t.pm[j], t.vr[j], a.gacc[j]}; };
+    auto body = [&](uint32_t j, const AdjPre& q) {
+        const float4 pj = q.pj;
         const float dx = pk.x - pj.x, dy = pk.y - pj.y, dz = pk.z - pj.z;  // x_k - x_j
         if (dx * dx + dy * dy + dz * dz >= k.h2) return;
         if (!water && pj.w < 0.f) return;
@@ -216,9 +241,9 @@ __global__ __launch_bounds__(kAB) void k_adj_water(SFConst k, TapeStep t, AdjFie
             // water-water: the pair term is antisymmetric up to the mass ratio,
             // T(j <- k) = -(m_k / m_j) T(k <- j), so one Jacobian of T(k <- j)
             // serves both sides, contracted with a_k - (m_k / m_j) a_j.
-            const float4 vj = t.vr[j];
+            const float4 vj = q.vj;
             const float2 wtj = t.wt[j];
-            const float3 aj = f3(a.gacc[j]);
+            const float3 aj = f3(q.wj);
             const float c = mk / pj.w;
             const float3 comb = make_float3(ak.x - c * aj.x, ak.y - c * aj.y, ak.z - c * aj.z);
             G3<D8> d = {dvar<8>(dx, 0), dvar<8>(dy, 1), dvar<8>(dz, 2)};
@@ -237,7 +262,7 @@ __global__ __launch_bounds__(kAB) void k_adj_water(SFConst k, TapeStep t, AdjFie
                            grt, gdr_unused);
             add3(gx, gd);
         } else {  // j-side: water j's seepage terms with sand k (k's avgDrag)
-            const float3 aj = f3(a.gacc[j]);
+            const float3 aj = f3(q.wj);
             v3 gd = mk3(0.f, 0.f, 0.f), gdr3 = gd;
             float gp_unused = 0.f, grt_unused = 0.f;
             vjp_water_sand(k, mk3(-dx, -dy, -dz), t.rp[j].y, t.wt[j].x, xyz(xfk), mk, t.rp[i].x,
@@ -245,7 +270,14 @@ __global__ __launch_bounds__(kAB) void k_adj_water(SFConst k, TapeStep t, AdjFie
             add3(gx, -gd.x, -gd.y, -gd.z);
             add3(gdr, gdr3);
         }
+    };
+    neighbours<L>(k, t, pk, sub, water ? kNeedBoth : kNeedWater, [&](uint32_t j) {
+        const float4 pj = t.pm[j];
+        const float dx = pk.x - pj.x, dy = pk.y - pj.y, dz = pk.z - pj.z;
+        const bool keep = (int)j != i && dx * dx + dy * dy + dz * dz < k.h2 && (water || pj.w > 0.f);
+        nl_push2(Lst, keep, j, fetch, body);
     });
+    nl_flush2(Lst, fetch, body);
     if (water) {
         boundary_neighbours<L>(k, t, pk, sub, [&](uint32_t b) {
             const float4 bp = t.bpv[b];
@@ -278,12 +310,11 @@ __global__ __launch_bounds__(kAB) void k_adj_water(SFConst k, TapeStep t, AdjFie
     }
     a.gx[i] = a.gx[i] + gx;
     a.gv[i] = a.gv[i] + gv;
+    adj_sand_local(k, t, a, i);
 }
 
-// ---- A4 (local): sand i's force and vw adjoints, drag, walls ------------------
-__global__ __launch_bounds__(kAB) void k_adj_sand_local(SFConst k, TapeStep t, AdjFields a, int n) {
-    const int i = blockIdx.x * kAB + threadIdx.x;
-    if (i >= n) return;
+// ---- A4 (local, the tail of A5): sand i's force and vw adjoints, drag, walls ----
+SF_DEV void adj_sand_local(const SFConst& k, const TapeStep& t, const AdjFields& a, int i) {
     const float4 p = t.pm[i];
     if (p.w >= 0.f) {
         a.o4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -360,9 +391,10 @@ __global__ __launch_bounds__(kAB) void k_adj_sand_pairs(SFConst k, TapeStep t, A
     const float sat_k = t.es[i].y;
     const float rho_k = t.rp[i].x;
     const float cap_r = twoR + k.cap_srup;
-    auto pair = [&](uint32_t j) {
-        if ((int)j == i) return;
-        const float4 pj = t.pm[j];
+    SF_ADJ_LIST(Lst);
+    auto fetch = [&](uint32_t j) { return AdjPre{t.pm[j], t.vr[j], a.o4[j]}; };
+    auto pair = [&](uint32_t j, const AdjPre& q) {
+        const float4 pj = q.pj;
         const float dx = pj.x - pk.x, dy = pj.y - pk.y, dz = pj.z - pk.z;  // x_j - x_k
         const float r2 = dx * dx + dy * dy + dz * dz;
         if (sand && pj.w < 0.f) {
@@ -371,8 +403,8 @@ __global__ __launch_bounds__(kAB) void k_adj_sand_pairs(SFConst k, TapeStep t, A
             if (r2 >= cap_r * cap_r) return;
             const float dist = sqrtf(r2);
             if (dist >= cap_r) return;
-            const float4 vj = t.vr[j];
-            const float4 oj = a.o4[j];
+            const float4 vj = q.vj;
+            const float4 oj = q.wj;
             const float3 dfb = make_float3(oj.x - fk.x, oj.y - fk.y, oj.z - fk.z);
             if (dist < twoR) {
                 using D7 = Dn<7>;
@@ -411,7 +443,7 @@ __global__ __launch_bounds__(kAB) void k_adj_sand_pairs(SFConst k, TapeStep t, A
                            mk / fmaxf(kEps, rho_k), fk3, ok.w, gd, gr_unused, ge_unused, gp_unused, ga_unused);
             add3(gx, -gd.x, -gd.y, -gd.z);
         } else if (!sand && pj.w < 0.f) {  // j-side: water k acts on sand j
-            const float4 oj = a.o4[j];
+            const float4 oj = q.wj;
             const float mi = -pj.w;
             const float rho_i = t.rp[j].x;
             const float2 rpk = t.rp[i];
@@ -422,11 +454,21 @@ __global__ __launch_bounds__(kAB) void k_adj_sand_pairs(SFConst k, TapeStep t, A
             add3(gadh, ga);
         }
     };
+    const float cap_r2 = cap_r * cap_r;
+    auto filter = [&](uint32_t j) {
+        const float4 pj = t.pm[j];
+        const float dx = pj.x - pk.x, dy = pj.y - pk.y, dz = pj.z - pk.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const bool sj = pj.w < 0.f;
+        const bool keep = (int)j != i && ((sand && sj) ? r2 < cap_r2 : (r2 < k.h2 && sand != sj));
+        nl_push2(Lst, keep, j, fetch, pair);
+    };
     if (sand) {
-        sand_partners<L>(k, t, pk, sub, cap_r + 1e-3f * k.h, pair);
+        sand_partners<L>(k, t, pk, sub, cap_r + 1e-3f * k.h, filter);
     } else {
-        neighbours<L>(k, t, pk, sub, kNeedSand, pair);
+        neighbours<L>(k, t, pk, sub, kNeedSand, filter);
     }
+    nl_flush2(Lst, fetch, pair);
     gx = lsum<L>(gx);
     gv = lsum<L>(gv);
     gth = lsum<L>(gth);
@@ -447,19 +489,18 @@ __global__ __launch_bounds__(kAB) void k_adj_sand_pairs(SFConst k, TapeStep t, A
         a.ges[i].x += geps;
         a.gxf[i] = gadh;  // d loss / d avgAdhesion of water k
     }
+    adj_voidage_local(k, t, a, i);
 }
 
-// ---- A3 (local) -------------------------------------------------------------
-__global__ __launch_bounds__(kAB) void k_adj_voidage_local(SFConst k, TapeStep t, AdjFields a, int n) {
-    const int i = blockIdx.x * kAB + threadIdx.x;
-    if (i >= n) return;
+// ---- A3 (local, the tail of A4's pairs) --------------------------------------
+SF_DEV void adj_voidage_local(const SFConst& k, const TapeStep& t, const AdjFields& a, int i) {
     const float4 p = t.pm[i];
     const float den = t.den4[i].y;
     if (p.w < 0.f) {  // eps = num / den
         const float eb = a.ges[i].x;
         const float eps = t.es[i].x;
-        a.o2[i] = den != 0.f ? make_float2(eb / den, -eb * eps / den) : make_float2(0.f, 0.f);
-        a.o4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        a.o2b[i] = den != 0.f ? make_float2(eb / den, -eb * eps / den) : make_float2(0.f, 0.f);
+        a.o4b[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     } else {  // adh = F, avgAdh = F / vs
         const float3 F = f3(t.adh[i]);
         const float3 ab = f3(a.gadh[i]);
@@ -472,8 +513,8 @@ __global__ __launch_bounds__(kAB) void k_adj_voidage_local(SFConst k, TapeStep t
             o.z += xb.z * iv;
             o.w = -(xb.x * F.x + xb.y * F.y + xb.z * F.z) * iv * iv;
         }
-        a.o4[i] = o;
-        a.o2[i] = make_float2(0.f, 0.f);
+        a.o4b[i] = o;
+        a.o2b[i] = make_float2(0.f, 0.f);
     }
 }
 
@@ -488,23 +529,29 @@ __global__ __launch_bounds__(kAB) void k_adj_voidage_pairs(SFConst k, TapeStep t
     const float mk = fabsf(pk.w);
     float4 gx = make_float4(0.f, 0.f, 0.f, 0.f);
     float gsat = 0.f, geps = 0.f, grho = 0.f;
-    const float2 o2k = a.o2[i];
-    const float4 o4k = a.o4[i];
+    const float2 o2k = a.o2b[i];
+    const float4 o4k = a.o4b[i];
     const float rho_k = t.rp[i].x;
     const float eps_k = t.es[i].x;
     const float sat_k = t.es[i].y;
-    neighbours<L>(k, t, pk, sub, sand ? kNeedWater : kNeedSand, [&](uint32_t j) {
-        const float4 pj = t.pm[j];
-        if ((pj.w < 0.f) == sand) return;
+    SF_ADJ_LIST(Lst);
+    // (rho_j, eps_j, sat_j) in vj; sand k: o4 of water j, water k: o2 of sand j in wj
+    auto fetch = [&](uint32_t j) {
+        const float2 rp = t.rp[j], es = t.es[j];
+        const float2 o2 = sand ? make_float2(0.f, 0.f) : a.o2b[j];
+        return AdjPre{t.pm[j], make_float4(rp.x, es.x, es.y, 0.f),
+                      sand ? a.o4b[j] : make_float4(o2.x, o2.y, 0.f, 0.f)};
+    };
+    auto body = [&](uint32_t j, const AdjPre& q) {
+        const float4 pj = q.pj;
         const float dx = pk.x - pj.x, dy = pk.y - pj.y, dz = pk.z - pj.z;  // x_k - x_j
         const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 >= k.h2) return;
         const v3 d = mk3(dx, dy, dz);
         if (sand) {  // j is water
-            const float Vj = pj.w / t.rp[j].x;
-            add3(gx, dpoly6(k, d, r2) * ((o2k.x * t.es[j].x + o2k.y) * Vj));  // i-side: sand voidage sums
+            const float Vj = pj.w / q.vj.x;
+            add3(gx, dpoly6(k, d, r2) * ((o2k.x * q.vj.y + o2k.y) * Vj));  // i-side: sand voidage sums
             // j-side: F_j and vs_j of water j with sand k as neighbour; dp = x_j - x_k
-            const float4 oj = a.o4[j];
+            const float4 oj = q.wj;
             v3 gd = mk3(0.f, 0.f, 0.f);
             vjp_adhesion(k, mk3(-dx, -dy, -dz), sat_k, pj.w, mk, mk3(oj.x, oj.y, oj.z), oj.w, gd, gsat);
             add3(gx, -gd.x, -gd.y, -gd.z);
@@ -512,11 +559,11 @@ __global__ __launch_bounds__(kAB) void k_adj_voidage_pairs(SFConst k, TapeStep t
             {  // i-side: F_k and vs_k
                 v3 gd = mk3(0.f, 0.f, 0.f);
                 float gsat_unused = 0.f;
-                vjp_adhesion(k, d, t.es[j].y, mk, -pj.w, mk3(o4k.x, o4k.y, o4k.z), o4k.w, gd, gsat_unused);
+                vjp_adhesion(k, d, q.vj.z, mk, -pj.w, mk3(o4k.x, o4k.y, o4k.z), o4k.w, gd, gsat_unused);
                 add3(gx, gd);
             }
             // j-side: sand j's voidage num += eps_k V_k W, den += V_k W
-            const float2 oj = a.o2[j];
+            const float2 oj = make_float2(q.wj.x, q.wj.y);
             const float Vk = mk / rho_k;
             const float w = k.w_coef * (k.h2 - r2) * (k.h2 - r2) * (k.h2 - r2);
             const float c = oj.x * eps_k + oj.y;
@@ -524,7 +571,14 @@ __global__ __launch_bounds__(kAB) void k_adj_voidage_pairs(SFConst k, TapeStep t
             grho += c * (-mk / (rho_k * rho_k)) * w;
             add3(gx, dpoly6(k, d, r2) * (c * Vk));
         }
+    };
+    neighbours<L>(k, t, pk, sub, sand ? kNeedWater : kNeedSand, [&](uint32_t j) {
+        const float4 pj = t.pm[j];
+        const float dx = pk.x - pj.x, dy = pk.y - pj.y, dz = pk.z - pj.z;
+        const bool keep = (pj.w < 0.f) != sand && dx * dx + dy * dy + dz * dz < k.h2;
+        nl_push2(Lst, keep, j, fetch, body);
     });
+    nl_flush2(Lst, fetch, body);
     gx = lsum<L>(gx);
     gsat = lsum<L>(gsat);
     geps = lsum<L>(geps);
@@ -537,12 +591,11 @@ __global__ __launch_bounds__(kAB) void k_adj_voidage_pairs(SFConst k, TapeStep t
         a.ges[i].x += geps;
         a.grp[i].x += grho;
     }
+    adj_flow_local(k, t, a, i);
 }
 
-// ---- A2 (local): rho~ factors of water, average-flow normalisation of sand ----
-__global__ __launch_bounds__(kAB) void k_adj_flow_local(SFConst k, TapeStep t, AdjFields a, int n) {
-    const int i = blockIdx.x * kAB + threadIdx.x;
-    if (i >= n) return;
+// ---- A2 (local, the tail of A3's pairs): rho~ factors of water, sand average flow
+SF_DEV void adj_flow_local(const SFConst& k, const TapeStep& t, const AdjFields& a, int i) {
     const float4 p = t.pm[i];
     if (p.w > 0.f) {
         const float2 rp = t.rp[i];
@@ -596,30 +649,37 @@ __global__ __launch_bounds__(kAB) void k_adj_flow_pairs(SFConst k, TapeStep t, A
     const float rho_k = t.rp[i].x;
     const float4 vk = t.vr[i];
     const float inv_rho_s = k.rho_s != 0.f ? 1.f / k.rho_s : 0.f;
-    neighbours<L>(k, t, pk, sub, sand ? kNeedWater : kNeedSand, [&](uint32_t j) {
-        const float4 pj = t.pm[j];
-        if ((pj.w < 0.f) == sand) return;
+    SF_ADJ_LIST(Lst);
+    auto fetch = [&](uint32_t j) { return AdjPre{t.pm[j], t.vr[j], a.o4[j]}; };
+    auto body = [&](uint32_t j, const AdjPre& q) {
+        const float4 pj = q.pj;
         const float dx = pk.x - pj.x, dy = pk.y - pj.y, dz = pk.z - pj.z;  // x_k - x_j
         const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 >= k.h2) return;
         const v3 dw = dpoly6(k, mk3(dx, dy, dz), r2);
         const float w = k.w_coef * (k.h2 - r2) * (k.h2 - r2) * (k.h2 - r2);
         if (!sand) {  // k water, j sand
             add3(gx, dw * (o2k.x * (-pj.w * inv_rho_s)));  // i-side: sandv_k
-            const float4 oj = a.o4[j];  // j-side: sand j's vsum / av with water k
+            const float4 oj = q.wj;  // j-side: sand j's vsum / av with water k
             const float Vk = mk / rho_k;
             const float g = oj.x * vk.x + oj.y * vk.y + oj.z * vk.z + oj.w;
             add3(gx, dw * (g * Vk));
             grho += g * (-mk / (rho_k * rho_k)) * w;
             add3(gv, oj.x * Vk * w, oj.y * Vk * w, oj.z * Vk * w);
         } else {  // k sand, j water
-            const float4 vj = t.vr[j];
+            const float4 vj = q.vj;
             const float Vj = pj.w / t.rp[j].x;
             const float g = o4k.x * vj.x + o4k.y * vj.y + o4k.z * vj.z + o4k.w;
             add3(gx, dw * (g * Vj));  // i-side: vsum_k, av_k
             add3(gx, dw * (a.o2[j].x * (mk * inv_rho_s)));  // j-side: sandv_j
         }
+    };
+    neighbours<L>(k, t, pk, sub, sand ? kNeedWater : kNeedSand, [&](uint32_t j) {
+        const float4 pj = t.pm[j];
+        const float dx = pk.x - pj.x, dy = pk.y - pj.y, dz = pk.z - pj.z;
+        const bool keep = (pj.w < 0.f) != sand && dx * dx + dy * dy + dz * dz < k.h2;
+        nl_push2(Lst, keep, j, fetch, body);
     });
+    nl_flush2(Lst, fetch, body);
     gx = lsum<L>(gx);
     gv = lsum<L>(gv);
     grho = lsum<L>(grho);
@@ -627,12 +687,11 @@ __global__ __launch_bounds__(kAB) void k_adj_flow_pairs(SFConst k, TapeStep t, A
     a.gx[i] = a.gx[i] + gx;
     a.gv[i] = a.gv[i] + gv;
     if (!sand) a.grp[i].x += grho;
+    adj_density_local(k, t, a, i);
 }
 
-// ---- A1: density and EOS ------------------------------------------------------
-__global__ __launch_bounds__(kAB) void k_adj_density_local(SFConst k, TapeStep t, AdjFields a, int n) {
-    const int i = blockIdx.x * kAB + threadIdx.x;
-    if (i >= n) return;
+// ---- A1: density and EOS (the EOS step is the tail of A2's pairs) --------------
+SF_DEV void adj_density_local(const SFConst& k, const TapeStep& t, const AdjFields& a, int i) {
     if (t.pm[i].w < 0.f) {
         a.grp[i] = make_float2(0.f, 0.f);
         return;
@@ -747,15 +806,12 @@ template <int L>
 void launch_pairs_step(const SFConst& k, const TapeStep& t, const AdjFields& a, const float* sat_seed, float dt, int n,
                        hipStream_t s) {
     const int g = blocks(n), gl = blocks(n * L);
+    // each pair kernel ends with the next per-particle step (see adj_sand_local)
     k_adj_advect<<<g, kAB, 0, s>>>(k, t, a, sat_seed, dt, n);
     k_adj_water<L><<<gl, kAB, 0, s>>>(k, t, a, n);
-    k_adj_sand_local<<<g, kAB, 0, s>>>(k, t, a, n);
     k_adj_sand_pairs<L><<<gl, kAB, 0, s>>>(k, t, a, n);
-    k_adj_voidage_local<<<g, kAB, 0, s>>>(k, t, a, n);
     k_adj_voidage_pairs<L><<<gl, kAB, 0, s>>>(k, t, a, n);
-    k_adj_flow_local<<<g, kAB, 0, s>>>(k, t, a, n);
     k_adj_flow_pairs<L><<<gl, kAB, 0, s>>>(k, t, a, n);
-    k_adj_density_local<<<g, kAB, 0, s>>>(k, t, a, n);
     k_adj_density_pairs<L><<<gl, kAB, 0, s>>>(k, t, a, n);
 }
 }  // namespace

@@ -42,8 +42,8 @@ struct DiffState {
     StateCopy fin;              // state after the window
     DevBuf<float2> fin_rp, fin_es;  // density/pressure, voidage/saturation of the window's last step
     std::vector<StepTapeBufs> tape;
-    DevBuf<float4> gx[2], gv[2], gacc, gflow, gadh, gxf, o4, gth, tmp4;
-    DevBuf<float2> grp, ges, gwt, o2;
+    DevBuf<float4> gx[2], gv[2], gacc, gflow, gadh, gxf, o4, o4b, gth, tmp4;
+    DevBuf<float2> grp, ges, gwt, o2, o2b;
     DevBuf<float> sat_seed, tmpf;
 };
 void diff_release(DiffState* d) { delete d; }
@@ -140,7 +140,7 @@ int alloc_diff(sf_system* h) {
     for (int q = 0; q < 2; ++q)
         if ((rc = d->gx[q].alloc(n)) || (rc = d->gv[q].alloc(n))) return rc;
     if ((rc = d->gacc.alloc(n)) || (rc = d->gflow.alloc(n)) || (rc = d->gadh.alloc(n)) || (rc = d->gxf.alloc(n)) ||
-        (rc = d->o4.alloc(n)) || (rc = d->gth.alloc(n)) || (rc = d->tmp4.alloc(n)) || (rc = d->grp.alloc(n)) ||
+        (rc = d->o4.alloc(n)) || (rc = d->o4b.alloc(n)) || (rc = d->o2b.alloc(n)) || (rc = d->gth.alloc(n)) || (rc = d->tmp4.alloc(n)) || (rc = d->grp.alloc(n)) ||
         (rc = d->ges.alloc(n)) || (rc = d->gwt.alloc(n)) || (rc = d->o2.alloc(n)) || (rc = d->sat_seed.alloc(n)) ||
         (rc = d->tmpf.alloc(3 * n)) || (rc = d->fin_rp.alloc(n)) || (rc = d->fin_es.alloc(n)))
         return rc;
@@ -304,7 +304,7 @@ int sf_diff_backward(sf_handle h, const float* pos_bar3, const float* vel_bar3, 
             TapeStep t{T.pm.p, T.vr.p, T.rp.p, T.es.p, T.wt.p, T.flow.p, T.adh.p, T.xf.p, T.acc.p, T.den4.p,
                        T.ct.p, h->bpv.p, h->bcell.p, h->bnear.p};
             AdjFields a{d->gx[cur].p, d->gv[cur].p, d->gacc.p, d->grp.p, d->ges.p, d->gwt.p, d->gflow.p,
-                        d->gadh.p, d->gxf.p, d->o4.p, d->o2.p, d->gth.p};
+                        d->gadh.p, d->gxf.p, d->o4.p, d->o2.p, d->gth.p, d->o4b.p, d->o2b.p};
             launch_adj_step(h->K, t, a, q == d->steps - 1 ? d->sat_seed.p : nullptr, d->dt, n, s);
             launch_adj_scatter(T.perm.p, d->gx[cur].p, d->gv[cur].p, d->gx[cur ^ 1].p, d->gv[cur ^ 1].p, n, s);
             cur ^= 1;
