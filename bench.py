@@ -32,7 +32,7 @@ PROFILE_STEPS = 10  # untimed steps with an event between every two passes
 HBM_PEAK_GBS = 8000.0
 
 
-def pass_bytes(n_w: int, n_s: int, nb: int, cells: int, passes: int, multi: bool, listed=(0, 0)) -> dict:
+def pass_bytes(n_w: int, n_s: int, nb: int, cells: int, multi: bool, listed=(0, 0)) -> dict:
     """Algorithmic HBM bytes per launch of each pass: every array element the
     pass must touch, read or written once (neighbour re-reads are served
     on-chip and not counted).  Layout: pm/vr/cs/flow/adh/xf/acc float4 (16 B),
@@ -45,13 +45,21 @@ def pass_bytes(n_w: int, n_s: int, nb: int, cells: int, passes: int, multi: bool
     aux = 24 if multi else 0
     lw, ls = listed
     return {
-        "hash": n * (16 + 4),
-        "sort": n * (4 + 16) * passes - n * 4,          # upsweep key read + key/value read+write per pass
-        "gather": n * (4 + 2 * (48 + aux)) + n * 16 * 2 + 4 * (n + 1),  # + water-prefix scan (2 reads of pm.w)
-        "cell_start": 4 * (cells + 1) * 3 + 4 * n,                          # cellStart + (start, prefix) table
-        # plist, own position (water: pm, sand: pph), water candidates' pph, rho/p;
+        # stepping loop: the advect computes the next keys (below), the stable
+        # sort is the incremental one (change flags and both prefixes, the
+        # newcomer lists, the placement: keys, prefixes, permutation, sorted
+        # keys; per cell the newcomer counts and starts)
+        "sort": n * 44 + 12 * (cells + 1),
+        # perm, persistent state read by perm and written, water flag; the
+        # water prefix then reads the flags and writes prefix + phase list
+        "gather": n * (4 + 2 * (48 + aux) + 1) + n * (1 + 4 + 4),
+        # per cell: old start, stayer / newcomer prefixes, water prefix read,
+        # cellStart, (start, prefix) table and newcomer counter written; per
+        # particle: phase list + position read, position in phase order written
+        "cell_start": (cells + 1) * (16 + 16) + n * (4 + 16 + 16),
+        # plist, own position (water: pm, sand: pph), water candidates' pph, rho/p (water);
         # unlisted: wt or es + flag; listed: velocity, (rho, p), list entry
-        "density": n_w * (4 + 16 + 16 + 8 + 1) + n_s * (4 + 16 + 8 + 1) + (n_w - lw) * 8 + (n_s - ls) * 8
+        "density": n_w * (4 + 16 + 16 + 8 + 1) + n_s * (4 + 16 + 1) + (n_w - lw) * 8 + (n_s - ls) * 8
                    + (lw + ls) * (16 + 4) + lw * 16 + 8 * (cells + 1) + 4 * (cells + 1) + cells + nb * 16,
         # listed only: list + slot, own position, phase-index position / velocity
         # (/ rho for water), flow, es, water wt and voidage, sand saturation
@@ -61,16 +69,19 @@ def pass_bytes(n_w: int, n_s: int, nb: int, cells: int, passes: int, multi: bool
         # sand voidage + adhesion 0, water adhesion + avgAdhesion (slot and phase index)
         "voidage_adhesion": lw * (4 + 4 + 16 + 16 + 16 + 48) + ls * (4 + 4 + 16 + 16 + 16 + 4 + 16)
                             + 8 * (cells + 1),
-        "sand_momentum": n * (16 + 16 + 8 + 8) + n_w * 16 + n_s * (16 + 16 + 16 + 16 + 16) + 8 * (cells + 1),
-        "water_momentum": n * (16 + 16 + 8) + n_s * 16 + n_w * (8 + 16 + 16) + nb * 16 + 8 * (cells + 1) + 4 * (cells + 1),
-        "advect": n * (16 * 3 + 16 * 2) + n_s * 16,
+        # per grain: plist, position, velocity, es, colour read and written,
+        # interface flag, avgDrag and acc written; the sand candidates'
+        # phase-order positions touched once; the cell table
+        "sand_momentum": n_s * (4 + 16 + 16 + 8 + 16 + 16 + 1 + 16 + 16) + n_s * 16 + 8 * (cells + 1),
+        # every position once (own and candidates), water velocity and wt,
+        # sand avgDrag; per water particle plist, rho/p and acc written; the
+        # cell and boundary tables, boundary particles
+        "water_momentum": n * 16 + n_w * (16 + 8) + n_s * 16 + n_w * (4 + 8 + 16) + 8 * (cells + 1)
+                          + 4 * (cells + 1) + cells + nb * 16,
+        # position, velocity, acc read; position, velocity written; sand colour;
+        # the next key written and the current sorted key read (change flag)
+        "advect": n * (16 * 3 + 16 * 2) + n_s * 16 + n * (4 + 4),
     }
-
-
-def radix_passes(cells: int) -> int:
-    """Digit passes of the key sort: ceil(bits(cells - 1) / 8), at least 1."""
-    bits = max(1, int(cells - 1).bit_length())
-    return (bits + 7) // 8
 
 
 def interface_counts(pos, lab, b) -> tuple[int, int]:
@@ -413,7 +424,7 @@ def main() -> int:
               f"{json.dumps({k: round(v, 4) for k, v in pass_ms.items()})}", file=sys.stderr, flush=True)
         n_s = int(np.count_nonzero(scene.label == 1))
         n_w = n - n_s
-        pb = pass_bytes(n_w, n_s, nb, cells, radix_passes(cells), scene.physics.sf_type == 1, listed)
+        pb = pass_bytes(n_w, n_s, nb, cells, scene.physics.sf_type == 1, listed)
         ach = pb[dom] / (dom_ms * 1e-3) / 1e9
         step_bytes = sum(pb.values())
         traffic = None

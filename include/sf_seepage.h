@@ -44,7 +44,7 @@
  *   sf_grid_build        <- CudaGNBaseSearcher::BuildGNSearcher  cuda_neighbor_searcher.cu:36-51 on caller arrays
  *   sf_set_emitter / sf_get_emitter <- CudaEmitter + CudaSphEmitterParams (see their declarations)
  *   sf_grid_build_noalloc, sf_batch_*, sf_step_graph, sf_*_launch_config, sf_autotune, sf_set_field, sf_diff_*,
- *   sf_set_sort_strategy, sf_set_interface_lanes, sf_step_timed:
+ *   sf_set_sort_strategy, sf_set_prefix_scan_threshold, sf_set_interface_lanes, sf_step_timed:
  *                           no counterpart (see their declarations)
  *
  * Threading: a handle is single-threaded; all its GPU work is ordered on the
@@ -446,6 +446,15 @@ int sf_step_graph(sf_handle h, float dt, int n_steps);
 #define SF_SORT_RADIX 0
 #define SF_SORT_INCREMENTAL 1
 int sf_set_sort_strategy(sf_handle h, int strategy);
+/* Largest number of 2048-slot tiles up to which the slot prefixes of the
+ * incremental sort and of the water prefix sum the earlier tiles' counts
+ * inside each prefix block; above it a one-block scan launch runs in between
+ * (default 4096 tiles, about 8.4M slots).  Both branches give the same
+ * prefixes; the switch lets a test run both on one input.  Process-wide (no
+ * handle); graphs captured before keep the branch they were captured with.
+ * tiles < 0 restores the default.  Returns the previous threshold.  No
+ * counterpart in the reference. */
+int sf_set_prefix_scan_threshold(int tiles);
 /* Launch shape of a neighbour pass (SF_PASS_DENSITY .. SF_PASS_WATER_MOMENTUM):
  * workgroup size nb in {64, 128, 256}; LDS candidate-list depth cap in {8, 16}
  * (sand / water momentum only).  Results do not depend on the shape. */

@@ -171,6 +171,7 @@ _SIG = {
     "sf_get_launch_config": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "sf_autotune": (C.c_int, [_VP, C.c_int, _VP, _VP]),
     "sf_set_sort_strategy": (C.c_int, [_VP, C.c_int]),
+    "sf_set_prefix_scan_threshold": (C.c_int, [C.c_int]),
     "sf_set_interface_lanes": (C.c_int, [_VP, C.c_int]),
     "sf_set_emitter": (C.c_int, [_VP, C.POINTER(sf_emitter_params)]),
     "sf_get_emitter": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),

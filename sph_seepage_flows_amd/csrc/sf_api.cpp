@@ -1416,6 +1416,8 @@ int sf_set_interface_lanes(sf_handle h, int lanes) {
     return SF_OK;
 }
 
+int sf_set_prefix_scan_threshold(int tiles) { return set_direct_tiles(tiles); }
+
 int sf_set_sort_strategy(sf_handle h, int strategy) {
     int rc;
     if ((rc = check_handle(h))) return rc;

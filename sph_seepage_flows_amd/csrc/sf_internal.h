@@ -120,6 +120,10 @@ struct GatherArgs {
 // Reorders the persistent state by perm, then wpre[0..n] = water count among
 // slots [0, i) and plist = water slots then sand slots, each in slot order.
 // wpart holds wprefix_blocks(n + 1) + 1 entries.
+// Largest tile count up to which the prefix launches sum the earlier tiles
+// themselves (above it a one-block scan runs in between); < 0 restores the
+// default.  Returns the previous value.  Process-wide.
+int set_direct_tiles(int tiles);
 void launch_gather_wprefix(const GatherArgs& a, int n, uint32_t* wpart, uint32_t* wpre, uint32_t* plist,
                            hipStream_t s);
 void launch_boundary_gather(const uint32_t* perm, const float4* bp_in, float4* bp_out, int nb, hipStream_t s);

@@ -404,6 +404,7 @@ SF_DEV void tile_sum(V v, int n, int tile, uint32_t* __restrict__ part_out) {
 // itself instead of waiting for a one-block scan launch: the sum of cnt[q]
 // over q in [lo, b) (before) and over [lo, hi) (all), by the whole block.
 constexpr int kDirectTiles = 4096;
+int g_direct_tiles = kDirectTiles;  // sf_set_prefix_scan_threshold
 SF_DEV void block_sums(const uint32_t* __restrict__ cnt, uint32_t lo, uint32_t b, uint32_t hi, uint32_t& before,
                        uint32_t& all) {
     __shared__ uint32_t red[2][kBlock / 64];
@@ -502,7 +503,7 @@ void launch_vprefix2(V1 v1, int n1, uint32_t* out1, V2 v2, int n2, uint32_t* out
     // receives the total from the prefix of the blocks before it
     const int nb1 = wprefix_blocks(n1 + 1), nb2 = wprefix_blocks(n2 + 1);
     k_vsum2<<<nb1 + nb2, kBlock, 0, s>>>(v1, n1, nb1, v2, n2, part);
-    const int scanned = nb1 + nb2 > kDirectTiles ? 1 : 0;
+    const int scanned = nb1 + nb2 > g_direct_tiles ? 1 : 0;
     if (scanned) k_rs_scan2<<<1, 1024, 0, s>>>(part, nb1, nb2, nullptr);
     k_vprefix2<<<nb1 + nb2, kBlock, 0, s>>>(v1, n1, nb1, out1, v2, n2, out2, part, scanned);
 }
@@ -568,11 +569,16 @@ __global__ __launch_bounds__(kBlock) void k_wprefix_plist(const uint8_t* __restr
         plist[water ? pre : nwater + (uint32_t)i - pre] = (uint32_t)i;
     });
 }
+int set_direct_tiles(int tiles) {
+    const int prev = g_direct_tiles;
+    g_direct_tiles = tiles < 0 ? kDirectTiles : tiles;
+    return prev;
+}
 void launch_gather_wprefix(const GatherArgs& a, int n, uint32_t* wpart, uint32_t* wpre, uint32_t* plist,
                            hipStream_t s) {
     const int nb = wprefix_blocks(n + 1);
     k_gather<<<nb, kBlock, 0, s>>>(a, n, wpart);
-    const int scanned = nb > kDirectTiles ? 1 : 0;
+    const int scanned = nb > g_direct_tiles ? 1 : 0;
     if (scanned) k_rs_scan2<<<1, 1024, 0, s>>>(wpart, nb, 0, wpart + nb);
     k_wprefix_plist<<<nb, kBlock, 0, s>>>(a.wflag, n, wpart, scanned, wpre, plist);
 }

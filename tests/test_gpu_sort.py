@@ -65,3 +65,29 @@ def test_incremental_sort_after_reupload():
     a.step(DT, 5)
     b.step(DT, 5)
     np.testing.assert_array_equal(a.get("pos"), b.get("pos"))
+
+
+def test_prefix_scan_branches_are_identical():
+    """The slot prefixes of the incremental sort and of the water prefix
+    have two branches: each prefix block sums the earlier tiles' counts
+    itself (up to 4096 tiles of 2048 slots, every scene here) or a one-block
+    scan launch runs in between (above; threshold 0 forces it).  Both must
+    give the same structures and trajectories, bit for bit."""
+    from sph_seepage_flows_amd import _abi
+    lib = _abi.lib()
+    makers = {
+        "mixed": lambda: SFSystem.from_scene(scenes.mixed(vel_sigma=1.0)),
+        "batched": lambda: BatchSystem.from_scenes(scenes.ensemble(seed=21, n_scenes=3, n_min=2000, n_max=5000)),
+    }
+    for name, make in makers.items():
+        a = make()
+        a.step(DT, 6)
+        prev = lib.sf_set_prefix_scan_threshold(0)
+        try:
+            assert prev == 4096
+            b = make()
+            b.step(DT, 6)
+        finally:
+            lib.sf_set_prefix_scan_threshold(-1)
+        for f in ("sort_perm", "sorted_keys", "cell_start", "pos", "vel", "density", "saturation"):
+            np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f"{name}: {f}")
