@@ -198,8 +198,8 @@ SF_DEV void staged_walk(const SFConst& k, const Fields& f, float4* __restrict__ 
 // loads them once: when every particle of the wave sits in one grid column
 // (most waves of an ordered region), the lanes walk the same 9 columns, and
 // a column whose union range (first lane's start to last lane's end) fits
-// KS candidates is staged in LDS with one coalesced load and filtered from
-// there.  Other columns, and waves spread over several columns, filter from
+// KS candidates is staged in LDS with coalesced loads (one per lane up to
+// 64) and filtered from there.  Other columns, and waves spread over several columns, filter from
 // memory per lane.  Either way a lane's list gets the same entries in the
 // same order, so which way a column goes does not change the sums.  Must be
 // called by all lanes of the wave.
@@ -207,7 +207,7 @@ template <uint32_t S, uint32_t KS, int NB, int CAP, class Test, class Fetch, cla
 SF_DEV void column_filter(const SFConst& k, const Fields& f, float4* __restrict__ stage, bool active, uint32_t cb,
                           int cx, int cy, int cz, NList<NB, CAP>& L, Test&& test, Fetch&& fetch, Apply&& apply) {
     static_assert(CAP >= 4, "a round pushes up to four candidates");
-    static_assert(KS <= 64, "one staged candidate per lane");
+    static_assert(KS <= 128, "at most two staged candidates per lane");
     const int lane = (int)(threadIdx.x & 63u);
     const uint32_t sub = (uint32_t)lane % S;
     const uint64_t act = __ballot(active);
@@ -225,7 +225,7 @@ SF_DEV void column_filter(const SFConst& k, const Fields& f, float4* __restrict_
             const uint32_t us = lane_value(rs, lead), ue = lane_value(re, last);
             if (us >= ue) return;
             if (ue - us <= KS) {
-                if ((uint32_t)lane < ue - us) stage[lane] = ldo(f.pm, us + (uint32_t)lane);
+                for (uint32_t q = (uint32_t)lane; q < ue - us; q += 64u) stage[q] = ldo(f.pm, us + q);
                 wave_sync();
                 uint32_t j = rs + sub;
                 for (; j + 3u * S < re; j += 4u * S) {
@@ -909,7 +909,7 @@ void launch_sand_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, h
 // its own list (column_filter, which stages the candidates in LDS where the
 // wave's ranges overlap), and the partial sums are combined with shuffles.
 constexpr int kWmL = 4;
-constexpr uint32_t kWmStage = 48;  // longest union range staged per wave
+constexpr uint32_t kWmStage = 64;  // longest union range staged per wave
 template <bool B, int NB, int CAP>
 __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int n) {
     SF_NLIST(L);
