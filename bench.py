@@ -9,8 +9,10 @@ the five solver passes + advect over all particles, inputs resident in HBM.
 Prints ONE JSON line.  `roofline` is for the dominant pass: its algorithmic
 HBM bytes per launch (byte model below) divided by its mean duration, timed
 with HIP events on the stream the passes run on, inside the timed region
-(events around that pass only; an untimed profiled run before it gives the
-per-pass breakdown `pass_ms_profiled` and picks the pass).
+(events around that pass on every EVENT_EVERY-th step only, since each event
+between two kernels costs the stream a few microseconds; an untimed profiled
+run before it gives the per-pass breakdown `pass_ms_profiled` and picks the
+pass).
 `cpu_baseline` times the FP32 C++ oracle (a restatement of the reference's
 kernels; the reference itself has no CPU path) on the same scene on the host
 cores, for a bounded number of steps.
@@ -29,6 +31,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "coupled particle-steps/s @1M particles (%HBM roofline); adjoint/forward time"
 PROFILE_STEPS = 10  # untimed steps with an event between every two passes
+EVENT_EVERY = 10  # timed steps: events around the dominant pass on every 10th step
 HBM_PEAK_GBS = 8000.0
 
 
@@ -312,14 +315,14 @@ def mixed_bench(scene, dt: float, steps: int) -> dict:
     the reference driver's loop from the initial state): the water column has
     run into the sand pile, so every neighbour pass carries interface work.
     Then the per-pass breakdown of 10 profiled steps and `steps` steps timed
-    with events around the water momentum pass only."""
+    with events around the water momentum pass (every EVENT_EVERY-th step) only."""
     import sph_seepage_flows_amd as sf
 
     g = sf.SFSystem.from_scene(scene, device=0)
     for _ in range(6):
         g.update_frame(1.0 / 30.0)
     _, passes = g.step_profiled(dt, PROFILE_STEPS, per_pass=True)
-    total_ms, _ = g.step_timed(dt, steps, "water_momentum")
+    total_ms, _ = g.step_timed(dt, steps, "water_momentum", every=min(EVENT_EVERY, steps))
     n = g.sizes()[0]
     g.close()
     return {"workload": f"{scene.name} after 0.2 s (six 1/30 s frames)", "steps": steps,
@@ -388,12 +391,13 @@ def main() -> int:
             g.step(dt, args.warmup)
         # untimed: per-pass breakdown (an event between every two passes) and
         # the dominant pass; the timed region then records events around that
-        # pass only, since each event between kernels costs the stream ~5 us
+        # pass only, on every EVENT_EVERY-th step, since each event between
+        # kernels costs the stream a few microseconds
         _, pass_ms = g.step_profiled(dt, PROFILE_STEPS, per_pass=True)
         dom = max(pass_ms, key=pass_ms.get)
         g.synchronize()
         t0 = time.perf_counter()
-        total_ms, dom_ms = g.step_timed(dt, args.steps, dom)
+        total_ms, dom_ms = g.step_timed(dt, args.steps, dom, every=min(EVENT_EVERY, args.steps))
         wall = time.perf_counter() - t0
         # neighbour-pair throughput of the dominant pass: its particles' pairs
         # within h (self included: the passes evaluate it, with a zero term)

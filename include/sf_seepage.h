@@ -44,7 +44,7 @@
  *   sf_grid_build        <- CudaGNBaseSearcher::BuildGNSearcher  cuda_neighbor_searcher.cu:36-51 on caller arrays
  *   sf_set_emitter / sf_get_emitter <- CudaEmitter + CudaSphEmitterParams (see their declarations)
  *   sf_grid_build_noalloc, sf_batch_*, sf_step_graph, sf_*_launch_config, sf_autotune, sf_set_field, sf_diff_*,
- *   sf_set_sort_strategy, sf_set_prefix_scan_threshold, sf_set_interface_lanes, sf_step_timed:
+ *   sf_set_sort_strategy, sf_set_prefix_scan_threshold, sf_set_interface_lanes, sf_step_timed(_every):
  *                           no counterpart (see their declarations)
  *
  * Threading: a handle is single-threaded; all its GPU work is ordered on the
@@ -301,6 +301,10 @@ int sf_step_profiled(sf_handle h, float dt, int n_steps, float* pass_ms, float* 
  * the pass's mean duration per step, *total_ms the whole region.  No
  * counterpart in the reference. */
 int sf_step_timed(sf_handle h, float dt, int n_steps, int pass, float* pass_ms, float* total_ms);
+/* sf_step_timed with the pass's events on steps 0, every, 2 every, ... only:
+ * *pass_ms is the mean over those steps, and the other steps run without
+ * events.  every = 1 is sf_step_timed.  No counterpart in the reference. */
+int sf_step_timed_every(sf_handle h, float dt, int n_steps, int pass, int every, float* pass_ms, float* total_ms);
 int sf_size(sf_handle h, size_t* n, size_t* nb, size_t* num_cells);
 
 /* ---- data access -------------------------------------------------------- */

@@ -176,6 +176,8 @@ _SIG = {
     "sf_set_emitter": (C.c_int, [_VP, C.POINTER(sf_emitter_params)]),
     "sf_get_emitter": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "sf_step_timed": (C.c_int, [_VP, C.c_float, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "sf_step_timed_every": (C.c_int, [_VP, C.c_float, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float),
+                                      C.POINTER(C.c_float)]),
     "sf_diff_forward": (C.c_int, [_VP, C.c_float, C.c_int, C.c_int]),
     "sf_diff_reset": (C.c_int, [_VP, _VP, C.c_int]),
     "sf_diff_get": (C.c_int, [_VP, C.c_int, _VP, C.c_int]),

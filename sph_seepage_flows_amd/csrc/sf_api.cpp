@@ -758,13 +758,19 @@ int sf_step_profiled(sf_handle h, float dt, int n_steps, float* pass_ms, float* 
 }
 
 int sf_step_timed(sf_handle h, float dt, int n_steps, int pass, float* pass_ms, float* total_ms) {
+    return sf_step_timed_every(h, dt, n_steps, pass, 1, pass_ms, total_ms);
+}
+
+int sf_step_timed_every(sf_handle h, float dt, int n_steps, int pass, int every, float* pass_ms, float* total_ms) {
     int rc;
     if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
     if (!h->uploaded) return set_error(SF_E_STATE, "no particles uploaded");
     if (!std::isfinite(dt) || !(dt > 0.f)) return set_error(SF_E_INVALID, "dt must be finite and > 0");
     if (n_steps < 1 || n_steps > 100000) return set_error(SF_E_INVALID, "n_steps must be in [1, 100000]");
     if (pass < 0 || pass >= SF_PASS_COUNT) return set_error(SF_E_INVALID, "pass must be one of the step's passes");
-    const size_t need = 2 * (size_t)n_steps;
+    if (every < 1 || every > n_steps) return set_error(SF_E_INVALID, "every must be in [1, n_steps]");
+    const int sampled = (n_steps + every - 1) / every;  // steps 0, every, 2 every, ...
+    const size_t need = 2 * (size_t)sampled;
     while (h->ev_pool.size() < need) {
         hipEvent_t e;
         SF_HIP(hipEventCreate(&e));
@@ -775,8 +781,9 @@ int sf_step_timed(sf_handle h, float dt, int n_steps, int pass, float* pass_ms, 
     if ((rc = views_in(h))) return rc;
     SF_HIP(hipEventRecord(h->ev0, s));
     for (int q = 0; q < n_steps; ++q) {
-        marks[pass] = h->ev_pool[2 * q];
-        marks[pass + 1] = h->ev_pool[2 * q + 1];
+        const bool mark = q % every == 0;
+        marks[pass] = mark ? h->ev_pool[2 * (q / every)] : nullptr;
+        marks[pass + 1] = mark ? h->ev_pool[2 * (q / every) + 1] : nullptr;
         if ((rc = one_step(h, dt, marks.data()))) return rc;
     }
     SF_HIP(hipEventRecord(h->ev1, s));
@@ -785,12 +792,12 @@ int sf_step_timed(sf_handle h, float dt, int n_steps, int pass, float* pass_ms, 
     if (total_ms) SF_HIP(hipEventElapsedTime(total_ms, h->ev0, h->ev1));
     if (pass_ms) {
         float sum = 0.f;
-        for (int q = 0; q < n_steps; ++q) {
+        for (int q = 0; q < sampled; ++q) {
             float ms = 0.f;
             SF_HIP(hipEventElapsedTime(&ms, h->ev_pool[2 * q], h->ev_pool[2 * q + 1]));
             sum += ms;
         }
-        *pass_ms = sum / (float)n_steps;
+        *pass_ms = sum / (float)sampled;
     }
     return SF_OK;
 }

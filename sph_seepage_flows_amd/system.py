@@ -164,12 +164,13 @@ class SFSystem:
         check(lib().sf_step_profiled(self._h, float(dt), int(n), ptr(pm), C.byref(tot)))
         return tot.value, (dict(zip(_abi.PASSES, map(float, pm))) if per_pass else None)
 
-    def step_timed(self, dt: float, n: int, pass_name: str):
-        """n fixed-dt steps with HIP events around one pass only (sf_step_timed).
-        Returns (total_ms, mean ms per step of that pass)."""
+    def step_timed(self, dt: float, n: int, pass_name: str, every: int = 1):
+        """n fixed-dt steps with HIP events around one pass only, on every
+        `every`-th step (sf_step_timed_every).  Returns (total_ms, mean ms of
+        that pass over the steps it was timed on)."""
         pm, tot = C.c_float(0.0), C.c_float(0.0)
-        check(lib().sf_step_timed(self._h, float(dt), int(n), _abi.PASSES.index(pass_name), C.byref(pm),
-                                  C.byref(tot)))
+        check(lib().sf_step_timed_every(self._h, float(dt), int(n), _abi.PASSES.index(pass_name), int(every),
+                                        C.byref(pm), C.byref(tot)))
         return tot.value, pm.value
 
     def set_params(self, params) -> None:

@@ -92,6 +92,14 @@ def test_profiled_step_matches_plain_step():
         d = SFSystem.from_scene(sc)
         assert d.step_timed(1e-4, 2, name)[1] > 0
     assert _abi.lib().sf_step_timed(c._h, 1e-4, 1, len(_abi.PASSES), None, None) == _abi.SF_E_INVALID
+    # events on every 3rd of 7 steps only (steps 0, 3, 6): same trajectory
+    e = SFSystem.from_scene(sc)
+    tot_e, ms_e = e.step_timed(1e-4, 7, "water_momentum", every=3)
+    a.step(1e-4, 2)
+    np.testing.assert_array_equal(a.get("pos"), e.get("pos"))
+    assert 0 < ms_e < tot_e
+    for every in (0, 8):
+        assert _abi.lib().sf_step_timed_every(e._h, 1e-4, 7, 0, every, None, None) == _abi.SF_E_INVALID
 
 
 def test_graph_after_an_eager_radix_step_matches_plain_steps():
