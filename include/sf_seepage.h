@@ -243,7 +243,9 @@ int sf_destroy(sf_handle h);
 /* Replace the parameter structs used by the next steps (the reference passes
  * them by value to every UpdateSolver call). grid/box must not change. */
 int sf_set_params(sf_handle h, const sf_params* p);
-/* hipStream_t as void*; NULL restores the handle's own stream. */
+/* hipStream_t as void*; NULL restores the handle's own stream.  Work the
+ * handle queued on the previous stream stays ahead of its work on the new
+ * one (the new stream waits for an event recorded on the old). */
 int sf_set_stream(sf_handle h, void* hip_stream);
 int sf_synchronize(sf_handle h);
 

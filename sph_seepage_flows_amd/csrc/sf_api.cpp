@@ -520,7 +520,8 @@ int sf_create(const sf_params* p, const sf_boundary_params* b, const sf_solver_o
         hipError_t e = hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking);
         if (e != hipSuccess) return bail(hip_error(e, "hipStreamCreate"));
         h->stream = h->own_stream;
-        if ((e = hipEventCreate(&h->ev0)) != hipSuccess || (e = hipEventCreate(&h->ev1)) != hipSuccess)
+        if ((e = hipEventCreate(&h->ev0)) != hipSuccess || (e = hipEventCreate(&h->ev1)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&h->ev_order, hipEventDisableTiming)) != hipSuccess)
             return bail(hip_error(e, "hipEventCreate"));
     }
     h->cap = max_num;
@@ -573,6 +574,7 @@ int sf_destroy(sf_handle h) {
     h->diff = nullptr;
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->ev_order) (void)hipEventDestroy(h->ev_order);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     delete h;
@@ -599,8 +601,15 @@ int sf_set_params(sf_handle h, const sf_params* p) {
 
 int sf_set_stream(sf_handle h, void* stream) {
     int rc;
-    if ((rc = check_handle(h))) return rc;
-    h->stream = stream ? (hipStream_t)stream : h->own_stream;
+    if ((rc = check_handle(h)) || (rc = set_device(h))) return rc;
+    hipStream_t next = stream ? (hipStream_t)stream : h->own_stream;
+    if (next != h->stream) {
+        // the handle's work still queued on the previous stream runs before
+        // anything the new stream gets from it
+        SF_HIP(hipEventRecord(h->ev_order, h->stream));
+        SF_HIP(hipStreamWaitEvent(next, h->ev_order, 0));
+        h->stream = next;
+    }
     return SF_OK;
 }
 

@@ -107,6 +107,7 @@ struct sf_system {
     // diagnostics
     DevBuf<uint32_t> nbr_cnt, nbr_cand;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_order = nullptr;  // orders a stream switch (sf_set_stream)
     std::vector<hipEvent_t> ev_pool;  // sf_step_profiled
     bool uploaded = false;
     bool built = false;

@@ -36,6 +36,12 @@ def runtime() -> C.CDLL:
         _rt.hipFree.restype = C.c_int
         _rt.hipFree.argtypes = [C.c_void_p]
         _rt.hipDeviceSynchronize.restype = C.c_int
+        _rt.hipStreamCreate.restype = C.c_int
+        _rt.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+        _rt.hipStreamDestroy.restype = C.c_int
+        _rt.hipStreamDestroy.argtypes = [C.c_void_p]
+        _rt.hipStreamSynchronize.restype = C.c_int
+        _rt.hipStreamSynchronize.argtypes = [C.c_void_p]
     return _rt
 
 
