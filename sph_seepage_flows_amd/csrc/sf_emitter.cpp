@@ -150,7 +150,9 @@ int emit_after_update(sf_system* h, float render_interval) {
     } else {
         // size_t(2 r / (|v| dt)); a period below one step emits every call
         // (the reference would divide by zero)
-        size_t period = (size_t)(two_r / (speed * h->P.dt));
+        const float q = two_r / (speed * h->P.dt);
+        // (a quotient beyond size_t, from a near-zero speed, never comes round)
+        size_t period = q < 1.8e19f ? (size_t)q : SIZE_MAX;
         if (period == 0) period = 1;
         if (e.counter++ % period == 0) return emit(h);
     }

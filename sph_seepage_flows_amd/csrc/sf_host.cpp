@@ -32,9 +32,7 @@ int validate(const sf_params* p, const sf_boundary_params* b) {
     if (!finite3(b->lowest_point) || !finite3(b->highest_point))
         return set_error(SF_E_INVALID, "box corners must be finite");
     sf_int3 g;
-    g.x = (int)((b->highest_point.x - b->lowest_point.x) / b->kernel_radius);
-    g.y = (int)((b->highest_point.y - b->lowest_point.y) / b->kernel_radius);
-    g.z = (int)((b->highest_point.z - b->lowest_point.z) / b->kernel_radius);
+    if (sf_grid_size(b->lowest_point, b->highest_point, b->kernel_radius, &g) != SF_OK) return SF_E_INVALID;
     if (b->grid_size.x < 1 || b->grid_size.y < 1 || b->grid_size.z < 1)
         return set_error(SF_E_INVALID, "grid_size must be >= 1 in every axis");
     if (g.x != b->grid_size.x || g.y != b->grid_size.y || g.z != b->grid_size.z)
@@ -107,9 +105,14 @@ int sf_validate_params(const sf_params* p, const sf_boundary_params* b) { return
 int sf_grid_size(sf_float3 lo, sf_float3 hi, float h, sf_int3* out) {
     if (!out) return set_error(SF_E_INVALID, "null output");
     if (!(h > 0.f)) return set_error(SF_E_INVALID, "kernel radius must be > 0");
-    out->x = (int)((hi.x - lo.x) / h);
-    out->y = (int)((hi.y - lo.y) / h);
-    out->z = (int)((hi.z - lo.z) / h);
+    const float e[3] = {(hi.x - lo.x) / h, (hi.y - lo.y) / h, (hi.z - lo.z) / h};
+    // the truncation below is defined only for values an int holds
+    for (float v : e)
+        if (!(v >= 0.f && v < 1073741824.f))
+            return set_error(SF_E_INVALID, "box extent / kernel radius must be in [0, 2^30) per axis");
+    out->x = (int)e[0];
+    out->y = (int)e[1];
+    out->z = (int)e[2];
     return SF_OK;
 }
 
@@ -124,6 +127,10 @@ int sf_build_world_boundary(sf_float3 lo, sf_float3 hi, float r, size_t capacity
     const float sx = (hi.x - lo.x) / sp, sy = (hi.y - lo.y) / sp, sz = (hi.z - lo.z) / sp;
     if (!(sx >= 0.f && sy >= 0.f && sz >= 0.f) || sx > 1e7f || sy > 1e7f || sz > 1e7f)
         return set_error(SF_E_INVALID, "box extents must be >= 0 and at most 1e7 spacings");
+    // at most (e + 2)^2 points per plane: keep the walk (and a count-only
+    // call, which runs it too) bounded
+    if (2.0 * ((sx + 2.0) * (sz + 2.0) + (sx + 2.0) * (sy + 2.0) + (sy + 2.0) * (sz + 2.0)) > 2e9)
+        return set_error(SF_E_INVALID, "the box needs more than 2e9 boundary particles");
     struct Plane {
         int a, b;         // axes iterated (0=x,1=y,2=z)
         int fixed_axis;   // axis held constant

@@ -46,6 +46,11 @@ def test_grid_size_matches_fp32_truncation():
         g = _abi.sf_int3()
         _abi.check(sf.lib().sf_grid_size(_abi.sf_float3.of(lo), _abi.sf_float3.of(hi), float(np.float32(h)), g))
         assert tuple(g.tolist()) == grid_size_py(lo, hi, h)
+    # extents an int cannot hold, negative or NaN: an error, not a truncation
+    for lo, hi, h in [((0, 0, 0), (1, 1, 1), 1e-12), ((0, 0, 0), (-1, 1, 1), 0.04), ((0, 0, 0), (np.nan, 1, 1), 0.04)]:
+        g = _abi.sf_int3()
+        rc = sf.lib().sf_grid_size(_abi.sf_float3.of(lo), _abi.sf_float3.of(hi), float(np.float32(h)), g)
+        assert rc == _abi.SF_E_INVALID
     b = make_params(Physics())[1]
     bad = _abi.sf_boundary_params.from_buffer_copy(b)
     bad.grid_size.x += 1
@@ -60,6 +65,18 @@ def test_world_boundary_matches_oracle(oracle_mod):
         np.testing.assert_array_equal(l, m)
     p, _ = sf.world_boundary((0, 0, 0), (1, 1, 1), 0.01)
     assert len(p) == 6 * 51 * 51  # 50 spacings per side -> 51 x 51 per plane, edges duplicated
+
+
+def test_world_boundary_size_limits():
+    """A box of 1e6 spacings per side would hold ~6e12 boundary particles:
+    refused up front rather than walked (a count-only call walks too)."""
+    n = C.c_size_t(0)
+    rc = sf.lib().sf_build_world_boundary(_abi.sf_float3.of((0, 0, 0)), _abi.sf_float3.of((1e4, 1e4, 1e4)), 0.005,
+                                          0, None, None, C.byref(n))
+    assert rc == _abi.SF_E_INVALID
+    rc = sf.lib().sf_build_world_boundary(_abi.sf_float3.of((0, 0, 0)), _abi.sf_float3.of((1, 1, 1)), 0.01, 0, None,
+                                          None, C.byref(n))
+    assert rc == 0 and n.value == 6 * 51 * 51
 
 
 def test_world_boundary_capacity_error():
