@@ -288,6 +288,14 @@ int wcsph_substep_tail(sf_system* h, float render_interval, FrameDev* frame) {
     return advect(h, 0.f, h->cfl_out.p);
 }
 
+// size_t(q) of the reference's nSub = renderInterval / params.dt, at least 1;
+// a quotient beyond 2^62 (which no frame could run) saturates there rather
+// than converting out of range
+size_t sub_step_count(float q) {
+    if (!(q >= 1.f)) return 1;
+    return q < 4.6e18f ? (size_t)q : ((size_t)1 << 62);
+}
+
 // CudaWCSphSFSolver::UpdateSolver / CudaSphSFSolver::UpdateSolver on the
 // current neighbour build: the passes, the sub-step count and the advect;
 // records ev1 after the advect and leaves h->nsub set (reads the WCSPH count
@@ -303,8 +311,7 @@ int update_solver(sf_system* h, float render_interval) {
         h->nsub = o.nsub;
     } else {
         // CudaSphSFSolver::UpdateSolver: nSub = renderInterval / params.dt, advect with params.dt
-        const float q = render_interval / h->P.dt;
-        h->nsub = q >= 1.f ? (size_t)q : 1;
+        h->nsub = sub_step_count(render_interval / h->P.dt);
         if ((rc = solve(h))) return rc;
         if ((rc = advect(h, h->P.dt, nullptr))) return rc;
         SF_HIP(hipEventRecord(h->ev1, h->stream));
@@ -954,8 +961,7 @@ static int frame_sph(sf_system* h, float ri, bool adaptive, size_t* steps) {
             if (done >= todo) break;
             cur = ri;
         }
-        const float q = cur / h->P.dt;
-        h->nsub = q >= 1.f ? (size_t)q : 1;
+        h->nsub = sub_step_count(cur / h->P.dt);
         if ((rc = build_neighbors(h)) || (rc = solve(h)) || (rc = advect(h, h->P.dt, nullptr))) return rc;
         if ((rc = emit_after_update(h, cur))) return rc;
         if (adaptive) r = r - r / static_cast<float>(h->nsub);
