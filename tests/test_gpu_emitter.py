@@ -106,3 +106,20 @@ def test_emitter_off_and_errors():
         g.set_emitter(**{**EMIT, "radius": 1e6})
     with pytest.raises(SFError):
         g.set_emitter(**{**EMIT, "pos": (float("nan"), 0.0, 0.0)})
+
+
+def test_emitter_sph_solver_near_zero_speed():
+    """Linear-EOS solver with an emission speed of 1e-20 m/s: the period
+    2 r / (|v| dt) ~ 2e22 lies beyond size_t, so it saturates (instead of
+    converting an out-of-range quotient); the first call emits and no later
+    one does."""
+    sc = scenes.mixed(collider=False)
+    p, b, o = sc.params()
+    p.solver_type = _abi.SF_SPH_SOLVER
+    g = SFSystem.from_scene(sc, max_num=sc.n + 4000)
+    g.set_params(p)
+    g.set_emitter(**{**EMIT, "vel": (0.0, 0.0, -1e-20)})
+    k = g.emitter_state()[1]
+    for _ in range(4):
+        g.UpdateSystem(RI)
+    assert g.emitter_state()[2] == k and g.sizes()[0] == sc.n + k
