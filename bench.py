@@ -39,11 +39,13 @@ def pass_bytes(n_w: int, n_s: int, nb: int, cells: int, multi: bool, listed=(0, 
     """Algorithmic HBM bytes per launch of each pass: every array element the
     pass must touch, read or written once (neighbour re-reads are served
     on-chip and not counted).  Layout: pm/vr/cs/flow/adh/xf/acc float4 (16 B),
-    rp/es/wt float2 (8 B), keys/perm/cellStart/lists u32 (4 B), flags u8,
-    boundary float4.  listed = (water, sand) particles with the other phase in
-    their 27 cells: the interface passes run over those only, and the density
-    pass writes the empty sums' results of the rest that a later pass reads
-    (water wt, sand voidage / saturation) plus one flag byte each."""
+    rp/es float2 (8 B), keys/perm/cellStart/lists u32 (4 B), flags u8,
+    boundary float4; the water momentum pass's two 16-byte neighbour records
+    per particle (every water particle has one mass, as in the benchmark
+    scenes).  listed = (water, sand) particles with the other phase in their
+    27 cells: the interface passes run over those only, and the density pass
+    writes the empty sums' results of the rest that a later pass reads
+    (water records, sand voidage / saturation) plus one flag byte each."""
     n = n_w + n_s
     aux = 24 if multi else 0
     lw, ls = listed
@@ -61,25 +63,26 @@ def pass_bytes(n_w: int, n_s: int, nb: int, cells: int, multi: bool, listed=(0, 
         # particle: phase list + position read, position in phase order written
         "cell_start": (cells + 1) * (16 + 16) + n * (4 + 16 + 16),
         # plist, own position (water: pm, sand: pph), water candidates' pph, rho/p (water);
-        # unlisted: wt or es + flag; listed: velocity, (rho, p), list entry
-        "density": n_w * (4 + 16 + 16 + 8 + 1) + n_s * (4 + 16 + 1) + (n_w - lw) * 8 + (n_s - ls) * 8
+        # sand: position record; unlisted water: velocity read, both records written;
+        # unlisted sand: es + flag; listed: velocity, (rho, p), list entry
+        "density": n_w * (4 + 16 + 16 + 8 + 1) + n_s * (4 + 16 + 1 + 16) + (n_w - lw) * (16 + 32) + (n_s - ls) * 8
                    + (lw + ls) * (16 + 4) + lw * 16 + 8 * (cells + 1) + 4 * (cells + 1) + cells + nb * 16,
         # listed only: list + slot, own position, phase-index position / velocity
-        # (/ rho for water), flow, es, water wt and voidage, sand saturation
-        "avg_flow": lw * (4 + 4 + 16 + 16 + 16 + 16 + 8 + 16 + 8 + 8 + 4) + ls * (4 + 4 + 16 + 16 + 16 + 16 + 8 + 4)
+        # (/ rho for water), flow, es, water velocity, records and voidage, sand saturation
+        "avg_flow": lw * (4 + 4 + 16 + 16 + 16 + 16 + 8 + 16 + 8 + 16 + 32 + 4) + ls * (4 + 4 + 16 + 16 + 16 + 16 + 8 + 4)
                     + 8 * (cells + 1),
         # listed only: list + slot, own position, phase-index position / (rho, voidage, sat);
         # sand voidage + adhesion 0, water adhesion + avgAdhesion (slot and phase index)
         "voidage_adhesion": lw * (4 + 4 + 16 + 16 + 16 + 48) + ls * (4 + 4 + 16 + 16 + 16 + 4 + 16)
                             + 8 * (cells + 1),
         # per grain: plist, position, velocity, es, colour read and written,
-        # interface flag, avgDrag and acc written; the sand candidates'
-        # phase-order positions touched once; the cell table
-        "sand_momentum": n_s * (4 + 16 + 16 + 8 + 16 + 16 + 1 + 16 + 16) + n_s * 16 + 8 * (cells + 1),
-        # every position once (own and candidates), water velocity and wt,
-        # sand avgDrag; per water particle plist, rho/p and acc written; the
-        # cell and boundary tables, boundary particles
-        "water_momentum": n * 16 + n_w * (16 + 8) + n_s * 16 + n_w * (4 + 8 + 16) + 8 * (cells + 1)
+        # interface flag, avgDrag (and its record) and acc written; the sand
+        # candidates' phase-order positions touched once; the cell table
+        "sand_momentum": n_s * (4 + 16 + 16 + 8 + 16 + 16 + 1 + 16 + 16 + 16) + n_s * 16 + 8 * (cells + 1),
+        # both records of every particle once (own and candidates); per water
+        # particle plist, rho/p and acc written; the cell and boundary
+        # tables, boundary particles
+        "water_momentum": n * 32 + n_w * (4 + 8 + 16) + 8 * (cells + 1)
                           + 4 * (cells + 1) + cells + nb * 16,
         # position, velocity, acc read; position, velocity written; sand colour;
         # the next key written and the current sorted key read (change flag)

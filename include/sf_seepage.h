@@ -322,7 +322,9 @@ int sf_set_field(sf_handle h, int field, const void* src, size_t src_bytes);
  * Once the position buffer (SF_FIELD_POS / MASS / LABEL) has been handed out,
  * every later neighbour build of the handle hashes the positions again, so
  * writes through it between steps are seen (otherwise the step's advect
- * already computed the next build's cell keys). */
+ * already computed the next build's cell keys), and the water momentum pass
+ * reads the slot arrays instead of its two-load records (the masses may
+ * change: 1-4 % slower per step at 1M particles, same results). */
 int sf_device_buffer(sf_handle h, int field, void** dptr, size_t* stride_bytes);
 /* A device array of the field in the reference's layout, as
  * CudaSFParticles::Get*Ptr() returns it: packed float3 (12-byte elements) for
@@ -343,7 +345,9 @@ int sf_device_buffer(sf_handle h, int field, void** dptr, size_t* stride_bytes);
  * stay > 0.  Writes into the other views are overwritten by the next step,
  * which recomputes those fields (in the reference too).  sf_download and
  * sf_export_bgeo read the state, not the views.  Handing out the position,
- * mass or label view makes every later build hash the positions again. */
+ * mass or label view makes every later build hash the positions again and
+ * turns the water momentum pass's two-load records off, as sf_device_buffer
+ * does. */
 int sf_particle_view(sf_handle h, int field, void** dptr);
 /* Writes folder/name.bgeo from the current state (creates folder). */
 int sf_export_bgeo(sf_handle h, const char* folder, const char* name);

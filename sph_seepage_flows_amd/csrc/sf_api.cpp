@@ -130,7 +130,7 @@ int ensure_sort_capacity(sf_system* h, size_t m) {
     if ((rc = h->wpre.alloc(c + 1)) || (rc = h->wpart.alloc((size_t)wprefix_blocks((int)c + 1) + 1)) ||
         (rc = h->plist.alloc(c)) || (rc = h->wflag.alloc(c)) || (rc = h->pph.alloc(c)) || (rc = h->ilist.alloc(c)) ||
         (rc = h->icount.alloc(2)) || (rc = h->iface.alloc(c)) || (rc = h->wl.alloc(c)) || (rc = h->vq.alloc(c)) || (rc = h->wq.alloc(c)) ||
-        (rc = h->xq.alloc(c)))
+        (rc = h->xq.alloc(c)) || (rc = h->rec0.alloc(c)) || (rc = h->rec1.alloc(c)))
         return rc;
     // the interface counters start at zero (each step's water momentum pass
     // clears them for the next)
@@ -638,9 +638,17 @@ int sf_upload_particles(sf_handle h, size_t n, const float* pos3, const float* v
     if (n > 0 && (!pos3 || !label || !mass)) return set_error(SF_E_INVALID, "pos3, label and mass are required");
     std::vector<float4> pm(n), vr(n), cs(n), cda(n);
     std::vector<float2> amc(n);
+    int wstate = 0;
+    float wmass = 0.f;
     for (size_t i = 0; i < n; ++i) {
         if (label[i] > 1) return set_error(SF_E_INVALID, "labels must be 0 (water) or 1 (sand)");
         if (!(mass[i] > 0.f) || !std::isfinite(mass[i])) return set_error(SF_E_INVALID, "masses must be finite and > 0");
+        if (label[i] == 0 && wstate == 0) {
+            wstate = 1;
+            wmass = mass[i];
+        } else if (label[i] == 0 && mass[i] != wmass) {
+            wstate = 2;
+        }
         const float m = label[i] == 1 ? -mass[i] : mass[i];
         pm[i] = make_float4(pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2], m);
         vr[i] = vel3 ? make_float4(vel3[3 * i], vel3[3 * i + 1], vel3[3 * i + 2], radius ? radius[i] : 0.f)
@@ -663,7 +671,11 @@ int sf_upload_particles(sf_handle h, size_t n, const float* pos3, const float* v
     h->uploaded = true;
     h->built = false;
     h->build_pending = false;
+    h->water_mass_state = wstate;
+    h->water_mass = wmass;
     h->drop_graph();
+    diff_release(h->diff);  // a recorded window no longer matches the particles
+    h->diff = nullptr;
     if ((rc = views_out(h))) return rc;
     SF_HIP(hipStreamSynchronize(h->stream));
     return SF_OK;

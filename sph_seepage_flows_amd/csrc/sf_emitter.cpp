@@ -68,6 +68,12 @@ int append_water(sf_system* h, const std::vector<F3>& p) {
     SF_HIP(hipMemcpyAsync(h->amc[h->cur_aux].p + n, amc.data(), k * sizeof(float2), hipMemcpyHostToDevice, s));
     SF_HIP(hipStreamSynchronize(s));
     h->n = n + k;
+    if (k && h->water_mass_state == 0) {
+        h->water_mass_state = 1;
+        h->water_mass = h->P.sph_mass;
+    } else if (k && h->water_mass != h->P.sph_mass) {
+        h->water_mass_state = 2;
+    }
     h->built = false;  // the new slots have no previous keys: full sort next build
     h->drop_graph();
     diff_release(h->diff);  // a recorded window no longer matches the particles

@@ -216,6 +216,16 @@ struct Fields {
     float4* vq;
     float4* wq;
     float4* xq;
+    // Neighbour records of the water momentum pass, by slot, written when
+    // rec is set (every water particle has the same mass mw): water
+    // rec0 = (position, rho~), rec1 = (velocity, p / rho~^2); sand
+    // rec0 = (position, -mass), rec1 = avgDrag.  A neighbour then costs two
+    // 16-byte loads instead of the three of pm, wt and vr (or pm and xf).
+    // With rec set, wt is written only for the differentiable window (den4).
+    float4* rec0;
+    float4* rec1;
+    float mw;
+    int rec;
 };
 // Launch shape of a neighbour pass: workgroup size and the per-lane LDS
 // candidate-list capacity (the latter only for the passes that use one).

@@ -307,7 +307,7 @@ void dispatch(const Fields& f, PassCfg c, L&& l) {
 
 // What P2 then P3 leave for particle i with empty sums; rpi = (rho, p).
 // Later passes read the sand's voidage / saturation (drag, capillary bridges)
-// and the water's wt; the rest is output only and written here when the
+// and the water's wt or records; the rest is output only and written here when the
 // fields are read directly (full_empty) or taped (den4), else on request.
 SF_DEV void write_empty_outputs(const Fields& f, int i, bool sand) {
     f.flow[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -317,12 +317,23 @@ SF_DEV void write_empty_outputs(const Fields& f, int i, bool sand) {
         f.xf[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
 }
-SF_DEV void write_no_interface(const Fields& f, int i, bool sand, float2 rpi) {
+// Water particle i's (rho~, p / rho~^2) from (rho, p) and its voidage eps:
+// wt, or the neighbour records with pi = (position, mass) (Fields::rec0).
+SF_DEV void write_water_terms(const Fields& f, int i, const float4& pi, float2 rpi, float eps) {
+    const float rt = rpi.x / fmaxf(kEps, eps);
+    const float pt = rpi.y / fmaxf(kEps, rt * rt);
+    if (!f.rec || f.den4) f.wt[i] = make_float2(rt, pt);
+    if (f.rec) {
+        const float4 v = ldo(f.vr, (uint32_t)i);
+        f.rec0[i] = make_float4(pi.x, pi.y, pi.z, rt);
+        f.rec1[i] = make_float4(v.x, v.y, v.z, pt);
+    }
+}
+SF_DEV void write_no_interface(const Fields& f, int i, bool sand, float2 rpi, const float4& pi) {
     if (sand) {
         f.es[i] = make_float2(0.f, 0.f);
     } else {
-        const float rt = rpi.x / fmaxf(kEps, 1.f);
-        f.wt[i] = make_float2(rt, rpi.y / fmaxf(kEps, rt * rt));
+        write_water_terms(f, i, pi, rpi, 1.f);
     }
     if (f.full_empty || f.den4) write_empty_outputs(f, i, sand);
     if (f.den4) {
@@ -376,6 +387,7 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
             // taped, else on request (launch_fill_empty)
             if (f.full_empty || f.den4) f.rp[i] = make_float2(k.rho_s, 0.f);
             const float4 p = ldo(f.pph, nw + t);
+            if (f.rec) f.rec0[i] = p;
             const uint32_t cb = scene_base<B>(k, f, i);
             int cx, cy, cz;
             cell_of(k, p.x, p.y, p.z, cx, cy, cz);
@@ -387,7 +399,7 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
         list_interface(f, listed, true, nw + t, nw, (uint32_t)n);
         if (valid) f.iface[t] = listed ? 1 : 0;
         if (listed) f.vq[nw + t] = ldo(f.vr, (uint32_t)i);
-        if (valid && !listed) write_no_interface(f, i, true, make_float2(k.rho_s, 0.f));
+        if (valid && !listed) write_no_interface(f, i, true, make_float2(k.rho_s, 0.f), make_float4(0.f, 0.f, 0.f, 0.f));
         return;
     }
     const uint32_t t = pb.blk * PB + threadIdx.x / kDenL;
@@ -461,7 +473,7 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
     const float2 rpi = make_float2(rho, eos(k, rho));
     f.rp[i] = rpi;
     if (!listed) {
-        write_no_interface(f, i, false, rpi);
+        write_no_interface(f, i, false, rpi, pi);
     } else {
         f.vq[t] = ldo(f.vr, (uint32_t)i);
         f.wq[t] = make_float4(rpi.x, rpi.y, 0.f, 0.f);  // voidage: avg flow
@@ -488,9 +500,11 @@ __global__ __launch_bounds__(kBlock) void k_interface(SFConst k, Fields f, int n
     const bool sand_i = t >= nw;
     int i = 0;
     uint32_t other = 0;
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
     if (valid) {
         i = (int)ldo(f.plist, t);
-        const float4 pi = ldo(f.pph, t);
+        pi = ldo(f.pph, t);
+        if (f.rec && sand_i) f.rec0[i] = pi;
         const uint32_t cb = scene_base<B>(k, f, i);
         int cx, cy, cz;
         cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
@@ -504,7 +518,7 @@ __global__ __launch_bounds__(kBlock) void k_interface(SFConst k, Fields f, int n
     list_interface(f, listed, sand_i, t, nw, (uint32_t)n);
     if (valid && !sand_i) f.wl[t] = listed ? 1 : 0;
     if (valid && sand_i) f.iface[t - nw] = listed ? 1 : 0;
-    if (valid && !listed) write_no_interface(f, i, sand_i, ldo(f.rp, (uint32_t)i));
+    if (valid && !listed) write_no_interface(f, i, sand_i, ldo(f.rp, (uint32_t)i), pi);
 }
 __global__ __launch_bounds__(kBlock) void k_fill_empty(SFConst k, Fields f, int n) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
@@ -614,9 +628,7 @@ __global__ __launch_bounds__(NB) void k_avg_flow(SFConst k, Fields f, int n) {
     f.es[i] = make_float2(eps_i, sand_i ? vsum : 0.f);
     if (sand_i) reinterpret_cast<float*>(f.wq)[4 * ps.t + 3] = vsum;  // saturation, for P3
     if (!sand_i) {
-        const float2 rpi = ldo(f.rp, (uint32_t)i);
-        const float rt = rpi.x / fmaxf(kEps, eps_i);
-        f.wt[i] = make_float2(rt, rpi.y / fmaxf(kEps, rt * rt));
+        write_water_terms(f, i, pi, ldo(f.rp, (uint32_t)i), eps_i);
         reinterpret_cast<float*>(f.wq)[4 * ps.t + 2] = eps_i;
     }
 }
@@ -880,7 +892,9 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
             df = (fl - vi) * (k.grav_abs * k.rho0 * (eps_i * eps_i) / dc * mi / rho_i);
         }
     }
-    f.xf[i] = vw != 0.f ? f4(df * (1.f / vw), 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 drag = vw != 0.f ? f4(df * (1.f / vw), 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+    f.xf[i] = drag;
+    if (f.rec) f.rec1[i] = drag;
     if (f.den4) reinterpret_cast<float*>(f.den4)[4 * i + 2] = vw;
     const v3 a = mk3(k.gvx, k.gvy, k.gvz) + (fsum + df) * (2.f / mi);
     f.acc[i] = f4(a, 0.f);
@@ -910,7 +924,29 @@ void launch_sand_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, h
 // wave's ranges overlap), and the partial sums are combined with shuffles.
 constexpr int kWmL = 4;
 constexpr uint32_t kWmStage = 64;  // longest union range staged per wave
-template <bool B, int NB, int CAP>
+// A neighbour's data in the water momentum pass: r0 = (position, w) with
+// w > 0 for water, w = -mass for sand.  WmRecords reads the two records
+// (Fields::rec0; every water particle has mass f.mw); WmSlots reads the
+// slot arrays pm (w = mass), wt and vr, and a grain's xf on its own.
+struct WmRecords {
+    float4 r0, r1;
+    SF_DEV static WmRecords load(const Fields& f, uint32_t j) { return {ldo(f.rec0, j), ldo(f.rec1, j)}; }
+    SF_DEV float water_mass(const Fields& f) const { return f.mw; }
+    SF_DEV float2 wt() const { return make_float2(r0.w, r1.w); }
+    SF_DEV v3 vel() const { return xyz(r1); }
+    SF_DEV v3 drag(const Fields&, uint32_t) const { return xyz(r1); }
+};
+struct WmSlots {
+    float4 r0;
+    float2 w;
+    float4 v;
+    SF_DEV static WmSlots load(const Fields& f, uint32_t j) { return {ldo(f.pm, j), ldo(f.wt, j), ldo(f.vr, j)}; }
+    SF_DEV float water_mass(const Fields&) const { return r0.w; }
+    SF_DEV float2 wt() const { return w; }
+    SF_DEV v3 vel() const { return xyz(v); }
+    SF_DEV v3 drag(const Fields& f, uint32_t j) const { return xyz(ldo(f.xf, j)); }
+};
+template <bool B, class Nbr, int NB, int CAP>
 __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int n) {
     SF_NLIST(L);
     __shared__ float4 stage_lds[(NB / 64) * kWmStage];
@@ -926,12 +962,13 @@ __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int 
     const uint32_t sub = threadIdx.x % kWmL;  // this lane's share of the particle's candidates
     // lanes past the last particle take part in the staged walk without a range
     const int i = t < nw ? (int)ldo(f.plist, t) : 0;
-    const float4 pi = ldo(f.pm, (uint32_t)i);
+    const Nbr self = Nbr::load(f, (uint32_t)i);
+    const float4 pi = self.r0;  // position; w > 0 for water
     const bool active = t < nw && pi.w >= 0.f;
     const uint32_t cb = scene_base<B>(k, f, i);
-    const float mi = pi.w;
-    const v3 vi = xyz(ldo(f.vr, (uint32_t)i));
-    const float2 wti = ldo(f.wt, (uint32_t)i);  // (rho~_i, p_i / rho~_i^2)
+    const float mi = self.water_mass(f);
+    const v3 vi = self.vel();
+    const float2 wti = self.wt();  // (rho~_i, p_i / rho~_i^2)
     const float p_i = ldo(f.rp, (uint32_t)i).y;
     const float rt_i = wti.x;
     const float inv_rt_i = frcp(rt_i);
@@ -940,15 +977,9 @@ __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int 
     int cx, cy, cz;
     cell_of(k, pi.x, pi.y, pi.z, cx, cy, cz);
     v3 a = mk3(0.f, 0.f, 0.f);
-    // neighbour data of the common (water) case; every slot has these arrays
-    struct Nbr {
-        float4 pj;
-        float2 wtj;
-        float4 vj;
-    };
-    auto fetch = [&](uint32_t j) { return Nbr{ldo(f.pm, j), ldo(f.wt, j), ldo(f.vr, j)}; };
+    auto fetch = [&](uint32_t j) { return Nbr::load(f, j); };
     auto pair = [&](uint32_t j, const Nbr& q) {
-        const float4 pj = q.pj;
+        const float4 pj = q.r0;
         const v3 d = mk3(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
         const float r2 = dot(d, d);
         const float rs = frsq(r2);
@@ -956,19 +987,20 @@ __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int 
         const float hr = k.h - r;
         const float gs = r >= kEps ? k.g_coef * hr * hr * rs : 0.f;  // spiky gradient / r
         if (pj.w > 0.f) {  // water j: ComputeSFWaterPressureTerm + ArtificialViscosity
-            const float2 wtj = q.wtj;
-            float s = -pj.w * (wti.y + wtj.y);
-            const v3 dv = vi - xyz(q.vj);
+            const float2 wtj = q.wt();
+            const float mj = q.water_mass(f);
+            float s = -mj * (wti.y + wtj.y);
+            const v3 dv = vi - q.vel();
             const float dd = dot(dv, d);
             const float pij = -k.nu * frcp(rt_i + wtj.x) * (dd * frcp(r2 + kEps));
-            s += dd < 0.f ? -pj.w * pij : 0.f;
+            s += dd < 0.f ? -mj * pij : 0.f;
             a += d * (s * gs);
         } else {  // sand j: ComputeSFWaterSeepageTerm
             const float mj = -pj.w;
             const float rho_j = k.rho_s;  // the density pass gives every grain rho_sand
             const float w = k.w_coef * sq3(k.h2 - r2);
             a += d * (-mj * p_i * frcp(fmaxf(kEps, rt_i * rho_j)) * gs);
-            a += xyz(ldo(f.xf, j)) * (-w * inv_rt_i);
+            a += q.drag(f, j) * (-w * inv_rt_i);
         }
     };
     // the particle's own slot passes too: d = 0 makes its term exactly zero
@@ -1016,8 +1048,10 @@ __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int 
 void launch_water_momentum(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
     if (n <= 0) return;
     dispatch(f, c, [&](auto B, auto NB, auto CAP) {
-        k_water_momentum<decltype(B)::value, decltype(NB)::value, decltype(CAP)::value>
-            <<<phase_grid(n * kWmL, NB), (int)NB, 0, s>>>(k, f, n);
+        constexpr bool b = decltype(B)::value;
+        constexpr int nb = decltype(NB)::value, cap = decltype(CAP)::value;
+        if (f.rec) k_water_momentum<b, WmRecords, nb, cap><<<phase_grid(n * kWmL, NB), nb, 0, s>>>(k, f, n);
+        else k_water_momentum<b, WmSlots, nb, cap><<<phase_grid(n * kWmL, NB), nb, 0, s>>>(k, f, n);
     });
 }
 

@@ -55,6 +55,7 @@ struct sf_system {
     size_t n_built = 0;
     DevBuf<uint8_t> bnear;            // cells with boundary particles near (Fields::bnear)
     DevBuf<float4> vq, wq, xq;        // listed particles' fields by phase index (Fields::wq)
+    DevBuf<float4> rec0, rec1;        // water momentum neighbour records (Fields::rec0)
     DevBuf<uint8_t> wflag;
     DevBuf<uint2> ct;
     // incremental sort (sf_set_sort_strategy): sorted-key ping-pong, permutation, work arrays
@@ -67,6 +68,14 @@ struct sf_system {
     // anything else that changes positions
     bool keys_fresh = false;
     bool pm_shared = false;  // the position buffer was handed out (sf_device_buffer)
+    // water masses of the particles: 0 no water, 1 all equal to water_mass,
+    // 2 not all equal (set by the upload and the emitter)
+    int water_mass_state = 0;
+    float water_mass = 0.f;
+    // the neighbour records are used while the water masses are equal and
+    // known on the host (a caller holding the position / mass buffer may
+    // change them)
+    bool records_on() const { return water_mass_state < 2 && !pm_shared; }
     const uint32_t* perm = nullptr;
     // transient fields
     DevBuf<float2> rp, es, wt;
@@ -143,6 +152,10 @@ struct sf_system {
         f.vq = vq.p;
         f.wq = wq.p;
         f.xq = xq.p;
+        f.rec0 = rec0.p;
+        f.rec1 = rec1.p;
+        f.mw = water_mass;
+        f.rec = records_on() ? 1 : 0;
         return f;
     }
     uint32_t local_cells() const { return K.C; }
