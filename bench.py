@@ -63,13 +63,17 @@ def pass_bytes(n_w: int, n_s: int, nb: int, cells: int, multi: bool, listed=(0, 
         # particle: phase list + position read, position in phase order written
         "cell_start": (cells + 1) * (16 + 16) + n * (4 + 16 + 16),
         # plist, own position (water: pm, sand: pph), water candidates' pph, rho/p (water);
-        # sand: position record; unlisted water: velocity read, both records written;
-        # unlisted sand: es + flag; listed: velocity, (rho, p), list entry
-        "density": n_w * (4 + 16 + 16 + 8 + 1) + n_s * (4 + 16 + 1 + 16) + (n_w - lw) * (16 + 32) + (n_s - ls) * 8
-                   + (lw + ls) * (16 + 4) + lw * 16 + 8 * (cells + 1) + 4 * (cells + 1) + cells + nb * 16,
+        # sand: position record, velocity read and written in phase order;
+        # unlisted water: velocity read, both records written; unlisted
+        # sand: es + flag; listed water: velocity read and written, (rho, p);
+        # listed: list entry
+        "density": n_w * (4 + 16 + 16 + 8 + 1) + n_s * (4 + 16 + 1 + 16 + 32) + (n_w - lw) * (16 + 32)
+                   + (n_s - ls) * 8 + lw * (32 + 16) + (lw + ls) * 4 + 8 * (cells + 1) + 4 * (cells + 1) + cells
+                   + nb * 16,
         # listed only: list + slot, own position, phase-index position / velocity
-        # (/ rho for water), flow, es, water velocity, records and voidage, sand saturation
-        "avg_flow": lw * (4 + 4 + 16 + 16 + 16 + 16 + 8 + 16 + 8 + 16 + 32 + 4) + ls * (4 + 4 + 16 + 16 + 16 + 16 + 8 + 4)
+        # (/ rho for water), flow, es, water velocity, records and voidage, sand
+        # saturation (twice: for P3, and beside the phase-order velocity for P4)
+        "avg_flow": lw * (4 + 4 + 16 + 16 + 16 + 16 + 8 + 16 + 8 + 16 + 32 + 4) + ls * (4 + 4 + 16 + 16 + 16 + 16 + 8 + 4 + 4)
                     + 8 * (cells + 1),
         # listed only: list + slot, own position, phase-index position / (rho, voidage, sat);
         # sand voidage + adhesion 0, water adhesion + avgAdhesion (slot and phase index)
@@ -77,8 +81,8 @@ def pass_bytes(n_w: int, n_s: int, nb: int, cells: int, multi: bool, listed=(0, 
                             + 8 * (cells + 1),
         # per grain: plist, position, velocity, es, colour read and written,
         # interface flag, avgDrag (and its record) and acc written; the sand
-        # candidates' phase-order positions touched once; the cell table
-        "sand_momentum": n_s * (4 + 16 + 16 + 8 + 16 + 16 + 1 + 16 + 16 + 16) + n_s * 16 + 8 * (cells + 1),
+        # candidates' phase-order positions and velocities touched once; the cell table
+        "sand_momentum": n_s * (4 + 16 + 16 + 8 + 16 + 16 + 1 + 16 + 16 + 16) + n_s * 32 + 8 * (cells + 1),
         # both records of every particle once (own and candidates); per water
         # particle plist, rho/p and acc written; the cell and boundary
         # tables, boundary particles

@@ -206,7 +206,9 @@ struct Fields {
     int full_empty;
     // Fields of the listed particles (those with the other phase in their 9
     // column ranges) by phase index t, written only for those:
-    //   vq[t] = the velocity (density pass);
+    //   vq[t] = the velocity (density pass); for every grain, not only the
+    //   listed ones, with its saturation in vq[t].w (0 from the density
+    //   pass, the avg-flow pass's value for the listed);
     //   water: wq[t] = (rho, p) (density pass), voidage in wq[t].z (avg
     //   flow), xq[t] = avgAdhesion (voidage/adhesion pass);
     //   sand: saturation in wq[t].w (avg flow).

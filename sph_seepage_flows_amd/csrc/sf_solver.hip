@@ -398,7 +398,10 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
         const bool listed = valid && other != 0u;
         list_interface(f, listed, true, nw + t, nw, (uint32_t)n);
         if (valid) f.iface[t] = listed ? 1 : 0;
-        if (listed) f.vq[nw + t] = ldo(f.vr, (uint32_t)i);
+        if (valid) {  // every grain's velocity and saturation by phase index (P2 sets the listed ones' .w)
+            const float4 v = ldo(f.vr, (uint32_t)i);
+            f.vq[nw + t] = make_float4(v.x, v.y, v.z, 0.f);
+        }
         if (valid && !listed) write_no_interface(f, i, true, make_float2(k.rho_s, 0.f), make_float4(0.f, 0.f, 0.f, 0.f));
         return;
     }
@@ -626,7 +629,10 @@ __global__ __launch_bounds__(NB) void k_avg_flow(SFConst k, Fields f, int n) {
     if (f.den4) reinterpret_cast<float*>(f.den4)[4 * i] = vsum;
     const float eps_i = 1.f - sandv;
     f.es[i] = make_float2(eps_i, sand_i ? vsum : 0.f);
-    if (sand_i) reinterpret_cast<float*>(f.wq)[4 * ps.t + 3] = vsum;  // saturation, for P3
+    if (sand_i) {  // saturation, for P3 and (with the velocity) P4
+        reinterpret_cast<float*>(f.wq)[4 * ps.t + 3] = vsum;
+        reinterpret_cast<float*>(f.vq)[4 * ps.t + 3] = vsum;
+    }
     if (!sand_i) {
         write_water_terms(f, i, pi, ldo(f.rp, (uint32_t)i), eps_i);
         reinterpret_cast<float*>(f.wq)[4 * ps.t + 2] = eps_i;
@@ -782,14 +788,11 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
 
     // sand-sand: DEM contact or liquid bridge
     struct SandNbr {
-        float4 pj, vj;
-        float satj;
+        float4 pj, vj;  // vj.w: saturation
     };
-    // candidates are phase indices u (slot plist[u]; pph[u] = pm[plist[u]])
-    auto fetch_sand = [&](uint32_t u) {
-        const uint32_t j = ldo(f.plist, u);
-        return SandNbr{ldo(f.pph, u), ldo(f.vr, j), ldo(f.es, j).y};
-    };
+    // candidates are phase indices u (pph[u] = pm[plist[u]], vq[u] = the
+    // velocity and saturation of slot plist[u])
+    auto fetch_sand = [&](uint32_t u) { return SandNbr{ldo(f.pph, u), ldo(f.vq, u)}; };
     auto contact = [&](uint32_t, const SandNbr& q) {
         const float4 pj = q.pj;
         const v3 dji = mk3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
@@ -798,7 +801,7 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
             fsum += dem_contact(dji, dist, xyz(q.vj) - vi, twoR, k.kn, k.ks, k.tan2);
         } else {
             const float H = dist - twoR;
-            if (H > 0.f && H < k.cap_srup) fsum += capillary(k, dji, dist, H, (sat_i + q.satj) * 0.5f);
+            if (H > 0.f && H < k.cap_srup) fsum += capillary(k, dji, dist, H, (sat_i + q.vj.w) * 0.5f);
         }
     };
     {

@@ -264,3 +264,20 @@ def test_diff_errors():
     b = BatchSystem.from_scenes(E)
     with pytest.raises(SFError):
         _abi.check(_abi.lib().sf_diff_forward(b.handle, DT, 2, 1))
+
+
+def test_upload_releases_the_window():
+    """A recorded window belongs to the particles it ran on: after a new
+    upload the backward (and a reset) report that no window has run."""
+    from sph_seepage_flows_amd import SFSystem
+
+    sc = scenes.mixed(collider=False)
+    g = SFSystem.from_scene(sc)
+    _abi.check(_abi.lib().sf_diff_forward(g.handle, DT, 2, 1))
+    g.upload_particles(scenes.mixed(collider=False, n_water=1000, n_sand=800))
+    with pytest.raises(SFError) as e:
+        _abi.check(_abi.lib().sf_diff_backward(g.handle, None, None, None, None, None, None, 0))
+    assert e.value.code == _abi.SF_E_STATE
+    with pytest.raises(SFError) as e:
+        _abi.check(_abi.lib().sf_diff_reset(g.handle, None, 0))
+    assert e.value.code == _abi.SF_E_STATE
