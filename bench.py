@@ -77,7 +77,8 @@ def pass_bytes(n_w: int, n_s: int, nb: int, cells: int, multi: bool, listed=(0, 
                     + 8 * (cells + 1),
         # listed only: list + slot, own position, phase-index position / (rho, voidage, sat);
         # sand voidage + adhesion 0, water adhesion + avgAdhesion (slot and phase index)
-        "voidage_adhesion": lw * (4 + 4 + 16 + 16 + 16 + 48) + ls * (4 + 4 + 16 + 16 + 16 + 4 + 16)
+        # and the buoyancy record (own (rho, p, voidage) read, position + factor written)
+        "voidage_adhesion": lw * (4 + 4 + 16 + 16 + 16 + 48 + 16 + 16) + ls * (4 + 4 + 16 + 16 + 16 + 4 + 16)
                             + 8 * (cells + 1),
         # per grain: plist, position, velocity, es, colour read and written,
         # interface flag, avgDrag (and its record) and acc written; the sand

@@ -130,7 +130,7 @@ int ensure_sort_capacity(sf_system* h, size_t m) {
     if ((rc = h->wpre.alloc(c + 1)) || (rc = h->wpart.alloc((size_t)wprefix_blocks((int)c + 1) + 1)) ||
         (rc = h->plist.alloc(c)) || (rc = h->wflag.alloc(c)) || (rc = h->pph.alloc(c)) || (rc = h->ilist.alloc(c)) ||
         (rc = h->icount.alloc(2)) || (rc = h->iface.alloc(c)) || (rc = h->wl.alloc(c)) || (rc = h->vq.alloc(c)) || (rc = h->wq.alloc(c)) ||
-        (rc = h->xq.alloc(c)) || (rc = h->rec0.alloc(c)) || (rc = h->rec1.alloc(c)))
+        (rc = h->xq.alloc(c)) || (rc = h->wbq.alloc(c)) || (rc = h->rec0.alloc(c)) || (rc = h->rec1.alloc(c)))
         return rc;
     // the interface counters start at zero (each step's water momentum pass
     // clears them for the next)

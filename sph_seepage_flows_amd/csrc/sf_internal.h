@@ -218,6 +218,10 @@ struct Fields {
     float4* vq;
     float4* wq;
     float4* xq;
+    // wbq[t] = (position, m / rho~) of listed water t, and xq[t].w = its
+    // m p / max(eps, rho_sand rho~) (voidage/adhesion pass; sand momentum's
+    // buoyancy pairs)
+    float4* wbq;
     // Neighbour records of the water momentum pass, by slot, written when
     // rec is set (every water particle has the same mass mw): water
     // rec0 = (position, rho~), rec1 = (velocity, p / rho~^2); sand

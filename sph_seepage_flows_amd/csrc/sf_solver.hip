@@ -727,7 +727,18 @@ __global__ __launch_bounds__(NB) void k_voidage_adhesion(SFConst k, Fields f, in
     f.adh[i] = f4(F, 0.f);
     const float4 avg = vs != 0.f ? f4(F * (1.f / vs), 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
     f.xf[i] = avg;
-    f.xq[ps.t] = avg;
+    // what a grain's buoyancy sums need of this particle (sand momentum
+    // pass): m / rho~ and m p / max(eps, rho_sand rho~), both zero unless the
+    // voidage lies in (0, 1]
+    const float4 w = ldo(f.wq, ps.t);  // (rho, p, voidage)
+    float a = 0.f, g = 0.f;
+    if (w.z > 0.f && w.z <= 1.f) {
+        const float dj = w.x * frcp(fmaxf(kEps, w.z));
+        a = mi * frcp(dj);
+        g = mi * w.y * frcp(fmaxf(kEps, k.rho_s * dj));
+    }
+    f.xq[ps.t] = make_float4(avg.x, avg.y, avg.z, g);
+    f.wbq[ps.t] = make_float4(pi.x, pi.y, pi.z, a);
     if (f.den4) reinterpret_cast<float*>(f.den4)[4 * i + 1] = vs;
 }
 void launch_voidage_adhesion(const SFConst& k, const Fields& f, int n, PassCfg c, hipStream_t s) {
@@ -826,32 +837,24 @@ __global__ __launch_bounds__(NB) void k_sand_momentum(SFConst k, Fields f, int n
     // water neighbours: buoyancy (ComputeSFSandBuoyancyForces) + adhesion
     // (ComputeSFSandAdhesionTerm)
     struct WaterNbr {
-        float4 pj, xfj;
-        float2 rpj;
-        float eps_j;
+        float4 pa;  // position, m_j / rho~_j
+        float4 xg;  // avgAdhesion, m_j p_j / max(eps, rho_sand rho~_j)
     };
     // a water neighbour within h has this grain in its columns, so it is
-    // listed and its fields are in wq / xq at its phase index u
-    auto fetch_water = [&](uint32_t u) {
-        const float4 w = ldo(f.wq, u);
-        return WaterNbr{ldo(f.pph, u), ldo(f.xq, u), make_float2(w.x, w.y), w.z};
-    };
+    // listed and the voidage/adhesion pass wrote its record at its phase
+    // index u (the two factors zero where rho~_j is undefined: voidage
+    // outside (0, 1])
+    auto fetch_water = [&](uint32_t u) { return WaterNbr{ldo(f.wbq, u), ldo(f.xq, u)}; };
     auto water = [&](uint32_t, const WaterNbr& q) {
-        const float4 pj = q.pj;
+        const float4 pj = q.pa;
         const v3 dji = mk3(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
         const float r2 = dot(dji, dji);
         const float w = k.w_coef * sq3(k.h2 - r2);
-        const float eps_j = q.eps_j;
-        if (eps_j > 0.f && eps_j <= 1.f) {
-            const float2 rpj = q.rpj;
-            const float mj = pj.w;
-            const float dj = rpj.x * frcp(fmaxf(kEps, eps_j));
-            vw += mj * frcp(dj) * w;
-            const float gs = spiky_s(k, fsqrt(r2));
-            // gradW(x_i - x_j) = gs * (x_i - x_j) = -gs * dji
-            fsum += dji * (mi * mj * rpj.y * frcp(fmaxf(kEps, rho_i * dj)) * gs);
-        }
-        fsum += xyz(q.xfj) * (w * vol_i);
+        vw += pj.w * w;
+        const float gs = spiky_s(k, fsqrt(r2));
+        // gradW(x_i - x_j) = gs * (x_i - x_j) = -gs * dji
+        fsum += dji * (mi * q.xg.w * gs);
+        fsum += xyz(q.xg) * (w * vol_i);
     };
     // a grain with no water in its 9 column ranges (the density pass flagged
     // the others) has nothing to walk

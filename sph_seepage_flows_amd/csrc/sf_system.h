@@ -54,7 +54,7 @@ struct sf_system {
     bool empty_pending = false;
     size_t n_built = 0;
     DevBuf<uint8_t> bnear;            // cells with boundary particles near (Fields::bnear)
-    DevBuf<float4> vq, wq, xq;        // listed particles' fields by phase index (Fields::wq)
+    DevBuf<float4> vq, wq, xq, wbq;   // listed particles' fields by phase index (Fields::wq)
     DevBuf<float4> rec0, rec1;        // water momentum neighbour records (Fields::rec0)
     DevBuf<uint8_t> wflag;
     DevBuf<uint2> ct;
@@ -152,6 +152,7 @@ struct sf_system {
         f.vq = vq.p;
         f.wq = wq.p;
         f.xq = xq.p;
+        f.wbq = wbq.p;
         f.rec0 = rec0.p;
         f.rec1 = rec1.p;
         f.mw = water_mass;
