@@ -366,6 +366,39 @@ SF_DEV void list_interface(const Fields& f, bool listed, bool sand, uint32_t t, 
 // rho_sand and p = 0.  Only water candidates are walked (sand adds nothing);
 // branch-free: max(h^2 - r^2, 0) vanishes outside the support.  The kDenL
 // lanes of a particle split both the candidates and the boundary particles.
+// The boundary particles of the 27 cells around (cx, cy, cz) for lane sub
+// of L lanes per particle: fn(b) for j = s + sub + L m of each column's
+// range [s, e), columns in for_columns order and j increasing (the order of
+// a plain walk).  The nine ranges, then each range's first candidate of the
+// lane, are loaded together: a lane near a wall has about one candidate per
+// column, so a column-by-column walk waited on three dependent loads per
+// column.
+template <int L, class Fn>
+SF_DEV void boundary_walk(const SFConst& k, const Fields& f, int cx, int cy, int cz, uint32_t sub, Fn&& fn) {
+    const int zlo = cz > 0 ? cz - 1 : 0;
+    const int zhi = cz + 1 < k.gz ? cz + 1 : k.gz - 1;
+    uint32_t s[9], e[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const int x = cx + q / 3 - 1, y = cy + q % 3 - 1;
+        s[q] = 0u;
+        e[q] = 0u;
+        if (x >= 0 && x < k.gx && y >= 0 && y < k.gy) {
+            const uint32_t base = (uint32_t)((x * k.gy + y) * k.gz);
+            s[q] = ldo(f.bcell, base + (uint32_t)zlo) + sub;
+            e[q] = ldo(f.bcell, base + (uint32_t)zhi + 1u);
+        }
+    }
+    float4 b0[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) b0[q] = s[q] < e[q] ? ldo(f.bpv, s[q]) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        if (s[q] < e[q]) fn(b0[q]);
+        for (uint32_t j = s[q] + (uint32_t)L; j < e[q]; j += (uint32_t)L) fn(ldo(f.bpv, j));
+    }
+}
+
 constexpr int kDenL = 2;  // lanes per water particle in the density pass
 template <bool B, int NB>
 __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
@@ -453,14 +486,25 @@ __global__ __launch_bounds__(NB) void k_density(SFConst k, Fields f, int n) {
     }
     if (water && ldo(f.bnear, cell_key(k, cx, cy, cz))) {  // boundary particles within the 27 cells
         const uint32_t sub = threadIdx.x % kDenL;
+        auto badd = [&](const float4& b) {
+            const float dx = pi.x - b.x, dy = pi.y - b.y, dz = pi.z - b.z;
+            const float d = fmaxf(k.h2 - (dx * dx + dy * dy + dz * dz), 0.f);
+            sb += fabsf(b.w) * sq3(d);
+        };
+        // column by column (this pass keeps its registers low), four
+        // candidates' loads in flight at a time, summed in increasing j
         for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
             const uint32_t be = ldo(f.bcell, c1);
-            for (uint32_t j = ldo(f.bcell, c0) + sub; j < be; j += kDenL) {
-                const float4 b = ldo(f.bpv, j);
-                const float dx = pi.x - b.x, dy = pi.y - b.y, dz = pi.z - b.z;
-                const float d = fmaxf(k.h2 - (dx * dx + dy * dy + dz * dz), 0.f);
-                sb += fabsf(b.w) * sq3(d);
+            uint32_t j = ldo(f.bcell, c0) + sub;
+            for (; j + 3u * kDenL < be; j += 4u * kDenL) {
+                const float4 b0 = ldo(f.bpv, j), b1 = ldo(f.bpv, j + kDenL), b2 = ldo(f.bpv, j + 2u * kDenL),
+                             b3 = ldo(f.bpv, j + 3u * kDenL);
+                badd(b0);
+                badd(b1);
+                badd(b2);
+                badd(b3);
             }
+            for (; j < be; j += kDenL) badd(ldo(f.bpv, j));
         });
     }
 #pragma unroll
@@ -1020,23 +1064,19 @@ __global__ __launch_bounds__(NB) void k_water_momentum(SFConst k, Fields f, int 
     nl_flush2(L, fetch, pair);
     if (!active) return;
     if (ldo(f.bnear, cell_key(k, cx, cy, cz))) {  // boundary particles within the 27 cells
-        for_columns(k, cx, cy, cz, [&](uint32_t c0, uint32_t c1) {
-            const uint32_t be = ldo(f.bcell, c1);
-            for (uint32_t j = ldo(f.bcell, c0) + sub; j < be; j += kWmL) {
-                const float4 b = ldo(f.bpv, j);
-                const v3 d = mk3(pi.x - b.x, pi.y - b.y, pi.z - b.z);
-                const float r2 = dot(d, d);
-                if (r2 >= k.h2) continue;
-                const float rs = frsq(r2);
-                const float r = r2 * rs;
-                const float hr = k.h - r;
-                const float gs = r >= kEps ? k.g_coef * hr * hr * rs : 0.f;
-                const float vb = fabsf(b.w);
-                float s = vb * bpress;  // ComputeBoundaryPressure (densityi = rho0, :944)
-                const float dd = dot(vi, d);
-                if (dd < 0.f) s += -vb * k.rho0 * (bvisc * (dd * frcp(r2 + kEps)));  // ComputeBoundaryViscosity
-                a += d * (s * gs);
-            }
+        boundary_walk<kWmL>(k, f, cx, cy, cz, sub, [&](const float4& b) {
+            const v3 d = mk3(pi.x - b.x, pi.y - b.y, pi.z - b.z);
+            const float r2 = dot(d, d);
+            if (r2 >= k.h2) return;
+            const float rs = frsq(r2);
+            const float r = r2 * rs;
+            const float hr = k.h - r;
+            const float gs = r >= kEps ? k.g_coef * hr * hr * rs : 0.f;
+            const float vb = fabsf(b.w);
+            float s = vb * bpress;  // ComputeBoundaryPressure (densityi = rho0, :944)
+            const float dd = dot(vi, d);
+            if (dd < 0.f) s += -vb * k.rho0 * (bvisc * (dd * frcp(r2 + kEps)));  // ComputeBoundaryViscosity
+            a += d * (s * gs);
         });
     }
 #pragma unroll
